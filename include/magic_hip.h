@@ -1,0 +1,116 @@
+/*
+ * magic_hip.h -- C ABI of the MI355X-native MAGIC wave-propagation step.
+ *
+ * The reference drives its hot path through Fortran operator calls from the
+ * time integrator claw3 (mpilib/claw3_mpi.f:765-905):
+ *     bc3 -> b4step3 -> copyq3 -> step3 -> MPI_ALLREDUCE(cfl) -> src3
+ * This library keeps q and aux resident on the GPU and exposes each of those
+ * operators, plus the claw3 step-control loop itself, as plain C entry
+ * points (no torch or HIP types in the signatures).  A Fortran host binds
+ * them with iso_c_binding (magic_master_amd/fortran/magic_hip.f90); Python
+ * binds them with ctypes (magic_master_amd/hip.py).
+ *
+ * Host arrays use the reference's own layouts (column-major Fortran):
+ *     q  (meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc, 1-mbc:mz+mbc)   REAL*8
+ *     aux(maux, 1-mbc:mx+mbc, 1-mbc:my+mbc, 1-mbc:mz+mbc)   REAL*8
+ * (mpilib/claw3ez_mpi.f:33-36).  Every entry point returns 0 on success or
+ * a MAGIC_E* status; the reference stops the process instead.
+ * Calls on one context are synchronous with respect to the host and are
+ * not thread-safe on the same context.
+ */
+#ifndef MAGIC_HIP_H
+#define MAGIC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MAGIC_OK            0
+#define MAGIC_ERHO          1   /* rho <= 0 in a Riemann problem   (rpn3euX.f:140-153) */
+#define MAGIC_EA2           2   /* a**2 <= 0 in a Roe average      (rpn3euX.f:176-185) */
+#define MAGIC_EBC           3   /* mthbc 0 or 4 on one GPU          (bc3_mpi.f:100-105) */
+#define MAGIC_EUNSUPPORTED  4   /* method / option not implemented  */
+#define MAGIC_EARG          5   /* invalid argument                 */
+#define MAGIC_EHIP          6   /* HIP runtime error                */
+#define MAGIC_ENOMEM        7   /* device allocation failed         */
+
+typedef struct magic_ctx magic_ctx;
+
+/* Problem parameters read by setprob (setprob.f:31-61), in that order. */
+typedef struct {
+    int loadprof, nspecadv;
+    double gamma, grav, dens, pres, dmolec0, dtherm0;
+    double forcemth, omega, amplitude, propx, propz;
+    double xpos, ypos, zpos, xwidth, ywidth, zwidth, tcenter, twidth, vsrcx;
+    double amp0, amp1, omega1, prop1z, phi, z1width, z1pos, z1Gpos, windscale1;
+} magic_params;
+
+/* Create a context for one grid on the current HIP device.
+ * Replaces the automatic arrays of claw3ez_mpi (mpilib/claw3ez_mpi.f:33-37)
+ * and the work partition of claw3 (mpilib/claw3_mpi.f:704-728).
+ * Requires mbc >= 2, meqn in 5..10, maux >= 19. */
+int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux,
+                 double xlower, double ylower, double zlower,
+                 double dx, double dy, double dz, magic_ctx **out);
+int magic_destroy(magic_ctx *ctx);
+const char *magic_strerror(int status);
+
+/* setprob commons (setprob.f:17-25); nspecadv selects advected tracers in
+ * rpn3 (rpn3euX.f:262-268), the rest drive src3 forcing. */
+int magic_set_params(magic_ctx *ctx, const magic_params *p);
+
+/* Upload the background aux (setaux output, maux slots) from host memory.
+ * Only the slots the step reads are kept on the device (1-5, 11, 12, 17,
+ * 18, 19).  Also precomputes the z-boundary density ratios of bc3
+ * ((rho_bar_ghost/rho_bar_1)**0.5, bc3_mpi.f:560-573) in FP64. */
+int magic_set_aux(magic_ctx *ctx, const double *aux);
+/* Upload / download q (host, reference layout, ghost cells included). */
+int magic_set_q(magic_ctx *ctx, const double *q);
+int magic_get_q(magic_ctx *ctx, double *q);
+
+/* bc3 (bc3_mpi.f:5-812): fill the ghost layers of q on the device.
+ * mthbc(k) = 1 extrapolation, 2 periodic, 3 wall, < 0 skip the face. */
+int magic_bc3(magic_ctx *ctx, const int mthbc[6]);
+/* b4step3 (b4step3.f:35-45): clamp tracers 6..meqn to [0,1] on all cells. */
+int magic_b4step3(magic_ctx *ctx);
+/* copyq3 (lib/copyq3.f:4-29) as a device copy q -> backup (claw3_mpi.f:807)
+ * and backup -> q (claw3_mpi.f:869). */
+int magic_backup(magic_ctx *ctx);
+int magic_restore(magic_ctx *ctx);
+/* step3 (lib/step3.f:3-626, with flux3fw.f, rpn3euX.f, rpt3euX.f,
+ * rptt3euX.f): q += unsplit x/y/z f-wave increments computed from the
+ * backup copy (qold); *cfl = max Courant number (step3.f:182/350/520). */
+int magic_step3(magic_ctx *ctx, double dt, double *cfl);
+/* src3 (src3.f90:203-634) with diffuse.f and conduct.f, forcemth 0/1/3,
+ * evaluated at time t over dt; returns the substep counts. */
+int magic_src3(magic_ctx *ctx, double t, double dt, const int mthbc[6],
+               int *ndtr, int *ndtrc);
+
+/* claw3 (mpilib/claw3_mpi.f:631-951) for method(1)=1, method(2)=2,
+ * method(3)=22, method(5)=1, method(6)=0: the step-control loop with CFL
+ * rejection/retry, on device-resident q.  Arguments mean what they mean
+ * in the reference (tend < tstart takes one accepted step).  If log_dt /
+ * log_cfl / log_acc are non-NULL, every attempted step is recorded (up to
+ * maxlog entries; *nlog returns the count). */
+int magic_claw3(magic_ctx *ctx, double tstart, double *tend, double dtv[5],
+                double cflv[4], int nv[2], const int method[7], const int mthbc[6],
+                double *log_dt, double *log_cfl, int *log_acc, int maxlog, int *nlog);
+
+/* Where the last MAGIC_ERHO / MAGIC_EA2 happened: sweep (1,2,3), the
+ * Fortran indices of the pencil's cell, as the reference prints them
+ * (rpn3euX.f:145-150). */
+int magic_error_location(magic_ctx *ctx, int *sweep, int *i, int *j, int *k);
+
+/* Per-kernel-class timing with HIP events on the library's stream
+ * (0 = off).  Classes: 0 sweep x, 1 sweep y, 2 sweep z, 3 bc3, 4 src3,
+ * 5 other.  magic_timing_read returns milliseconds and launch counts
+ * accumulated since the last reset. */
+int magic_timing_enable(magic_ctx *ctx, int on);
+int magic_timing_read(magic_ctx *ctx, double ms[6], long long launches[6]);
+int magic_timing_reset(magic_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAGIC_HIP_H */

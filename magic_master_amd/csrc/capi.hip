@@ -1,0 +1,682 @@
+// C ABI of the MAGIC step on MI355X (declared in include/magic_hip.h).
+//
+// A context owns the device-resident state of one grid: q and its backup
+// (the reference's copyq3 work array), the background planes the step reads,
+// the bc3 z-boundary density ratios, and the source-pass scratch planes.
+// Everything runs on one HIP stream of default priority; every entry point
+// returns after its work is complete.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/magic_hip.h"
+#include "bc_src.h"
+#include "magic_dev.h"
+
+using namespace magic;
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            fprintf(stderr, "magic_hip: %s failed: %s (%s:%d)\n", #x,                 \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                       \
+            return MAGIC_EHIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define TRY(x)                        \
+    do {                              \
+        int s_ = (x);                 \
+        if (s_ != MAGIC_OK) return s_; \
+    } while (0)
+
+enum { AX_RHO = 0, AX_MX, AX_MY, AX_MZ, AX_E, AX_NU, AX_KAP, AX_VG, AX_R, AX_GAM, NAUXP };
+enum { TC_SWX = 0, TC_SWY, TC_SWZ, TC_BC, TC_SRC, TC_OTHER, NTC };
+
+struct TimedSpan {
+    int cls;
+    hipEvent_t a, b;
+    int nlaunch;
+};
+
+struct magic_ctx {
+    int mx, my, mz, meqn, mbc, maux;
+    double xlower, ylower, zlower, dx, dy, dz;
+    Dims d;
+    long long P;
+    hipStream_t stream = nullptr;
+    double *q = nullptr, *qold = nullptr, *aux = nullptr, *zratio = nullptr;
+    double *stage = nullptr, *u = nullptr, *qt = nullptr, *ut = nullptr;
+    unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
+    unsigned long long *hscal = nullptr;   // pinned mirror
+    magic_params prm;
+    bool have_params = false, have_aux = false;
+    double aux11_top = 0.0, aux12_top = 0.0;
+    int errloc[4] = {0, 0, 0, 0};
+    bool timing = false;
+    double ms[NTC] = {0};
+    long long launches[NTC] = {0};
+    std::vector<TimedSpan> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+// ------------------------------------------------------------ timing -----
+static hipEvent_t ev_get(magic_ctx *c)
+{
+    hipEvent_t e;
+    if (!c->pool.empty()) {
+        e = c->pool.back();
+        c->pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    return e;
+}
+
+struct Span {
+    magic_ctx *c;
+    int cls, n;
+    hipEvent_t a = nullptr;
+    Span(magic_ctx *c_, int cls_, int n_) : c(c_), cls(cls_), n(n_)
+    {
+        if (c->timing && (a = ev_get(c))) hipEventRecord(a, c->stream);
+    }
+    ~Span()
+    {
+        if (!c->timing) {
+            return;
+        }
+        hipEvent_t b = ev_get(c);
+        if (a && b) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({cls, a, b, n});
+        }
+    }
+};
+
+static void drain_timing(magic_ctx *c)
+{
+    for (auto &s : c->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(s.b) == hipSuccess && hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            c->ms[s.cls] += ms;
+            c->launches[s.cls] += s.nlaunch;
+        }
+        c->pool.push_back(s.a);
+        c->pool.push_back(s.b);
+    }
+    c->pending.clear();
+}
+
+// ---------------------------------------------------------- helpers ------
+static inline unsigned blocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static int launch_check() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "magic_hip: kernel launch failed: %s\n", hipGetErrorString(e));
+        return MAGIC_EHIP;
+    }
+    return MAGIC_OK;
+}
+
+static int ntracer(const magic_ctx *c) { return c->meqn - 5; }
+
+static bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
+
+extern "C" const char *magic_strerror(int s)
+{
+    switch (s) {
+    case MAGIC_OK: return "ok";
+    case MAGIC_ERHO: return "rho <= 0 in a Riemann problem";
+    case MAGIC_EA2: return "a**2 <= 0 in a Roe average";
+    case MAGIC_EBC: return "boundary condition needs MPI or a user routine (mthbc 0 or 4)";
+    case MAGIC_EUNSUPPORTED: return "method or option not supported";
+    case MAGIC_EARG: return "invalid argument";
+    case MAGIC_EHIP: return "HIP runtime error";
+    case MAGIC_ENOMEM: return "device allocation failed";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux, double xlower,
+                            double ylower, double zlower, double dx, double dy, double dz,
+                            magic_ctx **out)
+{
+    if (!out) return MAGIC_EARG;
+    *out = nullptr;
+    if (mx < 1 || my < 1 || mz < 1 || mbc < 2 || meqn < 5 || meqn > 10 || maux < 19) return MAGIC_EARG;
+    if (!sweep_supported(meqn - 5)) return MAGIC_EUNSUPPORTED;
+    const int nmax = mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
+    if (nmax + 3 > 512) return MAGIC_EUNSUPPORTED;   // one workgroup per pencil
+    magic_ctx *c = new magic_ctx();
+    c->mx = mx; c->my = my; c->mz = mz; c->meqn = meqn; c->mbc = mbc; c->maux = maux;
+    c->xlower = xlower; c->ylower = ylower; c->zlower = zlower; c->dx = dx; c->dy = dy; c->dz = dz;
+    c->d.mx = mx; c->d.my = my; c->d.mz = mz; c->d.mbc = mbc;
+    c->d.NX = mx + 2 * mbc; c->d.NY = my + 2 * mbc; c->d.NZ = mz + 2 * mbc;
+    c->P = c->d.P = (long long)c->d.NX * c->d.NY * c->d.NZ;
+    const size_t P = (size_t)c->P;
+    const size_t NXY = (size_t)c->d.NX * c->d.NY;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc(&c->q, P * meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->qold, P * meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->aux, P * NAUXP * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->zratio, 2 * mbc * NXY * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->stage, P * meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->u, 3 * P * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->qt, P * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->ut, P * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->dscal, 8 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipHostMalloc(&c->hscal, 8 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    if (ok) {
+        ok = hipMemsetAsync(c->q, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->qold, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->aux, 0, P * NAUXP * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->zratio, 0, 2 * mbc * NXY * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->u, 0, 3 * P * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->qt, 0, P * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->ut, 0, P * sizeof(double), c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    if (!ok) {
+        magic_destroy(c);
+        return MAGIC_ENOMEM;
+    }
+    *out = c;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_destroy(magic_ctx *c)
+{
+    if (!c) return MAGIC_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drain_timing(c);
+    for (auto e : c->pool) hipEventDestroy(e);
+    hipFree(c->q); hipFree(c->qold); hipFree(c->aux); hipFree(c->zratio); hipFree(c->stage);
+    hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal);
+    if (c->hscal) hipHostFree(c->hscal);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_set_params(magic_ctx *c, const magic_params *p)
+{
+    if (!c || !p) return MAGIC_EARG;
+    c->prm = *p;
+    c->have_params = true;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
+{
+    if (!c || !aux) return MAGIC_EARG;
+    static const int slot[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
+    const long long P = c->P;
+    const int maux = c->maux;
+    std::vector<double> h((size_t)P * NAUXP);
+    for (int s = 0; s < NAUXP; s++) {
+        double *dst = h.data() + (size_t)s * P;
+        const double *src = aux + (slot[s] - 1);
+        for (long long k = 0; k < P; k++) dst[k] = src[k * maux];
+    }
+    HIPCHECK(hipMemcpyAsync(c->aux, h.data(), (size_t)P * NAUXP * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    // bc3 z-face scale factors (aux(1,ghost)/aux(1,boundary))**0.5 in FP64
+    // (bc3_mpi.f:560-573, 676-689), evaluated once on the host.
+    const int NX = c->d.NX, NY = c->d.NY, mbc = c->mbc, mz = c->mz;
+    const long long NXY = (long long)NX * NY;
+    std::vector<double> zr((size_t)2 * mbc * NXY);
+    const double *rho = h.data() + (size_t)AX_RHO * P;
+    for (int side = 0; side < 2; side++)
+        for (int kbc = 1; kbc <= mbc; kbc++)
+            for (long long col = 0; col < NXY; col++) {
+                const int kg = side == 0 ? 1 - kbc : mz + kbc;
+                const int kb = side == 0 ? 1 : mz;
+                const double g = rho[(long long)(kg + mbc - 1) * NXY + col];
+                const double b = rho[(long long)(kb + mbc - 1) * NXY + col];
+                zr[((size_t)side * mbc + (kbc - 1)) * NXY + col] = std::pow(g / b, 0.5);
+            }
+    HIPCHECK(hipMemcpyAsync(c->zratio, zr.data(), zr.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    // substep counts read the top cell of column (1,1): src3.f90:315, 319
+    const long long c11 = ((long long)(mz + mbc - 1) * NY + (1 + mbc - 1)) * NX + (1 + mbc - 1);
+    c->aux11_top = aux[c11 * maux + 10];
+    c->aux12_top = aux[c11 * maux + 11];
+    c->have_aux = true;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_set_q(magic_ctx *c, const double *q)
+{
+    if (!c || !q) return MAGIC_EARG;
+    const size_t n = (size_t)c->P * c->meqn;
+    HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_aos_to_planes<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->stage, c->P, c->meqn);
+    TRY(launch_check());
+    HIPCHECK(hipMemcpyAsync(c->qold, c->q, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+extern "C" int magic_get_q(magic_ctx *c, double *q)
+{
+    if (!c || !q) return MAGIC_EARG;
+    const size_t n = (size_t)c->P * c->meqn;
+    k_planes_to_aos<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->stage, c->q, c->P, c->meqn);
+    TRY(launch_check());
+    HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+// ------------------------------------------------------------- bc3 -------
+static int bc3_launch(magic_ctx *c, const int *mthbc)
+{
+    for (int f = 0; f < 6; f++)
+        if (mthbc[f] == 0 || mthbc[f] >= 4) return MAGIC_EBC;
+    Span sp(c, TC_BC, 3);
+    const Dims &d = c->d;
+    const double *A = c->aux;
+    auto lohi = [](int m) { return m < 0 ? 0 : m; };
+    if (mthbc[0] >= 0 || mthbc[1] >= 0) {
+        k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, c->stream>>>(d, c->q, c->meqn, lohi(mthbc[0]),
+                                                                 lohi(mthbc[1]));
+        TRY(launch_check());
+    }
+    if (mthbc[2] >= 0 || mthbc[3] >= 0) {
+        k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, c->stream>>>(d, c->q, c->meqn, lohi(mthbc[2]),
+                                                                 lohi(mthbc[3]));
+        TRY(launch_check());
+    }
+    if (mthbc[4] >= 0 || mthbc[5] >= 0) {
+        k_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, c->stream>>>(
+            d, c->q, c->meqn, lohi(mthbc[4]), lohi(mthbc[5]), A + AX_RHO * c->P, A + AX_MX * c->P,
+            A + AX_MY * c->P, A + AX_E * c->P, c->zratio);
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_bc3(magic_ctx *c, const int mthbc[6])
+{
+    if (!c || !mthbc) return MAGIC_EARG;
+    if (!c->have_aux) return MAGIC_EARG;
+    TRY(bc3_launch(c, mthbc));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+static int b4step3_launch(magic_ctx *c)
+{
+    if (c->meqn > 5) {
+        Span sp(c, TC_OTHER, 1);
+        k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->P, c->meqn);
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_b4step3(magic_ctx *c)
+{
+    if (!c) return MAGIC_EARG;
+    TRY(b4step3_launch(c));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+static int copy_q(magic_ctx *c, double *dst, const double *src)
+{
+    Span sp(c, TC_OTHER, 1);
+    HIPCHECK(hipMemcpyAsync(dst, src, (size_t)c->P * c->meqn * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+    return MAGIC_OK;
+}
+
+extern "C" int magic_backup(magic_ctx *c)
+{
+    if (!c) return MAGIC_EARG;
+    TRY(copy_q(c, c->qold, c->q));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+extern "C" int magic_restore(magic_ctx *c)
+{
+    if (!c) return MAGIC_EARG;
+    TRY(copy_q(c, c->q, c->qold));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+// ------------------------------------------------------------ step3 ------
+__global__ void k_zero_scalars(unsigned long long *s)
+{
+    if (threadIdx.x < 8) s[threadIdx.x] = 0ull;
+}
+
+template <int NTR>
+static int launch_sweep(magic_ctx *c, SweepArgs a)
+{
+    const int NI = a.n + 3;
+    const int threads = ((NI + 63) / 64) * 64;
+    const size_t smem = (size_t)(7 + NTR) * NI * sizeof(double) + (size_t)NI * sizeof(int);
+    for (int ca = 0; ca < 3; ca++)
+        for (int cb = 0; cb < 3; cb++) {
+            a.cA = ca;
+            a.cB = cb;
+            dim3 grid((unsigned)((a.nA + 1 - ca) / 3 + 1), (unsigned)((a.nB + 1 - cb) / 3 + 1));
+            sweep_kernel<NTR><<<grid, threads, smem, c->stream>>>(a);
+            TRY(launch_check());
+        }
+    return MAGIC_OK;
+}
+
+static int sweep_dispatch(magic_ctx *c, const SweepArgs &a)
+{
+    switch (ntracer(c)) {
+    case 0: return launch_sweep<0>(c, a);
+    case 1: return launch_sweep<1>(c, a);
+    case 2: return launch_sweep<2>(c, a);
+    case 5: return launch_sweep<5>(c, a);
+    default: return MAGIC_EUNSUPPORTED;
+    }
+}
+
+static int step3_launch(magic_ctx *c, double dt)
+{
+    const Dims &d = c->d;
+    const long long NXY = (long long)d.NX * d.NY;
+    k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);
+    TRY(launch_check());
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.qold = c->qold;
+    a.qnew = c->q;
+    a.gam = c->aux + AX_GAM * c->P;
+    a.vgrav = c->aux + AX_VG * c->P;
+    a.P = c->P;
+    a.mbc = c->mbc;
+    a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
+    a.cfl_bits = c->dscal;
+    a.status = (int *)(c->dscal + 1);
+    for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+    const double dtdx = dt / c->dx, dtdy = dt / c->dy, dtdz = dt / c->dz;
+    {   // x-sweeps: lib/step3.f:121-280 (g = y, h = z)
+        Span sp(c, TC_SWX, 9);
+        a.comp[0] = 0; a.comp[1] = 1; a.comp[2] = 2; a.comp[3] = 3; a.comp[4] = 4;
+        a.n = c->mx; a.nA = c->my; a.nB = c->mz;
+        a.sN = 1; a.sA = d.NX; a.sB = NXY;
+        a.gravity = 0; a.sweep = 1;
+        a.dtdn = dtdx; a.fg = dtdy; a.fh = dtdz;
+        TRY(sweep_dispatch(c, a));
+    }
+    {   // y-sweeps: lib/step3.f:289-450 (g = z, h = x)
+        Span sp(c, TC_SWY, 9);
+        a.comp[0] = 0; a.comp[1] = 2; a.comp[2] = 3; a.comp[3] = 1; a.comp[4] = 4;
+        a.n = c->my; a.nA = c->mz; a.nB = c->mx;
+        a.sN = d.NX; a.sA = NXY; a.sB = 1;
+        a.gravity = 0; a.sweep = 2;
+        a.dtdn = dtdy; a.fg = dtdz; a.fh = dtdx;
+        TRY(sweep_dispatch(c, a));
+    }
+    {   // z-sweeps: lib/step3.f:459-621 (g = x, h = y), gravity on
+        Span sp(c, TC_SWZ, 9);
+        a.comp[0] = 0; a.comp[1] = 3; a.comp[2] = 1; a.comp[3] = 2; a.comp[4] = 4;
+        a.n = c->mz; a.nA = c->mx; a.nB = c->my;
+        a.sN = NXY; a.sA = 1; a.sB = d.NX;
+        a.gravity = 1; a.sweep = 3;
+        a.dtdn = dtdz; a.fg = dtdx; a.fh = dtdy;
+        TRY(sweep_dispatch(c, a));
+    }
+    HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            c->stream));
+    return MAGIC_OK;
+}
+
+static int step3_finish(magic_ctx *c, double *cfl)
+{
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    double v;
+    memcpy(&v, &c->hscal[0], sizeof(double));
+    *cfl = v;
+    const int *st = (const int *)(c->hscal + 1);
+    if (st[0] != 0) {
+        for (int k = 0; k < 4; k++) c->errloc[k] = st[1 + k];
+        return st[0] == 1 ? MAGIC_ERHO : MAGIC_EA2;
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_step3(magic_ctx *c, double dt, double *cfl)
+{
+    if (!c || !cfl || !c->have_aux) return MAGIC_EARG;
+    TRY(step3_launch(c, dt));
+    int s = step3_finish(c, cfl);
+    if (c->timing) drain_timing(c);
+    return s;
+}
+
+// ------------------------------------------------------------- src3 ------
+static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, int *ndtr_out,
+                       int *ndtrc_out)
+{
+    const magic_params &p = c->prm;
+    if (p.omega1 != 0.0) return MAGIC_EUNSUPPORTED;   // time-dependent wind (src3.f90:389-399)
+    const int fm = (int)p.forcemth;
+    if (!(p.forcemth == 0.0 || p.forcemth == 1.0 || p.forcemth == 3.0)) return MAGIC_EUNSUPPORTED;
+    const Dims &d = c->d;
+    const long long P = c->P, NXY = (long long)d.NX * d.NY;
+    const double *A = c->aux;
+    Aux ax = {A + AX_RHO * P, A + AX_MX * P, A + AX_MY * P, A + AX_MZ * P, A + AX_E * P, A + AX_R * P,
+              A + AX_GAM * P};
+    // substep counts: src3.f90:314-322
+    double dmin = c->dx < c->dy ? c->dx : c->dy;
+    dmin = dmin < c->dz ? dmin : c->dz;
+    const double dxyzmin2 = dmin * dmin;
+    const double difmax = (4.0 / 3.0) * c->aux11_top;
+    const int ndtr = (int)std::ceil((dt * difmax / (dxyzmin2)) / 0.45);
+    const double dtr = dt / ndtr;
+    const int ndtrc = (int)std::ceil((dtr * c->aux12_top / (dxyzmin2)) / 0.45);
+    const double dtrc = dtr / ndtrc;
+    *ndtr_out = ndtr;
+    *ndtrc_out = ndtrc;
+    int mthbc[6];
+    memcpy(mthbc, mthbc_in, sizeof(mthbc));
+    mthbc[4] = -1;
+    mthbc[5] = -1;
+    const dim3 gcell(blocks(P, 256));
+    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gface(blocks(d.NX, 64), d.NY);
+    const double *nu = A + AX_NU * P, *kap = A + AX_KAP * P;
+    {
+        Span sp(c, TC_SRC, 1);
+        k_src_convert<<<gcell, 256, 0, c->stream>>>(P, c->q, ax);
+        TRY(launch_check());
+    }
+    for (int nn = 1; nn <= ndtr; nn++) {
+        TRY(bc3_launch(c, mthbc));
+        Span sp(c, TC_SRC, 34 + 7 * (ndtrc > 0 ? ndtrc : 0));
+        // diffuse: diffuse.f:37-210
+        const double fourthirds = 4.0 / 3.0;
+        double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
+        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q);
+        TRY(launch_check());
+        HIPCHECK(hipMemcpyAsync(c->u, c->q + P, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        const double dtr32 = dtr / 6.0;
+        k_diff_cross<<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
+                                                 dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz));
+        TRY(launch_check());
+        k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->u, c->q + P, 3);
+        TRY(launch_check());
+        for (int cc = 1; cc <= 3; cc++) {
+            if (cc == 1) {
+                dtdx2 = fourthirds * dtdx2;
+            } else if (cc == 2) {
+                dtdy2 = fourthirds * dtdy2;
+                dtdx2 = dtdx2 / fourthirds;
+            } else {
+                dtdz2 = fourthirds * dtdz2;
+                dtdy2 = dtdy2 / fourthirds;
+            }
+            double *uc = c->u + (cc - 1) * P;
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, 1);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, uc, uc, nu, c->qt, dtdx2);
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, d.NX);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, uc, uc, nu, c->qt, dtdy2);
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, -NXY);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, c->q + cc * P, uc, nu, c->qt, dtdz2);
+            TRY(launch_check());
+        }
+        // conduct: conduct.f:245-332
+        const double cx = (dtrc) / (c->dx * c->dx), cy = (dtrc) / (c->dy * c->dy), cz = (dtrc) / (c->dz * c->dz);
+        HIPCHECK(hipMemcpyAsync(c->ut, c->q + 4 * P, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (int k = 1; k <= ndtrc; k++) {
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, 1);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cx);
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, d.NX);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cy);
+            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut);
+            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, -NXY);
+            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cz);
+            TRY(launch_check());
+        }
+        k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->q + 4 * P, c->ut, 1);
+        TRY(launch_check());
+    }
+    {
+        Span sp(c, TC_SRC, 1);
+        Forcing f;
+        f.forcemth = fm;
+        f.t = t; f.dt = dt;
+        f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
+        f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+        f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
+        f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
+        f.tcenter = p.tcenter; f.twidth = p.twidth;
+        f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
+        f.tgauss = std::exp(-.5 * ((t - p.tcenter) * (t - p.tcenter) / (p.twidth * p.twidth)));
+        k_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q, c->meqn, ax, f);
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_src3(magic_ctx *c, double t, double dt, const int mthbc[6], int *ndtr, int *ndtrc)
+{
+    if (!c || !mthbc || !ndtr || !ndtrc || !c->have_aux || !c->have_params) return MAGIC_EARG;
+    TRY(src3_launch(c, t, dt, mthbc, ndtr, ndtrc));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->timing) drain_timing(c);
+    return MAGIC_OK;
+}
+
+// ------------------------------------------------------------ claw3 ------
+extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv[5], double cflv[4],
+                           int nv[2], const int method[7], const int mthbc[6], double *log_dt,
+                           double *log_cfl, int *log_acc, int maxlog, int *nlog)
+{
+    if (!c || !tend || !dtv || !cflv || !nv || !method || !mthbc) return MAGIC_EARG;
+    if (!c->have_aux || !c->have_params) return MAGIC_EARG;
+    if (method[0] != 1 || method[1] != 2 || method[2] != 22 || method[4] != 1 || method[5] != 0)
+        return MAGIC_EUNSUPPORTED;
+    if (cflv[1] > cflv[0]) return MAGIC_EARG;          // claw3_mpi.f:671-675
+    double t = tstart, dt = dtv[0], cflmax = 0.0, dtmin = dt, dtmax = dt, cfl = 0.0;
+    const int maxn = nv[0];
+    int st = MAGIC_OK, nl = 0;
+    nv[1] = 0;
+    for (int n = 1; n <= maxn; n++) {
+        const double told = t;
+        if (told + dt > *tend && tstart < *tend) dt = *tend - told;   // claw3_mpi.f:741
+        for (;;) {
+            if ((st = bc3_launch(c, mthbc))) goto out;
+            if ((st = b4step3_launch(c))) goto out;
+            if ((st = copy_q(c, c->qold, c->q))) goto out;
+            if ((st = step3_launch(c, dt))) goto out;
+            if ((st = step3_finish(c, &cfl))) goto out;
+            t = told + dt;
+            if (log_dt && nl < maxlog) {
+                log_dt[nl] = dt;
+                if (log_cfl) log_cfl[nl] = cfl;
+                if (log_acc) log_acc[nl] = cfl <= cflv[0];
+            }
+            nl++;
+            if (cfl <= cflv[0]) {                                  // claw3_mpi.f:861-886
+                cflmax = cflmax > cfl ? cflmax : cfl;
+                break;
+            }
+            t = told;
+            if ((st = copy_q(c, c->q, c->qold))) goto out;
+            dt = dtv[1] < dt * cflv[1] / cfl ? dtv[1] : dt * cflv[1] / cfl;
+        }
+        {
+            int a, b;
+            if ((st = src3_launch(c, told, dt, mthbc, &a, &b))) goto out;   // claw3_mpi.f:900-905
+        }
+        if (cfl == 0.0) dt = dtv[1];                               // claw3_mpi.f:910-919
+        else dt = dtv[1] < dt * cflv[1] / cfl ? dtv[1] : dt * cflv[1] / cfl;
+        dtmin = dt < dtmin ? dt : dtmin;
+        dtmax = dt > dtmax ? dt : dtmax;
+        nv[1] = nv[1] + 1;
+        if (t >= *tend) break;
+    }
+out:
+    hipStreamSynchronize(c->stream);
+    if (c->timing) drain_timing(c);
+    if (nlog) *nlog = nl < maxlog ? nl : maxlog;
+    *tend = t;
+    cflv[2] = cflmax;
+    cflv[3] = cfl;
+    dtv[2] = dtmin;
+    dtv[3] = dtmax;
+    dtv[4] = dt;
+    return st;
+}
+
+extern "C" int magic_error_location(magic_ctx *c, int *sweep, int *i, int *j, int *k)
+{
+    if (!c) return MAGIC_EARG;
+    if (sweep) *sweep = c->errloc[0];
+    if (i) *i = c->errloc[1];
+    if (j) *j = c->errloc[2];
+    if (k) *k = c->errloc[3];
+    return MAGIC_OK;
+}
+
+extern "C" int magic_timing_enable(magic_ctx *c, int on)
+{
+    if (!c) return MAGIC_EARG;
+    hipStreamSynchronize(c->stream);
+    drain_timing(c);
+    c->timing = on != 0;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_timing_read(magic_ctx *c, double ms[6], long long launches[6])
+{
+    if (!c) return MAGIC_EARG;
+    hipStreamSynchronize(c->stream);
+    drain_timing(c);
+    for (int k = 0; k < NTC; k++) {
+        if (ms) ms[k] = c->ms[k];
+        if (launches) launches[k] = c->launches[k];
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_timing_reset(magic_ctx *c)
+{
+    if (!c) return MAGIC_EARG;
+    hipStreamSynchronize(c->stream);
+    drain_timing(c);
+    for (int k = 0; k < NTC; k++) {
+        c->ms[k] = 0.0;
+        c->launches[k] = 0;
+    }
+    return MAGIC_OK;
+}

@@ -1,0 +1,37 @@
+// Shared declarations of the MAGIC device kernels (FP64 form).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace magic {
+
+// One directional sweep of step3 (lib/step3.f:121-621): pencils along the
+// normal direction n, with the "g" transverse direction (gadd, flux3fw.f:419)
+// and the "h" transverse direction (hadd, flux3fw.f:545).
+//   x-sweep: n = x, g = y, h = z      y-sweep: n = y, g = z, h = x
+//   z-sweep: n = z, g = x, h = y      (step3.f:161, 329, 499 argument order)
+struct SweepArgs {
+    const double *__restrict__ qold;   // component planes, stride P
+    double *__restrict__ qnew;
+    const double *__restrict__ gam;    // aux(19) plane
+    const double *__restrict__ vgrav;  // aux(17) plane
+    long long P;                       // cells per plane
+    int comp[10];                      // local component -> plane (rho, m_n, m_g', m_h', E, tracers)
+    int n;                             // interior cells along the pencil
+    int nA, nB;                        // interior extents along g and h
+    int mbc;
+    long long sN, sA, sB;              // strides (cells) along n, g, h
+    int cA, cB;                        // pencil colour (a mod 3, b mod 3)
+    int gravity;                       // 1 on the z-sweep (rpn3euX.f:202-208)
+    int nspecadv;
+    int sweep;                         // 1, 2, 3 (for error reports)
+    double dtdn, fg, fh;               // dt/dn, dt/dg, dt/dh
+    unsigned long long *cfl_bits;      // running max of the Courant number
+    int *status;                       // [0] code, [1..4] sweep,i,j,k of first failure
+    int i0, j0, k0;                    // Fortran index of (cell 0 along n, a = 0, b = 0) mapping helpers
+};
+
+template <int NTR>
+__global__ void sweep_kernel(SweepArgs a);
+
+}  // namespace magic

@@ -1,0 +1,240 @@
+"""Python binding of the MI355X MAGIC step (libmagic_hip.so, C ABI in
+include/magic_hip.h), mirroring the reference's operator interface:
+
+    bc3, b4step3, copyq3 (backup/restore), step3, src3 and the claw3
+    step-control loop (mpilib/claw3_mpi.f:4-951), plus the outstyle-3 frame
+    loop of claw3ez (mpilib/claw3ez_mpi.f:136-188).
+
+q and aux cross the boundary in the reference's Fortran layouts (numpy
+arrays of shape (nz+2mbc, ny+2mbc, nx+2mbc, ncomp)); on the device they stay
+resident.  There is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .problem import ClawData, Grid, ProbData
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmagic_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+STATUS = {0: "ok", 1: "rho <= 0 in a Riemann problem", 2: "a**2 <= 0 in a Roe average",
+          3: "boundary condition needs MPI or a user routine", 4: "unsupported option",
+          5: "invalid argument", 6: "HIP runtime error", 7: "device allocation failed"}
+
+EXPORTS = ["magic_create", "magic_destroy", "magic_strerror", "magic_set_params",
+           "magic_set_aux", "magic_set_q", "magic_get_q", "magic_bc3", "magic_b4step3",
+           "magic_backup", "magic_restore", "magic_step3", "magic_src3", "magic_claw3",
+           "magic_error_location", "magic_timing_enable", "magic_timing_read",
+           "magic_timing_reset"]
+
+
+class MagicError(RuntimeError):
+    def __init__(self, code: int, where: str, loc=None):
+        msg = f"{where}: {STATUS.get(code, 'status %d' % code)}"
+        if loc is not None and code in (1, 2):
+            msg += " (sweep %d at i=%d j=%d k=%d)" % tuple(loc)
+        super().__init__(msg)
+        self.code = code
+        self.loc = loc
+
+
+class CParams(C.Structure):
+    _fields_ = [("loadprof", C.c_int), ("nspecadv", C.c_int)] + [
+        (n, C.c_double) for n in ("gamma", "grav", "dens", "pres", "dmolec0", "dtherm0",
+                                  "forcemth", "omega", "amplitude", "propx", "propz", "xpos",
+                                  "ypos", "zpos", "xwidth", "ywidth", "zwidth", "tcenter",
+                                  "twidth", "vsrcx", "amp0", "amp1", "omega1", "prop1z", "phi",
+                                  "z1width", "z1pos", "z1Gpos", "windscale1")]
+
+    @classmethod
+    def of(cls, p: ProbData):
+        o = cls()
+        for name, _ in cls._fields_:
+            setattr(o, name, getattr(p, name))
+        return o
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmagic_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.magic_create.argtypes = [C.c_int] * 6 + [C.c_double] * 6 + [C.POINTER(vp)]
+    L.magic_destroy.argtypes = [vp]
+    L.magic_strerror.restype = C.c_char_p
+    L.magic_strerror.argtypes = [C.c_int]
+    L.magic_set_params.argtypes = [vp, C.POINTER(CParams)]
+    L.magic_set_aux.argtypes = [vp, _dp]
+    L.magic_set_q.argtypes = [vp, _dp]
+    L.magic_get_q.argtypes = [vp, _dp]
+    L.magic_bc3.argtypes = [vp, _ip]
+    L.magic_b4step3.argtypes = [vp]
+    L.magic_backup.argtypes = [vp]
+    L.magic_restore.argtypes = [vp]
+    L.magic_step3.argtypes = [vp, C.c_double, _dp]
+    L.magic_src3.argtypes = [vp, C.c_double, C.c_double, _ip, _ip, _ip]
+    L.magic_claw3.argtypes = [vp, C.c_double, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip, C.c_int, _ip]
+    L.magic_error_location.argtypes = [vp, _ip, _ip, _ip, _ip]
+    L.magic_timing_enable.argtypes = [vp, C.c_int]
+    L.magic_timing_read.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
+    L.magic_timing_reset.argtypes = [vp]
+    _LIB = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("arrays must be C-contiguous float64 in the reference layout")
+    return a.ctypes.data_as(_dp)
+
+
+@dataclass
+class StepLog:
+    dt: List[float]
+    cfl: List[float]
+    accepted: List[bool]
+
+
+class MagicHIP:
+    """One grid on the GPU (a magic_ctx)."""
+
+    def __init__(self, g: Grid, p: ProbData, aux: np.ndarray, q: Optional[np.ndarray] = None):
+        self.L = load_library()
+        self.g = g
+        self.p = p
+        h = C.c_void_p()
+        self._check(self.L.magic_create(g.mx, g.my, g.mz, g.meqn, g.mbc, g.maux, g.xlower,
+                                        g.ylower, g.zlower, g.dx, g.dy, g.dz, C.byref(h)),
+                    "magic_create")
+        self.h = h
+        self._check(self.L.magic_set_params(self.h, C.byref(CParams.of(p))), "magic_set_params")
+        aux = np.ascontiguousarray(aux, dtype=np.float64)
+        if aux.shape != g.shape_aux:
+            raise ValueError(f"aux shape {aux.shape} != {g.shape_aux}")
+        self._check(self.L.magic_set_aux(self.h, _ptr(aux)), "magic_set_aux")
+        if q is not None:
+            self.set_q(q)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.magic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int, where: str):
+        if st != 0:
+            loc = None
+            if st in (1, 2) and getattr(self, "h", None):
+                a = [C.c_int() for _ in range(4)]
+                self.L.magic_error_location(self.h, *[C.byref(x) for x in a])
+                loc = [x.value for x in a]
+            raise MagicError(st, where, loc)
+
+    # -- state ------------------------------------------------------------
+    def set_q(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != self.g.shape_q:
+            raise ValueError(f"q shape {q.shape} != {self.g.shape_q}")
+        self._check(self.L.magic_set_q(self.h, _ptr(q)), "magic_set_q")
+
+    def get_q(self) -> np.ndarray:
+        out = np.empty(self.g.shape_q)
+        self._check(self.L.magic_get_q(self.h, _ptr(out)), "magic_get_q")
+        return out
+
+    # -- operators (reference signatures in include/magic_hip.h) ----------
+    def bc3(self, mthbc: Sequence[int]):
+        self._check(self.L.magic_bc3(self.h, (C.c_int * 6)(*mthbc)), "bc3")
+
+    def b4step3(self):
+        self._check(self.L.magic_b4step3(self.h), "b4step3")
+
+    def backup(self):
+        self._check(self.L.magic_backup(self.h), "copyq3")
+
+    def restore(self):
+        self._check(self.L.magic_restore(self.h), "copyq3")
+
+    def step3(self, dt: float) -> float:
+        cfl = C.c_double()
+        self._check(self.L.magic_step3(self.h, dt, C.byref(cfl)), "step3")
+        return cfl.value
+
+    def src3(self, t: float, dt: float, mthbc: Sequence[int]):
+        a, b = C.c_int(), C.c_int()
+        self._check(self.L.magic_src3(self.h, t, dt, (C.c_int * 6)(*mthbc), C.byref(a), C.byref(b)),
+                    "src3")
+        return a.value, b.value
+
+    def claw3(self, tstart: float, tend: float, dtv: Sequence[float], cflv: Sequence[float],
+              nv: Sequence[int], method: Sequence[int], mthbc: Sequence[int], maxlog: int = 256):
+        """One call of claw3; returns (tend, dtv, cflv, nv, StepLog)."""
+        te = C.c_double(tend)
+        dtv_a = (C.c_double * 5)(*dtv)
+        cflv_a = (C.c_double * 4)(*cflv)
+        nv_a = (C.c_int * 2)(*nv)
+        ldt = (C.c_double * maxlog)()
+        lcfl = (C.c_double * maxlog)()
+        lacc = (C.c_int * maxlog)()
+        nl = C.c_int()
+        st = self.L.magic_claw3(self.h, tstart, C.byref(te), dtv_a, cflv_a, nv_a,
+                                (C.c_int * 7)(*method), (C.c_int * 6)(*mthbc), ldt, lcfl, lacc,
+                                maxlog, C.byref(nl))
+        self._check(st, "claw3")
+        n = nl.value
+        return te.value, list(dtv_a), list(cflv_a), list(nv_a), StepLog(
+            list(ldt[:n]), list(lcfl[:n]), [bool(x) for x in lacc[:n]])
+
+    def run_steps(self, c: ClawData, nsteps: int, t0: float = 0.0, dt0: Optional[float] = None):
+        """claw3ez outstyle-3 frame loop (mpilib/claw3ez_mpi.f:136-153): one
+        accepted step per claw3 call, dtv(1) = dtv(5) between calls.
+        Returns (t, dtv, log of every attempted step)."""
+        dtv = list(c.dtv)
+        if dt0 is not None:
+            dtv[0] = dt0
+        cflv = list(c.cflv)
+        t = t0
+        log = StepLog([], [], [])
+        for _ in range(nsteps):
+            t, dtv, cflv, nv, lg = self.claw3(t, t - 1.0, dtv, cflv, [c.nv[0], 0], c.method, c.mthbc)
+            log.dt += lg.dt
+            log.cfl += lg.cfl
+            log.accepted += lg.accepted
+            dtv[0] = dtv[4]
+        return t, dtv, log
+
+    # -- timing -----------------------------------------------------------
+    def timing(self, on: bool = True):
+        self._check(self.L.magic_timing_enable(self.h, 1 if on else 0), "timing")
+
+    def timing_reset(self):
+        self._check(self.L.magic_timing_reset(self.h), "timing")
+
+    def timing_read(self):
+        ms = (C.c_double * 6)()
+        n = (C.c_longlong * 6)()
+        self._check(self.L.magic_timing_read(self.h, ms, n), "timing")
+        names = ["sweep_x", "sweep_y", "sweep_z", "bc3", "src3", "other"]
+        return {k: (ms[i], n[i]) for i, k in enumerate(names)}

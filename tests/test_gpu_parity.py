@@ -1,0 +1,99 @@
+"""GPU parity: the HIP path (through the C ABI) against the C oracle on the
+same inputs.  Bars: 1 step <= 1e-12 (relative to each component's max), dt
+sequences identical to 1e-12 relative, multi-step fields <= 1e-9.
+The oracle is itself pinned bit-for-bit to the compiled reference
+(tests/test_oracle_reference.py, tests/test_oracle_golden.py)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import COracle
+from tests.helpers import CASES, interior, perturbed, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return COracle()
+
+
+def _dev(g, p, aux, q):
+    from magic_master_amd.hip import MagicHIP
+    return MagicHIP(g, p, aux, q)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_step3_one_call(case, oracle):
+    (mx, my, mz), mthbc = CASES[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=case)
+    print(f"[step3] case {case} grid {mx}x{my}x{mz}", flush=True)
+    qo, cfl_o, st = oracle.step3(g, q, aux, 2.0, p.nspecadv)
+    assert st == 0
+    d = _dev(g, p, aux, q)
+    cfl_g = d.step3(2.0)
+    qg = d.get_q()
+    d.close()
+    assert abs(cfl_g - cfl_o) <= 1e-12 * cfl_o
+    # every cell, ghosts included (step3 also updates the first ghost layers)
+    assert rel_err(qg, qo) <= 1e-12
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_bc3_and_b4step3(case, oracle):
+    (mx, my, mz), mthbc = CASES[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=case)
+    print(f"[bc3] case {case} mthbc {mthbc}", flush=True)
+    qo = q.copy()
+    assert oracle.bc3(g, qo, aux, mthbc) == 0
+    oracle.b4step3(g, qo)
+    d = _dev(g, p, aux, q)
+    d.bc3(mthbc)
+    d.b4step3()
+    qg = d.get_q()
+    d.close()
+    assert rel_err(qg, qo) <= 1e-14
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_src3(case, oracle):
+    (mx, my, mz), mthbc = CASES[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=case, visc=3e10)
+    p.tcenter = 0.0
+    print(f"[src3] case {case}", flush=True)
+    qo = q.copy()
+    st, ndtr_o, ndtrc_o = oracle.src3(g, p, qo, aux, 0.0, 5.0, mthbc)
+    assert st == 0
+    d = _dev(g, p, aux, q)
+    ndtr, ndtrc = d.src3(0.0, 5.0, mthbc)
+    qg = d.get_q()
+    d.close()
+    assert (ndtr, ndtrc) == (ndtr_o, ndtrc_o)
+    assert rel_err(qg, qo, interior(g)) <= 1e-12
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_claw3_steps(case, oracle):
+    """10 accepted steps of the claw3 loop (first one rejected: dtv(1) = 50 s)."""
+    (mx, my, mz), mthbc = CASES[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=case, vel=0.05)
+    print(f"[claw3] case {case}", flush=True)
+    d = _dev(g, p, aux, q)
+    qo = q.copy()
+    dto = [50.0, 10.0, 0, 0, 0]
+    dtg = list(dto)
+    t_o = t_g = 0.0
+    for n in range(10):
+        st, t_o, dto, _, _, log_o = oracle.claw3(g, p, qo, aux, t_o, t_o - 1.0, dto, c.cflv,
+                                                 [100, 0], c.method, mthbc)
+        assert st == 0
+        t_g, dtg, _, _, log_g = d.claw3(t_g, t_g - 1.0, dtg, c.cflv, [100, 0], c.method, mthbc)
+        assert len(log_g.dt) == len(log_o), (n, log_g, log_o)
+        for (a, _, acc_o), b, acc_g in zip(log_o, log_g.dt, log_g.accepted):
+            assert acc_o == acc_g
+            assert abs(a - b) <= 1e-12 * abs(a)
+        dto[0] = dto[4]
+        dtg[0] = dtg[4]
+    qg = d.get_q()
+    d.close()
+    assert abs(t_g - t_o) <= 1e-12 * t_o
+    assert rel_err(qg, qo, interior(g)) <= 1e-9

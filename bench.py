@@ -1,0 +1,153 @@
+"""Benchmark of the MI355X MAGIC step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload full|plain]
+
+A step is one accepted MAGIC time step on device-resident state: bc3,
+b4step3, the copyq3 backup, step3 (x/y/z f-wave sweeps), the CFL readback
+and src3 (diffusion, conduction, plume forcing), under claw3's step control
+(mpilib/claw3_mpi.f:736-927).  The default workload is the full-domain FP64
+case: 250x200x300 cells (500x400x300 km), standard temperature column,
+plume at (250, 200) km.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line: Mcell-updates/s = mx*my*mz*K / T / 1e6, the roofline
+of the dominant kernel (the sweep kernel, HIP events on the library's
+stream) and a CPU baseline (the C port of the step on the host, one core,
+on a bounded slab of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Mcell-updates/s FP64 vs FP32-batched on 1 MI355X at stated error; "
+          "p50/p99 latency")
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell-update of the hyperbolic pass at meqn = 6:
+# read q, write q (2 x 48 B) + aux(17), aux(19) (16 B); one step3 call is
+# 27 sweep launches (3 directions x 9 colours), each covering 1/27 of the
+# cell-direction work.
+HYP_BYTES_PER_CELL = 112.0
+SRC_BYTES_PER_CELL = 168.0
+
+
+def run_gpu(workload, steps, warmup):
+    from magic_master_amd import problem as P
+    from magic_master_amd.hip import MagicHIP
+
+    if workload == "full":
+        c, p, g, aux, q = P.config_full()
+        name = ("full-domain FP64 250x200x300 (500x400x300 km), standard T(z) column, "
+                "hydrostatic FP64 background, mu=1.8e-5 Pr=0.7, plume 20 mm/10min at t=2400 s")
+    else:
+        c, p, g, aux, q = P.config_plain()
+        name = "plain isothermal 250 K 64x64x48 (4x4x2 km), plume 20 mm/10min at t=600 s"
+    dev = MagicHIP(g, p, aux, q)
+    t0 = 0.0
+    t, dtv, log = dev.run_steps(c, warmup, t0=t0)
+    dev.timing(True)
+    dev.timing_reset()
+    step_ms = []
+    tw0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        t, dtv, lg = dev.run_steps(c, 1, t0=t, dt0=dtv[0])
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        log.dt += lg.dt
+        log.accepted += lg.accepted
+    wall = time.perf_counter() - tw0
+    times = dev.timing_read()
+    dev.close()
+    return dict(c=c, g=g, name=name, wall=wall, step_ms=step_ms, times=times, t=t,
+                attempts=len(log.dt), rejected=sum(not a for a in log.accepted))
+
+
+def cpu_baseline(budget_s=20.0):
+    """The C port (oracle) of the same step on one host core, on a slab of the
+    full-domain workload: 250 x 16 x 300 cells, same column and physics."""
+    from magic_master_amd import problem as P
+    from oracle.oracle import COracle
+
+    O = COracle()
+    c, p, g, aux, q = P.config_full(mx=250, my=16, mz=300)
+    dtv = list(c.dtv)
+    t = 0.0
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        st, t, dtv, cflv, nv, log = O.claw3(g, p, q, aux, t, t - 1.0, dtv, c.cflv,
+                                            [c.nv[0], 0], c.method, c.mthbc)
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        dtv[0] = dtv[4]
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    return dict(value=g.ncell * steps / el / 1e6, unit="Mcell-updates/s", cores=1, kind="port",
+                sample=f"C port of the step (oracle/magic_oracle.c, gcc -O2) on a 250x16x300 "
+                       f"slab of the full-domain workload, {steps} accepted steps in {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="full", choices=["full", "plain"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "
+              "supported (the domain is not decomposed across GPUs)", file=sys.stderr)
+        sys.exit(2)
+    r = run_gpu(a.workload, a.steps, a.warmup)
+    g = r["g"]
+    K = a.steps
+    cells = g.ncell
+    value = cells * K / r["wall"] / 1e6
+    sw_ms = sum(r["times"][k][0] for k in ("sweep_x", "sweep_y", "sweep_z"))
+    sw_n = sum(r["times"][k][1] for k in ("sweep_x", "sweep_y", "sweep_z"))
+    avg_launch_s = sw_ms / max(sw_n, 1) / 1e3
+    bytes_per_launch = HYP_BYTES_PER_CELL * cells / 27.0
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    st = np.array(r["step_ms"])
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["wall"] / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic background column, fluid at rest, plume forcing)",
+        "config": {"workload": r["name"], "cells": [g.mx, g.my, g.mz], "meqn": g.meqn,
+                   "precision": "fp64", "parallelism": "single GPU",
+                   "attempted_steps": r["attempts"], "rejected_steps": r["rejected"],
+                   "model_time_end_s": round(r["t"], 3)},
+        "latency": {"unit": "ms per accepted step", "p50": round(float(np.percentile(st, 50)), 4),
+                    "p99": round(float(np.percentile(st, 99)), 4)},
+        "kernel_ms": {k: [round(v[0], 3), int(v[1])] for k, v in r["times"].items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "magic::sweep_kernel (one colour launch of one sweep direction)",
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 5),
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -37,6 +37,32 @@ def build_hip() -> str:
     return HIP_LIB
 
 
+FC = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+BIN = os.path.join(HERE, "bin")
+
+
+def build_fortran() -> str:
+    """Fortran host: iso_c_binding module, drop-in claw3/step3, driver program
+    (claw3ez_hip) linked against libmagic_hip.so."""
+    os.makedirs(BIN, exist_ok=True)
+    mod = os.path.join(BIN, "mod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for src in ("magic_hip.f90", "claw3_hip.f90", "claw3ez_hip.f90"):
+        obj = os.path.join(mod, src.replace(".f90", ".o"))
+        _run([FC, "-c", "-O2", "-fPIC", "-module-dir", mod, "-I", mod,
+              os.path.join(FORTRAN, src), "-o", obj])
+        objs.append(obj)
+    exe = os.path.join(BIN, "claw3ez_hip")
+    _run([FC, "-o", exe] + objs + ["-L" + HERE, "-lmagic_hip", "-Wl,-rpath,$ORIGIN/.."])
+    # the drop-in operators as a library a reference-style caller links against
+    lib = os.path.join(BIN, "libmagic_f77.so")
+    _run([FC, "-shared", "-o", lib, objs[0], objs[1], "-L" + HERE, "-lmagic_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
+    return exe
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "libmagic_oracle.so")
@@ -51,7 +77,7 @@ def build_reference(ref_root: str = "/root/reference"):
 
 
 def build_all():
-    out = [build_hip(), build_oracle()]
+    out = [build_hip(), build_fortran(), build_oracle()]
     ref = build_reference()
     if ref:
         out.append(ref)

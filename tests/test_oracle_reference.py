@@ -1,0 +1,59 @@
+"""The C oracle against the reference Fortran itself (oracle/_ref, compiled
+from the reference's own sources), on fresh random inputs: bit-for-bit.
+Skipped where the reference build is absent."""
+import numpy as np
+import pytest
+
+from oracle.oracle import COracle, Reference
+from tests.helpers import CASES, perturbed
+
+pytestmark = [pytest.mark.reference,
+              pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built")]
+
+
+@pytest.fixture(scope="module")
+def both():
+    return COracle(), Reference.get()
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+@pytest.mark.parametrize("seed", [11, 12])
+def test_operators_bitwise(ci, seed, both):
+    O, R = both
+    (mx, my, mz), mthbc = CASES[ci]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=seed, visc=1e9)
+    for nspec in (0, 1):
+        p.nspecadv = nspec
+        R.setup(p, mthbc)
+        qo, co, st = O.step3(g, q, aux, 1.5, nspec)
+        qr, cr = R.step3(g, q, aux, 1.5)
+        assert st == 0 and co == cr and np.array_equal(qo, qr)
+    q1, q2 = q.copy(), q.copy()
+    O.bc3(g, q1, aux, mthbc)
+    R.bc3(g, q2, aux, mthbc)
+    assert np.array_equal(q1, q2)
+    st, a, b = O.src3(g, p, q1, aux, 20.0, 4.0, mthbc)
+    R.src3(g, q2, aux, 20.0, 4.0, mthbc)
+    assert st == 0 and np.array_equal(q1, q2)
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_claw3_bitwise(ci, both):
+    O, R = both
+    (mx, my, mz), mthbc = CASES[ci]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=ci + 40, vel=0.1, visc=1e8)
+    R.setup(p, mthbc)
+    qo, qr = q.copy(), q.copy()
+    dto = [30.0, 10.0, 0, 0, 0]
+    dtr = list(dto)
+    t_o = t_r = 0.0
+    for n in range(5):
+        st, t_o, dto, co, _, _ = O.claw3(g, p, qo, aux, t_o, t_o - 1.0, dto, c.cflv, [100, 0],
+                                         c.method, mthbc)
+        info, t_r, dtr, cr, _ = R.claw3(g, qr, aux, t_r, t_r - 1.0, dtr, c.cflv, [100, 0],
+                                        c.method, mthbc)
+        assert st == 0 and info == 0
+        assert t_o == t_r and dto == dtr and co == cr
+        dto[0] = dto[4]
+        dtr[0] = dtr[4]
+    assert np.array_equal(qo, qr)

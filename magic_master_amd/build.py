@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["sweep.hip", "bc_src.hip", "capi.hip"]
+HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "bc_src.hip", "capi.hip"]
 HIP_LIB = os.path.join(HERE, "libmagic_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -60,6 +60,12 @@ def build_fortran() -> str:
     lib = os.path.join(BIN, "libmagic_f77.so")
     _run([FC, "-shared", "-o", lib, objs[0], objs[1], "-L" + HERE, "-lmagic_hip",
           "-Wl,-rpath,$ORIGIN/.."])
+    # a caller that uses claw3 with the reference's argument list
+    chk = os.path.join(mod, "dropin_check.o")
+    _run([FC, "-c", "-O2", "-module-dir", mod, "-I", mod,
+          os.path.join(FORTRAN, "dropin_check.f90"), "-o", chk])
+    _run([FC, "-o", os.path.join(BIN, "dropin_check"), chk, "-L" + BIN, "-lmagic_f77",
+          "-L" + HERE, "-lmagic_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
     return exe
 
 

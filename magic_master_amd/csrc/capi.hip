@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -377,6 +378,42 @@ static int launch_sweep(magic_ctx *c, SweepArgs a)
     return MAGIC_OK;
 }
 
+template <int NTR>
+static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g)
+{
+    const int TA = pm ? 4 : 16;
+    const int ntiles = (a.nA + 2 + TA - 1) / TA;
+    for (int ca = 0; ca < 2; ca++)
+        for (int cb = 0; cb < 3; cb++) {
+            a.cA = ca;
+            a.cB = cb;
+            const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
+            const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
+            if (gx == 0) continue;
+            if (pm) sweep_tile_kernel<NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            else sweep_tile_kernel<NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            TRY(launch_check());
+        }
+    return MAGIC_OK;
+}
+
+static int tile_dispatch(magic_ctx *c, const SweepArgs &a, bool pm, int a_is_g)
+{
+    switch (ntracer(c)) {
+    case 0: return launch_tile<0>(c, a, pm, a_is_g);
+    case 1: return launch_tile<1>(c, a, pm, a_is_g);
+    case 2: return launch_tile<2>(c, a, pm, a_is_g);
+    case 5: return launch_tile<5>(c, a, pm, a_is_g);
+    default: return MAGIC_EUNSUPPORTED;
+    }
+}
+
+static bool use_tile()
+{
+    const char *e = getenv("MAGIC_SWEEP");
+    return !(e && strcmp(e, "pencil") == 0);
+}
+
 static int sweep_dispatch(magic_ctx *c, const SweepArgs &a)
 {
     switch (ntracer(c)) {
@@ -414,7 +451,8 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = 1; a.sA = d.NX; a.sB = NXY;
         a.gravity = 0; a.sweep = 1;
         a.dtdn = dtdx; a.fg = dtdy; a.fh = dtdz;
-        TRY(sweep_dispatch(c, a));
+        if (use_tile()) TRY(tile_dispatch(c, a, true, 1));   // tile axis y = g
+        else TRY(sweep_dispatch(c, a));
     }
     {   // y-sweeps: lib/step3.f:289-450 (g = z, h = x)
         Span sp(c, TC_SWY, 9);
@@ -423,7 +461,13 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = d.NX; a.sA = NXY; a.sB = 1;
         a.gravity = 0; a.sweep = 2;
         a.dtdn = dtdy; a.fg = dtdz; a.fh = dtdx;
-        TRY(sweep_dispatch(c, a));
+        if (use_tile()) {                                     // tile axis x = h
+            SweepArgs t = a;
+            t.nA = c->mx; t.nB = c->mz; t.sA = 1; t.sB = NXY;
+            TRY(tile_dispatch(c, t, false, 0));
+        } else {
+            TRY(sweep_dispatch(c, a));
+        }
     }
     {   // z-sweeps: lib/step3.f:459-621 (g = x, h = y), gravity on
         Span sp(c, TC_SWZ, 9);
@@ -432,7 +476,8 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = NXY; a.sA = 1; a.sB = d.NX;
         a.gravity = 1; a.sweep = 3;
         a.dtdn = dtdz; a.fg = dtdx; a.fh = dtdy;
-        TRY(sweep_dispatch(c, a));
+        if (use_tile()) TRY(tile_dispatch(c, a, false, 1));  // tile axis x = g
+        else TRY(sweep_dispatch(c, a));
     }
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));

@@ -34,4 +34,11 @@ struct SweepArgs {
 template <int NTR>
 __global__ void sweep_kernel(SweepArgs a);
 
+// Tiled variant: PM = lanes along the pencil (x-sweep), otherwise lanes along
+// the tile axis a (y- and z-sweeps).  a_is_g says whether the tile axis is
+// the sweep's g direction; sA/sB/nA/nB then describe the tile axis a and the
+// coloured axis b.
+template <int NTR, bool PM>
+__global__ void sweep_tile_kernel(SweepArgs a, int a_is_g);
+
 }  // namespace magic

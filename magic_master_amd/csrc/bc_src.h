@@ -38,10 +38,9 @@ __global__ void k_zghost_uvw(Dims d, double *q);
 __global__ void k_diff_cross(Dims d, double *q, const double *u, const double *coef,
                              double dxy, double dxz, double dyz);
 __global__ void k_copy_interior(Dims d, double *dst, const double *src, int nplanes);
-__global__ void k_stencil(Dims d, double *qt, const double *u, long long s);
-__global__ void k_assign(Dims d, double *out, const double *u, const double *coef,
-                         const double *qt, double f);
 __global__ void k_conduct_zghost(Dims d, double *u);
+__global__ void k_pass(Dims d, double *out, const double *src, const double *ghost,
+                       const double *coef, double f, int axis);
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
 __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, int nc);

@@ -181,20 +181,27 @@ __global__ void k_copy_interior(Dims d, double *dst, const double *src, int npla
     for (int m = 0; m < nplanes; m++) dst[m * d.P + c] = src[m * d.P + c];
 }
 
-// qt = second difference of u along one axis (diffuse.f:156-159 etc.)
-__global__ void k_stencil(Dims d, double *qt, const double *u, long long s)
+// One explicit directional diffusion pass (diffuse.f:153-208, conduct.f:262-323):
+//   out = u + f*coef*(second difference of u along axis),
+// computed on the interior.  The interior values of u come from src and the
+// values on the ghost layers (which the reference copies once per call and
+// never refreshes between passes) from ghost; src may equal ghost.  The
+// second difference keeps the reference's operand order: (up-mid-mid+down)
+// along x and y, (down-mid-mid+up) along z.
+__global__ void k_pass(Dims d, double *out, const double *src, const double *ghost,
+                       const double *coef, double f, int axis)
 {
     INTERIOR_IJK
-    const double am = u[c - s], aq = u[c], ap = u[c + s];
-    qt[c] = (ap - aq - aq + am);
-}
-
-// out = u + f*coef*qt on the interior (diffuse.f:165-166 etc.)
-__global__ void k_assign(Dims d, double *out, const double *u, const double *coef,
-                         const double *qt, double f)
-{
-    INTERIOR_IJK
-    out[c] = u[c] + f * coef[c] * qt[c];
+    long long s;
+    bool lo_in, hi_in;
+    if (axis == 0) { s = 1; lo_in = i > 1; hi_in = i < d.mx; }
+    else if (axis == 1) { s = d.NX; lo_in = j > 1; hi_in = j < d.my; }
+    else { s = (long long)d.NX * d.NY; lo_in = k > 1; hi_in = k < d.mz; }
+    const double aq = src[c];
+    const double am = lo_in ? src[c - s] : ghost[c - s];
+    const double ap = hi_in ? src[c + s] : ghost[c + s];
+    const double qt = axis == 2 ? (am - aq - aq + ap) : (ap - aq - aq + am);
+    out[c] = aq + f * coef[c] * qt;
 }
 
 // conduct.f:86-91: z ghosts of u, for j in [1-mbc, my+mbc] and i in

@@ -546,8 +546,11 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     }
     for (int nn = 1; nn <= ndtr; nn++) {
         TRY(bc3_launch(c, mthbc));
-        Span sp(c, TC_SRC, 34 + 7 * (ndtrc > 0 ? ndtrc : 0));
-        // diffuse: diffuse.f:37-210
+        Span sp(c, TC_SRC, 13 + 6 * (ndtrc > 0 ? ndtrc : 0));
+        // diffuse: diffuse.f:37-210.  u = q(2:4) after the z-ghost copy; the
+        // cross terms write q's interior; each component then takes three
+        // directional passes whose interior input is the previous pass's
+        // output and whose ghost input is u (stale, as in the reference).
         const double fourthirds = 4.0 / 3.0;
         double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q);
@@ -556,8 +559,6 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         const double dtr32 = dtr / 6.0;
         k_diff_cross<<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
                                                  dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz));
-        TRY(launch_check());
-        k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->u, c->q + P, 3);
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             if (cc == 1) {
@@ -569,30 +570,31 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdz2 = fourthirds * dtdz2;
                 dtdy2 = dtdy2 / fourthirds;
             }
-            double *uc = c->u + (cc - 1) * P;
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, 1);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, uc, uc, nu, c->qt, dtdx2);
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, d.NX);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, uc, uc, nu, c->qt, dtdy2);
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, uc, -NXY);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, c->q + cc * P, uc, nu, c->qt, dtdz2);
+            double *uc = c->u + (cc - 1) * P, *qc = c->q + cc * P;
+            k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0);
+            k_pass<<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1);
+            k_pass<<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2);
             TRY(launch_check());
         }
-        // conduct: conduct.f:245-332
+        // conduct: conduct.f:245-332.  ut = q(5) on every cell once; the
+        // interior ping-pongs between ut and qt, the ghost layers stay in ut
+        // (x/y ghosts never refreshed, z ghosts refreshed for i <= mz+mbc
+        // before each z pass); the last z pass writes q(5) directly.
         const double cx = (dtrc) / (c->dx * c->dx), cy = (dtrc) / (c->dy * c->dy), cz = (dtrc) / (c->dz * c->dz);
-        HIPCHECK(hipMemcpyAsync(c->ut, c->q + 4 * P, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        double *q5 = c->q + 4 * P;
+        HIPCHECK(hipMemcpyAsync(c->ut, q5, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, 1);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cx);
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, d.NX);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cy);
+            k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0);
+            k_pass<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1);
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut);
-            k_stencil<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, -NXY);
-            k_assign<<<gint, 64, 0, c->stream>>>(d, c->ut, c->ut, kap, c->qt, cz);
+            if (k < ndtrc) {
+                k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2);
+                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1);
+            } else {
+                k_pass<<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2);
+            }
             TRY(launch_check());
         }
-        k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->q + 4 * P, c->ut, 1);
-        TRY(launch_check());
     }
     {
         Span sp(c, TC_SRC, 1);

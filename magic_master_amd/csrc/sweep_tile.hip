@@ -301,13 +301,9 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(SweepArgs A, int a_is_g
             for (int m = 0; m < NC; m++) cqxx[m] = k1 * fw1[m] + k2 * fw2[m] + k3 * fw3[m];
             cflmax = fmaxF(cflmax, fmaxF(fmaxF(c1, c2), c3));
         }
-        double Bmq[NC], Bpq[NC], Cmq[NC], Cpq[NC];
-        if (act2) {
-            tsplit<NC, 2, false>(R, cqxx, Bmq, Bpq);
-            tsplit<NC, 3, false>(R, cqxx, Cmq, Cpq);
-        }
-
         // ---------------- part 3: transverse terms, two phases
+        // Each phase recomputes the two cqxx splits it needs, in an order that
+        // keeps at most four split vectors alive next to the accumulator.
         for (int phase = 0; phase < 2; phase++) {
             // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
             const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
@@ -315,31 +311,42 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(SweepArgs A, int a_is_g
             acc.zero();
             if (on) {
                 const double sgn = phase == 0 ? 1.0 : -1.0;   // +3 cqxx mixing for amdq, -3 for apdq
-                const double (&asd)[NC] = phase == 0 ? amdq : apdq;
-                double Bm[NC], Bp[NC], Cm[NC], Cp[NC], t1[NC], t2[NC], x1[NC], x2[NC];
-                tsplit<NC, 2, false>(R, asd, Bm, Bp);
-                tsplit<NC, 3, false>(R, asd, Cm, Cp);
                 const double cc = phase == 0 ? -d : d;        // sign of the correction-wave term
-                acc.g(1, 0, -0.5 * d * cg, Bm); acc.g(2, 0, -0.5 * d * cg, Bp);
-                acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
-                axpy3(t1, Cp, 3.0 * sgn, Cpq);
-                axpy3(t2, Cm, 3.0 * sgn, Cmq);
-                tsplit<NC, 2, true>(R, t1, x1, x2);
-                acc.g(2, 0, k6g * cg, x2);  acc.g(1, 0, k6g * cg, x1);
-                acc.g(2, 1, -k6g * cg, x2); acc.g(1, 1, -k6g * cg, x1);
-                tsplit<NC, 2, true>(R, t2, x1, x2);
-                acc.g(2, 0, -k6g * cg, x2); acc.g(1, 0, -k6g * cg, x1);
-                acc.g(2, -1, k6g * cg, x2); acc.g(1, -1, k6g * cg, x1);
-                acc.h(1, 0, -0.5 * d * ch, Cm); acc.h(2, 0, -0.5 * d * ch, Cp);
-                acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
-                axpy3(t1, Bp, 3.0 * sgn, Bpq);
-                axpy3(t2, Bm, 3.0 * sgn, Bmq);
-                tsplit<NC, 3, true>(R, t1, x1, x2);
-                acc.h(2, 0, k6h * ch, x2);  acc.h(1, 0, k6h * ch, x1);
-                acc.h(2, 1, -k6h * ch, x2); acc.h(1, 1, -k6h * ch, x1);
-                tsplit<NC, 3, true>(R, t2, x1, x2);
-                acc.h(2, 0, -k6h * ch, x2); acc.h(1, 0, -k6h * ch, x1);
-                acc.h(2, -1, k6h * ch, x2); acc.h(1, -1, k6h * ch, x1);
+                const double (&asd)[NC] = phase == 0 ? amdq : apdq;
+                {
+                    // C splits (h direction): 2-D part, correction, then the
+                    // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
+                    double Cm[NC], Cp[NC], Cmq[NC], Cpq[NC];
+                    tsplit<NC, 3, false>(R, cqxx, Cmq, Cpq);
+                    tsplit<NC, 3, false>(R, asd, Cm, Cp);
+                    acc.h(1, 0, -0.5 * d * ch, Cm); acc.h(2, 0, -0.5 * d * ch, Cp);
+                    acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
+                    axpy3(Cp, Cp, 3.0 * sgn, Cpq);
+                    axpy3(Cm, Cm, 3.0 * sgn, Cmq);
+                    tsplit<NC, 2, true>(R, Cp, Cmq, Cpq);      // B-, B+ of c+asdq2
+                    acc.g(2, 0, k6g * cg, Cpq);  acc.g(1, 0, k6g * cg, Cmq);
+                    acc.g(2, 1, -k6g * cg, Cpq); acc.g(1, 1, -k6g * cg, Cmq);
+                    tsplit<NC, 2, true>(R, Cm, Cmq, Cpq);      // B-, B+ of c-asdq2
+                    acc.g(2, 0, -k6g * cg, Cpq); acc.g(1, 0, -k6g * cg, Cmq);
+                    acc.g(2, -1, k6g * cg, Cpq); acc.g(1, -1, k6g * cg, Cmq);
+                }
+                {
+                    // B splits (g direction): 2-D part, correction, then the
+                    // h-direction corner terms of b*asdq +- 3 b*cqxx (flux3fw.f:510-538)
+                    double Bm[NC], Bp[NC], Bmq[NC], Bpq[NC];
+                    tsplit<NC, 2, false>(R, cqxx, Bmq, Bpq);
+                    tsplit<NC, 2, false>(R, asd, Bm, Bp);
+                    acc.g(1, 0, -0.5 * d * cg, Bm); acc.g(2, 0, -0.5 * d * cg, Bp);
+                    acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
+                    axpy3(Bp, Bp, 3.0 * sgn, Bpq);
+                    axpy3(Bm, Bm, 3.0 * sgn, Bmq);
+                    tsplit<NC, 3, true>(R, Bp, Bmq, Bpq);      // C-, C+ of b+asdq
+                    acc.h(2, 0, k6h * ch, Bpq);  acc.h(1, 0, k6h * ch, Bmq);
+                    acc.h(2, 1, -k6h * ch, Bpq); acc.h(1, 1, -k6h * ch, Bmq);
+                    tsplit<NC, 3, true>(R, Bm, Bmq, Bpq);      // C-, C+ of b-asdq
+                    acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
+                    acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
+                }
 #pragma unroll
                 for (int m = 0; m < NC; m++)
                     acc.D[1][1][m] += (phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);

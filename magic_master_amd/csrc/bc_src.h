@@ -39,6 +39,7 @@ __global__ void k_diff_cross(Dims d, double *q, const double *u, const double *c
                              double dxy, double dxz, double dyz);
 __global__ void k_copy_interior(Dims d, double *dst, const double *src, int nplanes);
 __global__ void k_conduct_zghost(Dims d, double *u);
+__global__ void k_conduct_zspill(Dims d, double *u, int S);
 __global__ void k_pass(Dims d, double *out, const double *src, const double *ghost,
                        const double *coef, double f, int axis);
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);

@@ -208,10 +208,10 @@ __global__ void k_pass(Dims d, double *out, const double *src, const double *gho
 // [1-mbc, mz+mbc] -- the reference bounds i by mz, not mx.  When mx > mz the
 // columns i > mz+mbc keep their stale z ghosts (which then enter the z
 // stencil of the interior cells of those columns).  When mz > mx the
-// reference's out-of-row i spills into the next row: it rewrites ghosts of
-// the next row with their own correct values, and on the last row it writes
-// j = 1-mbc ghost-row cells that no interior stencil reads; neither changes
-// any interior result, so the fill here stops at the row end.
+// out-of-row i addresses the array linearly: within a layer the writes land
+// on later rows of the same layer with those rows' own values (what the
+// in-row fill writes anyway), so this kernel fills whole rows and
+// k_conduct_zspill adds the writes that cross into the next layer.
 __global__ void k_conduct_zghost(Dims d, double *u)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
@@ -222,6 +222,28 @@ __global__ void k_conduct_zghost(Dims d, double *u)
     for (int kbc = 1; kbc <= d.mbc; kbc++) {
         u[cid(d, i, j, 1 - kbc)] = lo;
         u[cid(d, i, j, d.mz + kbc)] = hi;
+    }
+}
+
+// The layer-crossing writes of conduct.f:86-91 when mz > mx.  The rows at
+// the end of a layer address i up to mz+mbc, S = mz-mx elements past the
+// layer's last cell, so at in-layer offsets r < S the loop writes
+//   layer 2-kbc <- layer 2          (u(i,j,1-kbc) = u(i,j,1), kbc = 1..mbc)
+//   layer mz+1+kbc <- layer mz+1    (u(i,j,mz+kbc) = u(i,j,mz))
+// after the in-row fills of the rows those offsets fall in (the host checks
+// that the rows written lie before the rows that write them).  Writes past
+// the last layer leave the reference's array; they are dropped here.
+// Layers are Fortran k indices; r counts from cell (1-mbc, 1-mbc).
+__global__ void k_conduct_zspill(Dims d, double *u, int S)
+{
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= S) return;
+    const long long NXY = (long long)d.NX * d.NY;
+    auto L = [&](int k) { return (long long)(k + d.mbc - 1) * NXY + r; };
+    const double lo = u[L(2)], hi = u[L(d.mz + 1)];
+    for (int kbc = 1; kbc <= d.mbc; kbc++) {
+        u[L(2 - kbc)] = lo;
+        if (d.mz + 1 + kbc <= d.mz + d.mbc) u[L(d.mz + 1 + kbc)] = hi;
     }
 }
 

@@ -154,6 +154,14 @@ extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux,
     if (!sweep_supported(meqn - 5)) return MAGIC_EUNSUPPORTED;
     const int nmax = mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
     if (nmax + 3 > 512) return MAGIC_EUNSUPPORTED;   // one workgroup per pencil
+    if (mz > mx) {
+        // conduct.f:86-91 writes S = mz-mx elements past each layer; the
+        // device replays those writes after the in-row fills, which is the
+        // reference's order only while the rows they land in come before
+        // the rows that write them.
+        const long long S = mz - mx, NX = mx + 2 * mbc, NY = my + 2 * mbc;
+        if (2 * ((S - 1) / NX) + 1 >= NY) return MAGIC_EUNSUPPORTED;
+    }
     magic_ctx *c = new magic_ctx();
     c->mx = mx; c->my = my; c->mz = mz; c->meqn = meqn; c->mbc = mbc; c->maux = maux;
     c->xlower = xlower; c->ylower = ylower; c->zlower = zlower; c->dx = dx; c->dy = dy; c->dz = dz;
@@ -587,6 +595,10 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
             k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0);
             k_pass<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1);
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut);
+            if (c->mz > c->mx) {
+                const int S = c->mz - c->mx;
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S);
+            }
             if (k < ndtrc) {
                 k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2);
                 k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1);

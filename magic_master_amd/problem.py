@@ -272,10 +272,12 @@ def _vgrav(col):
 
 
 def _fill_aux_from_column(g: Grid, col) -> np.ndarray:
-    aux = np.zeros(g.shape_aux)
+    tmpl = np.zeros((g.shape_aux[0], g.maux))
     for key, slot in (("rho", AUX_RHO), ("mx", AUX_MX), ("E", AUX_E), ("nu", AUX_NU),
                       ("kap", AUX_KAPPA), ("vg", AUX_VGRAV), ("R", AUX_R), ("gam", AUX_GAMMA)):
-        aux[:, :, :, slot - 1] = np.asarray(col[key])[:, None, None]
+        tmpl[:, slot - 1] = np.asarray(col[key])
+    aux = np.empty(g.shape_aux)
+    aux[...] = tmpl[:, None, None, :]          # one contiguous pass
     return aux
 
 

@@ -1,0 +1,119 @@
+"""GPU parity on the two named workloads.
+
+1. Plain isothermal workload (64x64x48, problem.config_plain) for 200 claw3
+   calls against the compiled reference's own run (tests/golden/plain_200.npz):
+   every step's next dt and CFL, and every 4th cell of the state after steps
+   1, 10 and 200, plus per-component sums.  Bars: 1e-12 after one step, 1e-9
+   after 200 (relative to each component's largest value; the momenta share
+   one scale).
+2. Full-domain workload (250x200x300, problem.config_full) at its real size,
+   through a size-independent property: with one horizontal axis periodic and
+   the state uniform along it, every row along that axis must equal the
+   oracle's run of the same problem two cells wide.  This covers the sweeps,
+   bc3 and src3 at full extents, where the oracle cannot run in seconds.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from magic_master_amd import problem as P
+from oracle.oracle import COracle
+from tests.helpers import rel_err_mom, uniform_state
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _dev(g, p, aux, q):
+    from magic_master_amd.hip import MagicHIP
+    return MagicHIP(g, p, aux, q)
+
+
+def test_plain_workload_200_steps_vs_reference_run():
+    z = np.load(os.path.join(GOLD, "plain_200.npz"))
+    c, p, g, aux, q = P.config_plain()
+    m = g.mbc
+    print(f"[plain200] grid {g.mx}x{g.my}x{g.mz}", flush=True)
+    d = _dev(g, p, aux, q)
+    dtv = list(c.dtv)
+    cflv = list(c.cflv)
+    t = 0.0
+    dts, cfls = [], []
+    try:
+        for n in range(1, 201):
+            t, dtv, cflv, _, _ = d.claw3(t, t - 1.0, dtv, cflv, [c.nv[0], 0], c.method, c.mthbc)
+            dts.append(dtv[4])
+            cfls.append(cflv[3])
+            dtv[0] = dtv[4]
+            if n in (1, 10, 200):
+                qg = d.get_q()
+                it = qg[m:m + g.mz, m:m + g.my, m:m + g.mx]
+                sub = qg[m:m + g.mz:4, m:m + g.my:4, m:m + g.mx:4]
+                tol = 1e-12 if n == 1 else 1e-9
+                err = rel_err_mom(sub, z[f"step{n}_sub"])
+                print(f"  step {n}: field err {err:.3e}", flush=True)
+                assert err <= tol, (n, err)
+                s = it.reshape(-1, g.meqn).sum(0)
+                ref = z[f"step{n}_sum"]
+                bound = np.sqrt(z[f"step{n}_sumsq"] * it[..., 0].size)   # >= sum |q|
+                assert np.all(np.abs(s - ref) <= tol * bound), (n, s, ref)
+    finally:
+        d.close()
+    dts = np.array(dts)
+    cfls = np.array(cfls)
+    assert abs(dts[0] - z["next_dt"][0]) <= 1e-12 * z["next_dt"][0]
+    assert np.abs(dts / z["next_dt"] - 1.0).max() <= 1e-9
+    assert np.abs(cfls / z["cfl"] - 1.0).max() <= 1e-9
+    assert abs(t - float(z["t"])) <= 1e-9 * float(z["t"])
+
+
+def _oracle_run(c, p, g, aux, q, nsteps):
+    O = COracle()
+    dtv = list(c.dtv)
+    t = 0.0
+    out = []
+    for _ in range(nsteps):
+        st, t, dtv, _, _, _ = O.claw3(g, p, q, aux, t, t - 1.0, dtv, c.cflv, [c.nv[0], 0],
+                                      c.method, c.mthbc)
+        assert st == 0
+        out.append((t, dtv[4], q.copy()))
+        dtv[0] = dtv[4]
+    return out
+
+
+@pytest.mark.timeout(400)        # host set-up of the 4 GB reference-layout aux array
+@pytest.mark.parametrize("axis", ["y", "x"])
+def test_full_domain_rows_match_narrow_oracle_run(axis):
+    mx, my, mz = 250, 200, 300
+    nsteps = 2
+    narrow = (mx, 2, mz) if axis == "y" else (2, my, mz)
+    print(f"[full-uniform-{axis}] oracle on {narrow}", flush=True)
+    ref = _oracle_run(*uniform_state(*narrow, axis), nsteps)
+    c, p, g, aux, q = uniform_state(mx, my, mz, axis)
+    print(f"[full-uniform-{axis}] device on {mx}x{my}x{mz}", flush=True)
+    d = _dev(g, p, aux, q)
+    del aux, q
+    m = g.mbc
+    dtv = list(c.dtv)
+    t = 0.0
+    try:
+        for n in range(nsteps):
+            t, dtv, _, _, _ = d.claw3(t, t - 1.0, dtv, c.cflv, [c.nv[0], 0], c.method, c.mthbc)
+            t_o, dt_o, q_o = ref[n]
+            assert abs(t - t_o) <= 1e-12 * t_o
+            assert abs(dtv[4] - dt_o) <= 1e-12 * dt_o
+            dtv[0] = dtv[4]
+            qg = d.get_q()
+            tol = 1e-12 if n == 0 else 1e-11
+            if axis == "y":
+                row = q_o[m:m + mz, m, m:m + mx]
+                err = max(rel_err_mom(qg[m:m + mz, j, m:m + mx], row) for j in range(m, m + my))
+            else:
+                row = q_o[m:m + mz, m:m + my, m]
+                err = max(rel_err_mom(qg[m:m + mz, m:m + my, i], row) for i in range(m, m + mx))
+            print(f"  step {n + 1}: worst row err {err:.3e}", flush=True)
+            assert err <= tol
+            del qg
+    finally:
+        d.close()

@@ -28,12 +28,18 @@ sys.path.insert(0, ROOT)
 METRIC = ("Mcell-updates/s FP64 vs FP32-batched on 1 MI355X at stated error; "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell-update of the hyperbolic pass at meqn = 6:
-# read q, write q (2 x 48 B) + aux(17), aux(19) (16 B); one step3 call is
-# 27 sweep launches (3 directions x 9 colours), each covering 1/27 of the
-# cell-direction work.
+# Algorithmic bytes per cell-update at meqn = 6.  Hyperbolic pass: read q,
+# write q (2 x 48 B) + aux(17), aux(19) (16 B), shared evenly by the three
+# sweep directions; a direction's launches (tile colours) each cover an equal
+# share of its cells.  Source pass: read q, write q, read aux 1-5, 11, 12, 18,
+# 19 (96 + 72 B).
 HYP_BYTES_PER_CELL = 112.0
 SRC_BYTES_PER_CELL = 168.0
+SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
+    "sweep_x": "magic::sweep_tile_kernel<{ntr}, true>",
+    "sweep_y": "magic::sweep_tile_kernel<{ntr}, false>",
+    "sweep_z": "magic::sweep_tile_kernel<{ntr}, false>",
+}
 
 
 def run_gpu(workload, steps, warmup):
@@ -53,6 +59,7 @@ def run_gpu(workload, steps, warmup):
     dev.timing(True)
     dev.timing_reset()
     step_ms = []
+    timed = 0
     tw0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
@@ -60,11 +67,12 @@ def run_gpu(workload, steps, warmup):
         step_ms.append((time.perf_counter() - s0) * 1e3)
         log.dt += lg.dt
         log.accepted += lg.accepted
+        timed += len(lg.dt)
     wall = time.perf_counter() - tw0
     times = dev.timing_read()
     dev.close()
     return dict(c=c, g=g, name=name, wall=wall, step_ms=step_ms, times=times, t=t,
-                attempts=len(log.dt), rejected=sum(not a for a in log.accepted))
+                attempts=len(log.dt), rejected=sum(not a for a in log.accepted), timed_attempts=timed)
 
 
 def cpu_baseline(budget_s=20.0):
@@ -112,10 +120,20 @@ def main():
     K = a.steps
     cells = g.ncell
     value = cells * K / r["wall"] / 1e6
-    sw_ms = sum(r["times"][k][0] for k in ("sweep_x", "sweep_y", "sweep_z"))
-    sw_n = sum(r["times"][k][1] for k in ("sweep_x", "sweep_y", "sweep_z"))
-    avg_launch_s = sw_ms / max(sw_n, 1) / 1e3
-    bytes_per_launch = HYP_BYTES_PER_CELL * cells / 27.0
+    # dominant sweep kernel: the instantiation with the most time (y and z
+    # share one); its average launch from HIP events on the library stream
+    per_kernel = {}
+    for cls, kname in SWEEP_KERNELS.items():
+        ms, n = r["times"][cls]
+        k = kname.format(ntr=g.meqn - 5)
+        acc = per_kernel.setdefault(k, [0.0, 0])
+        acc[0] += ms
+        acc[1] += n
+    kname, (k_ms, k_n) = max(per_kernel.items(), key=lambda kv: kv[1][0])
+    sw_n = sum(r["times"][k][1] for k in SWEEP_KERNELS)
+    launches_per_step3 = sw_n / max(r["timed_attempts"], 1)
+    avg_launch_s = k_ms / max(k_n, 1) / 1e3
+    bytes_per_launch = HYP_BYTES_PER_CELL * cells / launches_per_step3
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     st = np.array(r["step_ms"])
     out = {
@@ -140,9 +158,12 @@ def main():
         "kernel_ms": {k: [round(v[0], 3), int(v[1])] for k, v in r["times"].items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "magic::sweep_kernel (one colour launch of one sweep direction)",
-                     "avg_launch_ms": round(avg_launch_s * 1e3, 5),
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel": kname, "avg_launch_ms": round(avg_launch_s * 1e3, 5),
+                     "launches": k_n, "sweep_launches_per_step3": launches_per_step3,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "bytes_per_cell_update": HYP_BYTES_PER_CELL + SRC_BYTES_PER_CELL,
+                     "step_hbm_frac": round(value * 1e6 * (HYP_BYTES_PER_CELL + SRC_BYTES_PER_CELL)
+                                            / (HBM_PEAK_GBS * 1e9), 5)},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_budget)

@@ -370,7 +370,7 @@ __global__ void k_zero_scalars(unsigned long long *s)
 }
 
 template <int NTR>
-static int launch_sweep(magic_ctx *c, SweepArgs a)
+static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
 {
     const int NI = a.n + 3;
     const int threads = ((NI + 63) / 64) * 64;
@@ -382,12 +382,13 @@ static int launch_sweep(magic_ctx *c, SweepArgs a)
             dim3 grid((unsigned)((a.nA + 1 - ca) / 3 + 1), (unsigned)((a.nB + 1 - cb) / 3 + 1));
             sweep_kernel<NTR><<<grid, threads, smem, c->stream>>>(a);
             TRY(launch_check());
+            ++*nl;
         }
     return MAGIC_OK;
 }
 
 template <int NTR>
-static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g)
+static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g, int *nl)
 {
     const int TA = pm ? 4 : 16;
     const int ntiles = (a.nA + 2 + TA - 1) / TA;
@@ -401,17 +402,18 @@ static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g)
             if (pm) sweep_tile_kernel<NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
             else sweep_tile_kernel<NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
             TRY(launch_check());
+            ++*nl;
         }
     return MAGIC_OK;
 }
 
-static int tile_dispatch(magic_ctx *c, const SweepArgs &a, bool pm, int a_is_g)
+static int tile_dispatch(magic_ctx *c, const SweepArgs &a, bool pm, int a_is_g, int *nl)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<0>(c, a, pm, a_is_g);
-    case 1: return launch_tile<1>(c, a, pm, a_is_g);
-    case 2: return launch_tile<2>(c, a, pm, a_is_g);
-    case 5: return launch_tile<5>(c, a, pm, a_is_g);
+    case 0: return launch_tile<0>(c, a, pm, a_is_g, nl);
+    case 1: return launch_tile<1>(c, a, pm, a_is_g, nl);
+    case 2: return launch_tile<2>(c, a, pm, a_is_g, nl);
+    case 5: return launch_tile<5>(c, a, pm, a_is_g, nl);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -422,13 +424,13 @@ static bool use_tile()
     return !(e && strcmp(e, "pencil") == 0);
 }
 
-static int sweep_dispatch(magic_ctx *c, const SweepArgs &a)
+static int sweep_dispatch(magic_ctx *c, const SweepArgs &a, int *nl)
 {
     switch (ntracer(c)) {
-    case 0: return launch_sweep<0>(c, a);
-    case 1: return launch_sweep<1>(c, a);
-    case 2: return launch_sweep<2>(c, a);
-    case 5: return launch_sweep<5>(c, a);
+    case 0: return launch_sweep<0>(c, a, nl);
+    case 1: return launch_sweep<1>(c, a, nl);
+    case 2: return launch_sweep<2>(c, a, nl);
+    case 5: return launch_sweep<5>(c, a, nl);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -453,17 +455,17 @@ static int step3_launch(magic_ctx *c, double dt)
     for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
     const double dtdx = dt / c->dx, dtdy = dt / c->dy, dtdz = dt / c->dz;
     {   // x-sweeps: lib/step3.f:121-280 (g = y, h = z)
-        Span sp(c, TC_SWX, 9);
+        Span sp(c, TC_SWX, 0);
         a.comp[0] = 0; a.comp[1] = 1; a.comp[2] = 2; a.comp[3] = 3; a.comp[4] = 4;
         a.n = c->mx; a.nA = c->my; a.nB = c->mz;
         a.sN = 1; a.sA = d.NX; a.sB = NXY;
         a.gravity = 0; a.sweep = 1;
         a.dtdn = dtdx; a.fg = dtdy; a.fh = dtdz;
-        if (use_tile()) TRY(tile_dispatch(c, a, true, 1));   // tile axis y = g
-        else TRY(sweep_dispatch(c, a));
+        if (use_tile()) TRY(tile_dispatch(c, a, true, 1, &sp.n));   // tile axis y = g
+        else TRY(sweep_dispatch(c, a, &sp.n));
     }
     {   // y-sweeps: lib/step3.f:289-450 (g = z, h = x)
-        Span sp(c, TC_SWY, 9);
+        Span sp(c, TC_SWY, 0);
         a.comp[0] = 0; a.comp[1] = 2; a.comp[2] = 3; a.comp[3] = 1; a.comp[4] = 4;
         a.n = c->my; a.nA = c->mz; a.nB = c->mx;
         a.sN = d.NX; a.sA = NXY; a.sB = 1;
@@ -472,20 +474,20 @@ static int step3_launch(magic_ctx *c, double dt)
         if (use_tile()) {                                     // tile axis x = h
             SweepArgs t = a;
             t.nA = c->mx; t.nB = c->mz; t.sA = 1; t.sB = NXY;
-            TRY(tile_dispatch(c, t, false, 0));
+            TRY(tile_dispatch(c, t, false, 0, &sp.n));
         } else {
-            TRY(sweep_dispatch(c, a));
+            TRY(sweep_dispatch(c, a, &sp.n));
         }
     }
     {   // z-sweeps: lib/step3.f:459-621 (g = x, h = y), gravity on
-        Span sp(c, TC_SWZ, 9);
+        Span sp(c, TC_SWZ, 0);
         a.comp[0] = 0; a.comp[1] = 3; a.comp[2] = 1; a.comp[3] = 2; a.comp[4] = 4;
         a.n = c->mz; a.nA = c->mx; a.nB = c->my;
         a.sN = NXY; a.sA = 1; a.sB = d.NX;
         a.gravity = 1; a.sweep = 3;
         a.dtdn = dtdz; a.fg = dtdx; a.fh = dtdy;
-        if (use_tile()) TRY(tile_dispatch(c, a, false, 1));  // tile axis x = g
-        else TRY(sweep_dispatch(c, a));
+        if (use_tile()) TRY(tile_dispatch(c, a, false, 1, &sp.n));  // tile axis x = g
+        else TRY(sweep_dispatch(c, a, &sp.n));
     }
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));

@@ -312,7 +312,9 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(SweepArgs A, int a_is_g
             if (on) {
                 const double sgn = phase == 0 ? 1.0 : -1.0;   // +3 cqxx mixing for amdq, -3 for apdq
                 const double cc = phase == 0 ? -d : d;        // sign of the correction-wave term
-                const double (&asd)[NC] = phase == 0 ? amdq : apdq;
+                double asd[NC];
+#pragma unroll
+                for (int m = 0; m < NC; m++) asd[m] = phase == 0 ? amdq[m] : apdq[m];
                 {
                     // C splits (h direction): 2-D part, correction, then the
                     // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
@@ -356,11 +358,19 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(SweepArgs A, int a_is_g
 #pragma unroll
             for (int da = 0; da < 3; da++) {
                 if (on) {
+                    // the accumulator is indexed [g offset][h offset]; both
+                    // branches keep every index a compile-time constant so
+                    // it stays in registers
+                    if (a_is_g) {
 #pragma unroll
-                    for (int db = 0; db < 3; db++) {
-                        const int dg = a_is_g ? da : db, dh = a_is_g ? db : da;
+                        for (int db = 0; db < 3; db++)
 #pragma unroll
-                        for (int m = 0; m < NC; m++) T[tix(m, db, pa + da, ci)] += acc.D[dg][dh][m];
+                            for (int m = 0; m < NC; m++) T[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
+                    } else {
+#pragma unroll
+                        for (int db = 0; db < 3; db++)
+#pragma unroll
+                            for (int m = 0; m < NC; m++) T[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
                     }
                 }
                 __syncthreads();

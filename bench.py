@@ -102,6 +102,19 @@ def cpu_baseline(budget_s=20.0):
                        f"slab of the full-domain workload, {steps} accepted steps in {el:.1f} s")
 
 
+def copy_calibration():
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 8, "bytes_written": n * 8,
+                                           "launches": 3}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +123,13 @@ def main():
     ap.add_argument("--workload", default="full", choices=["full", "plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB (read) + 1 GiB (write), 3 times: "
+                         "the known-size kernel that HBM byte counters are scaled against")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "
               "supported (the domain is not decomposed across GPUs)", file=sys.stderr)

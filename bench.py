@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 HYP_BYTES_PER_CELL = 112.0
 SRC_BYTES_PER_CELL = 168.0
 SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
-    "sweep_x": "magic::sweep_tile_kernel<{ntr}, true>",
-    "sweep_y": "magic::sweep_tile_kernel<{ntr}, false>",
-    "sweep_z": "magic::sweep_tile_kernel<{ntr}, false>",
+    "sweep_x": "magic::tile_sweep<double, false, {ntr}, true>",
+    "sweep_y": "magic::tile_sweep<double, false, {ntr}, false>",
+    "sweep_z": "magic::tile_sweep<double, false, {ntr}, false>",
 }
 
 

@@ -399,8 +399,8 @@ static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g, int *nl)
             const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
             const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
             if (gx == 0) continue;
-            if (pm) sweep_tile_kernel<NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            else sweep_tile_kernel<NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            if (pm) tile_sweep<double, false, NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            else tile_sweep<double, false, NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
             TRY(launch_check());
             ++*nl;
         }

@@ -1,4 +1,4 @@
-// Shared declarations of the MAGIC device kernels (FP64 form).
+// Shared declarations of the MAGIC device kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -10,9 +10,10 @@ namespace magic {
 // and the "h" transverse direction (hadd, flux3fw.f:545).
 //   x-sweep: n = x, g = y, h = z      y-sweep: n = y, g = z, h = x
 //   z-sweep: n = z, g = x, h = y      (step3.f:161, 329, 499 argument order)
-struct SweepArgs {
-    const double *__restrict__ qold;   // component planes, stride P
-    double *__restrict__ qnew;
+template <typename T>
+struct SweepArgsT {
+    const T *__restrict__ qold;        // component planes, stride P
+    T *__restrict__ qnew;
     const double *__restrict__ gam;    // aux(19) plane
     const double *__restrict__ vgrav;  // aux(17) plane
     long long P;                       // cells per plane
@@ -29,7 +30,12 @@ struct SweepArgs {
     unsigned long long *cfl_bits;      // running max of the Courant number
     int *status;                       // [0] code, [1..4] sweep,i,j,k of first failure
     int i0, j0, k0;                    // Fortran index of (cell 0 along n, a = 0, b = 0) mapping helpers
+    // perturbation form only: background columns indexed by k + mbc - 1
+    // (rho_bar, E_bar, gamma, aux17, and the background's own z jump
+    // p_bar(k) - p_bar(k-1) - .5 (rho_bar(k) + rho_bar(k-1)) aux17(k))
+    const float *crho, *cE, *cgam, *cvg, *cres;
 };
+using SweepArgs = SweepArgsT<double>;
 
 template <int NTR>
 __global__ void sweep_kernel(SweepArgs a);
@@ -37,8 +43,9 @@ __global__ void sweep_kernel(SweepArgs a);
 // Tiled variant: PM = lanes along the pencil (x-sweep), otherwise lanes along
 // the tile axis a (y- and z-sweeps).  a_is_g says whether the tile axis is
 // the sweep's g direction; sA/sB/nA/nB then describe the tile axis a and the
-// coloured axis b.
-template <int NTR, bool PM>
-__global__ void sweep_tile_kernel(SweepArgs a, int a_is_g);
+// coloured axis b.  (T, PERT) = (double, false): FP64 form; (float, true):
+// FP32 perturbation form.
+template <typename T, bool PERT, int NTR, bool PM>
+__global__ void tile_sweep(SweepArgsT<T> a, int a_is_g);
 
 }  // namespace magic

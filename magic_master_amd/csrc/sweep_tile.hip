@@ -1,420 +1,16 @@
-// Tiled directional sweep of the MAGIC unsplit f-wave step (FP64).
-//
-// Same per-interface algebra as sweep.hip (rpn3euX.f, rpt3euX.f, rptt3euX.f,
-// flux3fw.f, step3.f); different data movement:
-//   * a workgroup owns TA pencils that are neighbours along the tile axis "a"
-//     (x-sweep: a = y, 4 pencils, lanes along the pencil; y- and z-sweeps:
-//     a = x, 16 pencils, lanes along x so every access is a 128-byte row) and
-//     marches along the pencils in chunks of CH interfaces (overlap 2);
-//   * the 3x3 increments of each interface (its cells i-1 and i into the
-//     pencil and its 8 transverse neighbours, step3.f:189-226) are summed in
-//     an LDS tile T[b offset][a][cell] in barrier-separated sub-steps (one per
-//     phase and a offset, so no two lanes add to the same entry at once), and
-//     the tile is flushed to q with one coalesced read-modify-write per row;
-//   * tiles are coloured 2 ways along a and 3 ways along the other transverse
-//     axis b, so no two workgroups of a launch flush to the same rows.
-// The result is deterministic.
-#include "magic_dev.h"
-#include "magic_solver.h"
+// FP64 instantiations of the tiled sweep (built without contraction of
+// products and sums into fused multiply-adds, as the reference rounds).
+#include "sweep_tile_body.h"
 
 namespace magic {
 
-template <bool PM>
-struct TileShape;
-template <>
-struct TileShape<true> {   // pencil-major lanes (x-sweep)
-    static constexpr int TA = 4, CH = 64;
-};
-template <>
-struct TileShape<false> {  // tile-axis-major lanes (y- and z-sweeps)
-    static constexpr int TA = 16, CH = 16;
-};
-
-__device__ __forceinline__ void report_t(const SweepArgs &A, int code, int i, int pg, int ph)
-{
-    if (atomicCAS(A.status, 0, code) == 0) {
-        int fi, fj, fk;
-        if (A.sweep == 1) { fi = i; fj = pg; fk = ph; }
-        else if (A.sweep == 2) { fi = ph; fj = i; fk = pg; }
-        else { fi = pg; fj = ph; fk = i; }
-        A.status[1] = A.sweep; A.status[2] = fi; A.status[3] = fj; A.status[4] = fk;
-    }
-}
-
-template <int NTR, bool PM>
-__global__ __launch_bounds__(256) void sweep_tile_kernel(SweepArgs A, int a_is_g)
-{
-    constexpr int NC = 5 + NTR;
-    constexpr int TA = TileShape<PM>::TA, CH = TileShape<PM>::CH, NT = TA * CH;
-    constexpr int NCELL = CH - 1;               // cells S .. S+CH-2 of a chunk
-    constexpr int TROW = TA + 2;                // a offsets A0-1 .. A0+TA
-    constexpr double beta = 0.666666666666666;
-    static_assert(NT == 256, "tile is 256 lanes");
-
-    __shared__ double s_del[5][NT];
-    __shared__ double s_s2[NT];
-    __shared__ double s_trw[NTR > 0 ? NTR : 1][NT];
-    __shared__ double s_cfo[NT];
-    __shared__ int s_flag[NT];
-    __shared__ double s_cprev[TA];
-    __shared__ double T[NC * 3 * TROW * NCELL];
-
-    const int t = threadIdx.x;
-    const int pa = PM ? (t / CH) : (t % TA);    // pencil within the tile
-    const int c = PM ? (t % CH) : (t / TA);     // interface offset within the chunk
-    const int nb = PM ? 1 : TA;                 // lane distance between neighbouring interfaces
-    const int n = A.n;
-    const int A0 = (2 * (int)blockIdx.x + A.cA) * TA;
-    const int a = A0 + pa;                      // tile-axis coordinate of this lane's pencil
-    const int b = A.cB + 3 * (int)blockIdx.y;   // other transverse coordinate
-    const bool pen = a <= A.nA + 1;
-    const int pg = a_is_g ? a : b, ph = a_is_g ? b : a;   // (g, h) coordinates
-    const long long pbase = (long long)(a + A.mbc - 1) * A.sA + (long long)(b + A.mbc - 1) * A.sB;
-    const double dtdn = A.dtdn;
-    const double cg = A.fg, ch = A.fh;
-    const double d = dtdn;
-    const double k6g = (1.0 / 6.0) * d * A.fh;
-    const double k6h = (1.0 / 6.0) * d * A.fg;
-
-    // T index: (m, db, a offset ai, cell ci); the fastest index follows memory
-    auto tix = [&](int m, int db, int ai, int ci) {
-        return PM ? ((m * 3 + db) * TROW + ai) * NCELL + ci : ((m * 3 + db) * NCELL + ci) * TROW + ai;
-    };
-    for (int e = t; e < NC * 3 * TROW * NCELL; e += NT) T[e] = 0.0;
-    if (t < TA) s_cprev[t] = 0.0;
-    double cflmax = 0.0;
-    const int nchunk = (n + 1 + (CH - 2) - 1) / (CH - 2);
-
-    for (int chunk = 0; chunk < nchunk; chunk++) {
-        const int S = chunk * (CH - 2);
-        const int I = S + c;                    // Fortran interface index
-        const bool act1 = pen && I <= n + 2;
-        const bool act2 = act1 && c >= 1 && c <= CH - 2 && I >= 1 && I <= n + 1;
-        __syncthreads();   // previous chunk's readers of s_* and T are done
-
-        // ---------------- part 1: Roe solve of interface I (rpn3euX.f:138-421)
-        Roe R = {0, 0, 0, 0, 1, 0, 0, 0};
-        double fw1[NC], fw2[NC], fw3[NC], amdq[NC], apdq[NC];
-        double s1 = 0, s2 = 0, s3 = 0, c1 = 0, c2 = 0, c3 = 0;
-        bool f1 = false, f2 = false, f5 = false;
-#pragma unroll
-        for (int m = 0; m < NC; m++) { fw1[m] = fw2[m] = fw3[m] = amdq[m] = apdq[m] = 0.0; }
-        if (act1) {
-            const long long ca = pbase + (long long)(I - 1 + A.mbc - 1) * A.sN;
-            const long long cb = ca + A.sN;
-            double qa[NC], qb[NC];
-#pragma unroll
-            for (int m = 0; m < NC; m++) {
-                qa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
-                qb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
-            }
-            const double ga = A.gam[ca], gb = A.gam[cb];
-            if (!(qa[0] > 0.0) || !(qb[0] > 0.0)) report_t(A, 1, I, pg, ph);
-            const double rhsqrtl = sqrt(qa[0]), rhsqrtr = sqrt(qb[0]);
-            const double rhsq2 = rhsqrtl + rhsqrtr;
-            const double gamma = (rhsqrtr * gb + rhsqrtl * ga) / rhsq2;
-            const double gamma1 = gamma - 1.0;
-            const double pl = (ga - 1.0) * (qa[4] - 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
-            const double pr = (gb - 1.0) * (qb[4] - 0.5 * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
-            R.u = (qa[1] / rhsqrtl + qb[1] / rhsqrtr) / rhsq2;
-            R.v = (qa[2] / rhsqrtl + qb[2] / rhsqrtr) / rhsq2;
-            R.w = (qa[3] / rhsqrtl + qb[3] / rhsqrtr) / rhsq2;
-            R.enth = ((qa[4] + pl) / rhsqrtl + (qb[4] + pr) / rhsqrtr) / rhsq2;
-            R.u2v2w2 = R.u * R.u + R.v * R.v + R.w * R.w;
-            const double a2 = gamma1 * (R.enth - 0.5 * R.u2v2w2);
-            if (!(a2 > 0.0)) report_t(A, 2, I, pg, ph);
-            R.a = sqrt(a2);
-            R.g1a2 = gamma1 / a2;
-            R.euv = R.enth - R.u2v2w2;
-            const double ur = qb[1] / qb[0], ul = qa[1] / qa[0];
-            const double d1 = qb[1] - qa[1];
-            double d2 = (qb[1] * ur + pr) - (qa[1] * ul + pl);
-            const double d3 = ur * qb[2] - ul * qa[2];
-            const double d4 = ur * qb[3] - ul * qa[3];
-            double d5 = (qb[4] + pr) * ur - (qa[4] + pl) * ul;
-            if (A.gravity) {
-                const double vg = A.vgrav[cb];
-                d2 = d2 - 0.5 * (qb[0] + qa[0]) * vg;
-                d5 = d5 - 0.5 * (qb[1] + qa[1]) * vg;
-            }
-            const double b4 = R.g1a2 * (R.euv * d1 + R.u * d2 + R.v * d3 + R.w * d4 - d5);
-            const double b2 = d3 - R.v * d1;
-            const double b3 = d4 - R.w * d1;
-            const double b5 = (d2 + (R.a - R.u) * d1 - R.a * b4) / (2.0 * R.a);
-            const double b1 = d1 - b4 - b5;
-            fw1[0] = b1; fw1[1] = b1 * (R.u - R.a); fw1[2] = b1 * R.v; fw1[3] = b1 * R.w;
-            fw1[4] = b1 * (R.enth - R.u * R.a);
-            s1 = R.u - R.a;
-            fw2[0] = b4; fw2[1] = b4 * R.u; fw2[2] = b2 + b4 * R.v; fw2[3] = b3 + b4 * R.w;
-            fw2[4] = b2 * R.v + b3 * R.w + b4 * 0.5 * R.u2v2w2;
-            s2 = R.u;
-            fw3[0] = b5; fw3[1] = b5 * (R.u + R.a); fw3[2] = b5 * R.v; fw3[3] = b5 * R.w;
-            fw3[4] = b5 * (R.enth + R.u * R.a);
-            s3 = R.u + R.a;
-#pragma unroll
-            for (int m = 5; m < NC; m++)
-                fw2[m] = (m - 4 <= A.nspecadv) ? s2 * (qb[m] - qa[m]) : ur * qb[m] - ul * qa[m];
-            // entropy fix and amdq: rpn3euX.f:316-398
-            {
-                const double gmm = ga, gm1 = ga - 1.0;
-                const double rhoim1 = qa[0];
-                const double pim1 = gm1 * (qa[4] - 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / rhoim1);
-                const double cim1 = sqrt(gmm * pim1 / rhoim1);
-                const double s0 = qa[1] / rhoim1 - cim1;
-                if (!(s0 >= 0.0 && s1 > 0.0)) {
-                    const double rho1 = qa[0] + fw1[0] / s1;
-                    const double rhou1 = qa[1] + fw1[1] / s1;
-                    const double rhov1 = qa[2] + fw1[2] / s1;
-                    const double rhow1 = qa[3] + fw1[3] / s1;
-                    const double en1 = qa[4] + fw1[4] / s1;
-                    const double p1 = gm1 * (en1 - 0.5 * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
-                    const double cc1 = sqrt(gmm * p1 / rho1);
-                    const double sa = rhou1 / rho1 - cc1;
-                    double sfract;
-                    if (s0 < 0.0 && sa > 0.0) sfract = s0 * (sa - s1) / (sa - s0);
-                    else if (s1 < 0.0) sfract = s1;
-                    else sfract = 0.0;
-#pragma unroll
-                    for (int m = 0; m < NC; m++) amdq[m] = sfract * fw1[m] / s1;
-                    if (s2 < 0.0) {
-#pragma unroll
-                        for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + fw2[m];
-                        const double gm = gb, g1 = gb - 1.0;
-                        const double rhoi = qb[0];
-                        const double pi_ = g1 * (qb[4] - 0.5 * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / rhoi);
-                        const double ci = sqrt(gm * pi_ / rhoi);
-                        const double s3r = qb[1] / rhoi + ci;
-                        const double rho2 = qb[0] - fw3[0] / s3;
-                        const double rhou2 = qb[1] - fw3[1] / s3;
-                        const double rhov2 = qb[2] - fw3[2] / s3;
-                        const double rhow2 = qb[3] - fw3[3] / s3;
-                        const double en2 = qb[4] - fw3[4] / s3;
-                        const double p2 = g1 * (en2 - 0.5 * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
-                        const double cc2 = sqrt(gm * p2 / rho2);
-                        const double sb = rhou2 / rho2 + cc2;
-                        bool add = true;
-                        if (sb < 0.0 && s3r > 0.0) sfract = sb * (s3r - s3) / (s3r - sb);
-                        else if (s3 < 0.0) sfract = s3;
-                        else add = false;
-                        if (add) {
-#pragma unroll
-                            for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + sfract * fw3[m] / s3;
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < NC; m++) apdq[m] = ((0.0 + fw1[m]) + fw2[m]) + fw3[m] - amdq[m];
-            c1 = fabs(s1 * dtdn); c2 = fabs(s2 * dtdn); c3 = fabs(s3 * dtdn);
-            const double b2r = fw2[2] - b4 * R.v;
-            f1 = b1 != 0.0; f2 = b2r != 0.0; f5 = b5 != 0.0;
-            s_del[0][t] = d1; s_del[1][t] = d2; s_del[2][t] = d3; s_del[3][t] = d4; s_del[4][t] = d5;
-            s_s2[t] = s2;
-#pragma unroll
-            for (int k = 0; k < NTR; k++) s_trw[k][t] = fw2[5 + k];
-            s_flag[t] = (act2 && (f1 || f2 || f5)) ? 1 : 0;
-            s_cfo[t] = f5 ? c3 : (f2 ? c2 : c1);
-        } else {
-            s_flag[t] = 0;
-        }
-        __syncthreads();
-
-        // ---------------- part 2: limiters and cqxx (rpn3euX.f:434-676, flux3fw.f:270-284)
-        double cqxx[NC];
-#pragma unroll
-        for (int m = 0; m < NC; m++) cqxx[m] = 0.0;
-        if (act2) {
-#pragma unroll
-            for (int k = 0; k < NTR; k++) {
-                const double fc = fw2[5 + k];
-                if (fc != 0.0) {
-                    const double r = (s2 > 0.0) ? s_trw[k][t - nb] / fc : s_trw[k][t + nb] / fc;
-                    double wl;
-                    if (k + 1 <= A.nspecadv) {
-                        const int up = (int)(-copysign(1.0, s2));      // Id(2,i) - i
-                        wl = cflsupbee(0.5, beta, r, c2, fabs(s_s2[t + up * nb] * dtdn));
-                    } else {
-                        wl = fmaxF(0.0, fminF(1.0, r));
-                    }
-                    fw2[5 + k] = wl * fc;
-                }
-            }
-            const double b1 = fw1[0], b4 = fw2[0];
-            const double b2 = fw2[2] - b4 * R.v, b3 = fw2[3] - b4 * R.w, b5 = fw3[0];
-            double wl1 = 1.0, wl2 = 1.0, wl3 = 1.0, wl4 = 1.0, wl5 = 1.0;
-            auto del = [&](int k, int up) { return s_del[k][t + up * nb]; };
-            if (b1 != 0.0) {
-                const int up = (int)(-copysign(1.0, s1));
-                const double b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                              R.w * del(3, up) - del(4, up));
-                const double b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) / (2.0 * R.a);
-                const double b1Id = del(0, up) - b4Id - b5Id;
-                wl1 = suppow(b1Id / b1, c1, c1);
-            }
-            if (b2 != 0.0) {
-                const int up = (int)(-copysign(1.0, s2));
-                const double b2Id = del(2, up) - R.v * del(0, up);
-                wl2 = cflsupbee(1.0, beta, b2Id / b2, c2, c2);
-            }
-            if (b3 != 0.0 || b4 != 0.0) {
-                double cflup;
-                if (f2) cflup = c2;
-                else if (f1) cflup = c1;
-                else {
-                    cflup = s_cprev[pa];
-                    for (int j = c - 1; j >= 1; j--)
-                        if (s_flag[t - (c - j) * nb]) { cflup = s_cfo[t - (c - j) * nb]; break; }
-                }
-                const int up = (int)(-copysign(1.0, s2));
-                if (b3 != 0.0) {
-                    const double b3Id = del(3, up) - R.w * del(0, up);
-                    wl3 = cflsupbee(1.0, beta, b3Id / b3, c2, cflup);
-                }
-                if (b4 != 0.0) {
-                    const double b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                                  R.w * del(3, up) - del(4, up));
-                    wl4 = cflsupbee(1.0, beta, b4Id / b4, c2, cflup);
-                }
-            }
-            if (b5 != 0.0) {
-                const int up = (int)(-copysign(1.0, s3));
-                const double b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                              R.w * del(3, up) - del(4, up));
-                const double b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) / (2.0 * R.a);
-                wl5 = suppow(b5Id / b5, c3, c3);
-            }
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                fw1[m] = fw1[m] * wl1;
-                fw3[m] = fw3[m] * wl5;
-            }
-            const double b2l = wl2 * b2, b3l = wl3 * b3, b4l = wl4 * b4;
-            fw2[0] = b4l;
-            fw2[1] = b4l * R.u;
-            fw2[2] = b2l + b4l * R.v;
-            fw2[3] = b3l + b4l * R.w;
-            fw2[4] = b2l * R.v + b3l * R.w + b4l * 0.5 * R.u2v2w2;
-            const double k1 = 0.5 * copysign(1.0, s1) * (1.0 - fabs(s1) * dtdn);
-            const double k2 = 0.5 * copysign(1.0, s2) * (1.0 - fabs(s2) * dtdn);
-            const double k3 = 0.5 * copysign(1.0, s3) * (1.0 - fabs(s3) * dtdn);
-#pragma unroll
-            for (int m = 0; m < NC; m++) cqxx[m] = k1 * fw1[m] + k2 * fw2[m] + k3 * fw3[m];
-            cflmax = fmaxF(cflmax, fmaxF(fmaxF(c1, c2), c3));
-        }
-        // ---------------- part 3: transverse terms, two phases
-        // Each phase recomputes the two cqxx splits it needs, in an order that
-        // keeps at most four split vectors alive next to the accumulator.
-        for (int phase = 0; phase < 2; phase++) {
-            // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
-            const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
-            Acc<NC> acc;
-            acc.zero();
-            if (on) {
-                const double sgn = phase == 0 ? 1.0 : -1.0;   // +3 cqxx mixing for amdq, -3 for apdq
-                const double cc = phase == 0 ? -d : d;        // sign of the correction-wave term
-                double asd[NC];
-#pragma unroll
-                for (int m = 0; m < NC; m++) asd[m] = phase == 0 ? amdq[m] : apdq[m];
-                {
-                    // C splits (h direction): 2-D part, correction, then the
-                    // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
-                    double Cm[NC], Cp[NC], Cmq[NC], Cpq[NC];
-                    tsplit<NC, 3, false>(R, cqxx, Cmq, Cpq);
-                    tsplit<NC, 3, false>(R, asd, Cm, Cp);
-                    acc.h(1, 0, -0.5 * d * ch, Cm); acc.h(2, 0, -0.5 * d * ch, Cp);
-                    acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
-                    axpy3(Cp, Cp, 3.0 * sgn, Cpq);
-                    axpy3(Cm, Cm, 3.0 * sgn, Cmq);
-                    tsplit<NC, 2, true>(R, Cp, Cmq, Cpq);      // B-, B+ of c+asdq2
-                    acc.g(2, 0, k6g * cg, Cpq);  acc.g(1, 0, k6g * cg, Cmq);
-                    acc.g(2, 1, -k6g * cg, Cpq); acc.g(1, 1, -k6g * cg, Cmq);
-                    tsplit<NC, 2, true>(R, Cm, Cmq, Cpq);      // B-, B+ of c-asdq2
-                    acc.g(2, 0, -k6g * cg, Cpq); acc.g(1, 0, -k6g * cg, Cmq);
-                    acc.g(2, -1, k6g * cg, Cpq); acc.g(1, -1, k6g * cg, Cmq);
-                }
-                {
-                    // B splits (g direction): 2-D part, correction, then the
-                    // h-direction corner terms of b*asdq +- 3 b*cqxx (flux3fw.f:510-538)
-                    double Bm[NC], Bp[NC], Bmq[NC], Bpq[NC];
-                    tsplit<NC, 2, false>(R, cqxx, Bmq, Bpq);
-                    tsplit<NC, 2, false>(R, asd, Bm, Bp);
-                    acc.g(1, 0, -0.5 * d * cg, Bm); acc.g(2, 0, -0.5 * d * cg, Bp);
-                    acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
-                    axpy3(Bp, Bp, 3.0 * sgn, Bpq);
-                    axpy3(Bm, Bm, 3.0 * sgn, Bmq);
-                    tsplit<NC, 3, true>(R, Bp, Bmq, Bpq);      // C-, C+ of b+asdq
-                    acc.h(2, 0, k6h * ch, Bpq);  acc.h(1, 0, k6h * ch, Bmq);
-                    acc.h(2, 1, -k6h * ch, Bpq); acc.h(1, 1, -k6h * ch, Bmq);
-                    tsplit<NC, 3, true>(R, Bm, Bmq, Bpq);      // C-, C+ of b-asdq
-                    acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
-                    acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
-                }
-#pragma unroll
-                for (int m = 0; m < NC; m++)
-                    acc.D[1][1][m] += (phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
-            }
-            const int ci = phase == 0 ? c - 1 : c;  // cell index within the chunk's tile
-            // one sub-step per offset along a: lanes then add to distinct T entries
-#pragma unroll
-            for (int da = 0; da < 3; da++) {
-                if (on) {
-                    // the accumulator is indexed [g offset][h offset]; both
-                    // branches keep every index a compile-time constant so
-                    // it stays in registers
-                    if (a_is_g) {
-#pragma unroll
-                        for (int db = 0; db < 3; db++)
-#pragma unroll
-                            for (int m = 0; m < NC; m++) T[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
-                    } else {
-#pragma unroll
-                        for (int db = 0; db < 3; db++)
-#pragma unroll
-                            for (int m = 0; m < NC; m++) T[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
-                    }
-                }
-                __syncthreads();
-            }
-        }
-
-        // ---------------- flush the tile to q (one read-modify-write per row)
-        for (int e = t; e < NC * 3 * TROW * NCELL; e += NT) {
-            int ai, ci, db, m;
-            if (PM) {
-                ci = e % NCELL; int r = e / NCELL; ai = r % TROW; r /= TROW; db = r % 3; m = r / 3;
-            } else {
-                ai = e % TROW; int r = e / TROW; ci = r % NCELL; r /= NCELL; db = r % 3; m = r / 3;
-            }
-            const int cell = S + ci;                // Fortran index along the pencil
-            const int aa = A0 - 1 + ai;             // tile-axis coordinate of the target row
-            const double v = T[e];
-            T[e] = 0.0;
-            if (cell >= 1 && cell <= n && aa <= A.nA + 2) {
-                const long long off = (long long)(cell + A.mbc - 1) * A.sN + (long long)(aa + A.mbc - 1) * A.sA +
-                                      (long long)(b + db - 1 + A.mbc - 1) * A.sB;
-                double *p = A.qnew + (long long)A.comp[m] * A.P + off;
-                *p = *p + v;
-            }
-        }
-        // CFLup carried into the next chunk: last assignment among this chunk's interfaces
-        if (t < TA) {
-            for (int j = CH - 2; j >= 1; j--) {
-                const int tt = PM ? t * CH + j : j * TA + t;
-                if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) cflmax = fmaxF(cflmax, __shfl_xor(cflmax, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(A.cfl_bits, (unsigned long long)__double_as_longlong(cflmax));
-}
-
-template __global__ void sweep_tile_kernel<0, true>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<0, false>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<1, true>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<1, false>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<2, true>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<2, false>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<5, true>(SweepArgs, int);
-template __global__ void sweep_tile_kernel<5, false>(SweepArgs, int);
+template __global__ void tile_sweep<double, false, 0, true>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 0, false>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 1, true>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 1, false>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 2, true>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 2, false>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 5, true>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 5, false>(SweepArgsT<double>, int);
 
 }  // namespace magic

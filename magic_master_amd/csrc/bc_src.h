@@ -4,8 +4,10 @@
 
 namespace magic {
 
-__device__ __forceinline__ double fminD(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double fmaxD(double a, double b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T fminD(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T fmaxD(T a, T b) { return a > b ? a : b; }
 
 struct Dims {
     int mx, my, mz, mbc;
@@ -27,21 +29,28 @@ struct Forcing {
     double tgauss;        // exp(-.5*((t-tcenter)**2/twidth**2)), host FP64
 };
 
-__global__ void k_bc_x(Dims d, double *q, int nc, int lo, int hi);
-__global__ void k_bc_y(Dims d, double *q, int nc, int lo, int hi);
+template <typename T>
+__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi);
+template <typename T>
+__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi);
 __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
                        const double *aux2, const double *aux3, const double *aux5,
                        const double *zratio);
-__global__ void k_clamp(double *q, long long P, int nc);
+template <typename T>
+__global__ void k_clamp(T *q, long long P, int nc);
 __global__ void k_src_convert(long long P, double *q, const Aux a);
-__global__ void k_zghost_uvw(Dims d, double *q);
-__global__ void k_diff_cross(Dims d, double *q, const double *u, const double *coef,
-                             double dxy, double dxz, double dyz);
-__global__ void k_copy_interior(Dims d, double *dst, const double *src, int nplanes);
-__global__ void k_conduct_zghost(Dims d, double *u);
-__global__ void k_conduct_zspill(Dims d, double *u, int S);
-__global__ void k_pass(Dims d, double *out, const double *src, const double *ghost,
-                       const double *coef, double f, int axis);
+template <typename T>
+__global__ void k_zghost_uvw(Dims d, T *q);
+template <typename T, bool COLC>
+__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz);
+template <typename T>
+__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes);
+template <typename T>
+__global__ void k_conduct_zghost(Dims d, T *u);
+template <typename T>
+__global__ void k_conduct_zspill(Dims d, T *u, int S);
+template <typename T, bool COLC>
+__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis);
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
 __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, int nc);

@@ -1,60 +1,18 @@
 // Boundary (bc3), pre-step (b4step3) and source-pass (src3, diffuse,
-// conduct) kernels of the MAGIC step, FP64, on component planes.
+// conduct) kernels of the MAGIC step, FP64 form, on component planes.  The
+// kernels shared with the FP32 form live in stencil_body.h and are
+// instantiated for double at the end of this file.
 //
 // Index convention: a cell with Fortran indices (i, j, k) in
 // [1-mbc, mx+mbc] x [1-mbc, my+mbc] x [1-mbc, mz+mbc] lives at
 // ((k+mbc-1)*NY + (j+mbc-1))*NX + (i+mbc-1) of every plane.
 #include "magic_dev.h"
-#include "bc_src.h"
+#include "stencil_body.h"
 
 namespace magic {
 
-__device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
-{
-    return ((long long)(k + d.mbc - 1) * d.NY + (j + d.mbc - 1)) * d.NX + (i + d.mbc - 1);
-}
-
 // ---------------------------------------------------------------- bc3 ----
-// x faces: bc3_mpi.f:97-312.  One thread per (j, k) of the padded face.
-__global__ void k_bc_x(Dims d, double *q, int nc, int lo, int hi)
-{
-    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int k = (int)blockIdx.y + 1 - d.mbc;
-    if (j > d.my + d.mbc || k > d.mz + d.mbc) return;
-    const bool inner = j >= 1 && j <= d.my && k >= 1 && k <= d.mz;
-    for (int m = 0; m < nc; m++) {
-        double *p = q + (long long)m * d.P;
-        for (int ibc = 1; ibc <= d.mbc; ibc++) {
-            if (lo == 1) p[cid(d, 1 - ibc, j, k)] = p[cid(d, 1, j, k)];
-            else if (lo == 3) p[cid(d, 1 - ibc, j, k)] = (m == 1 ? -1.0 : 1.0) * p[cid(d, ibc, j, k)];
-            else if (lo == 2 && inner) p[cid(d, 1 - ibc, j, k)] = p[cid(d, d.mx - d.mbc + (d.mbc + 1 - ibc), j, k)];
-            if (hi == 1) p[cid(d, d.mx + ibc, j, k)] = p[cid(d, d.mx, j, k)];
-            else if (hi == 3) p[cid(d, d.mx + ibc, j, k)] = (m == 1 ? -1.0 : 1.0) * p[cid(d, d.mx + 1 - ibc, j, k)];
-            else if (hi == 2 && inner) p[cid(d, d.mx + ibc, j, k)] = p[cid(d, ibc, j, k)];
-        }
-    }
-}
-
-// y faces: bc3_mpi.f:320-536.  One thread per (i, k).
-__global__ void k_bc_y(Dims d, double *q, int nc, int lo, int hi)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int k = (int)blockIdx.y + 1 - d.mbc;
-    if (i > d.mx + d.mbc || k > d.mz + d.mbc) return;
-    const bool inner = k >= 1 && k <= d.mz;
-    for (int m = 0; m < nc; m++) {
-        double *p = q + (long long)m * d.P;
-        for (int jbc = 1; jbc <= d.mbc; jbc++) {
-            if (lo == 1) p[cid(d, i, 1 - jbc, k)] = p[cid(d, i, 1, k)];
-            else if (lo == 3) p[cid(d, i, 1 - jbc, k)] = (m == 2 ? -1.0 : 1.0) * p[cid(d, i, jbc, k)];
-            else if (lo == 2 && inner) p[cid(d, i, 1 - jbc, k)] = p[cid(d, i, d.my - d.mbc + (d.mbc + 1 - jbc), k)];
-            if (hi == 1) p[cid(d, i, d.my + jbc, k)] = p[cid(d, i, d.my, k)];
-            else if (hi == 3) p[cid(d, i, d.my + jbc, k)] = (m == 2 ? -1.0 : 1.0) * p[cid(d, i, d.my + 1 - jbc, k)];
-            else if (hi == 2 && inner) p[cid(d, i, d.my + jbc, k)] = p[cid(d, i, jbc, k)];
-        }
-    }
-}
-
+// x, y faces and the tracer clamp: stencil_body.h.
 // z faces: bc3_mpi.f:544-809.  Extrapolation and wall keep the perturbation
 // from the background scaled by sc = (rho_bar_ghost/rho_bar_1)**0.5, which
 // the host precomputes (zratio[side][kbc-1][plane index of the (i,j) column]).
@@ -100,17 +58,6 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
     }
 }
 
-// b4step3.f:35-45: clamp tracers to [0, 1] on every cell.
-__global__ void k_clamp(double *q, long long P, int nc)
-{
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= P) return;
-    for (int m = 5; m < nc; m++) {
-        const double v = q[m * P + c];
-        q[m * P + c] = fmaxD(0.0, fminD(v, 1.0));
-    }
-}
-
 // ---------------------------------------------------------------- src3 ---
 // conserved -> (rho, u', v', w', T') on every cell: src3.f90:242-304
 __global__ void k_src_convert(long long P, double *q, const Aux a)
@@ -128,123 +75,6 @@ __global__ void k_src_convert(long long P, double *q, const Aux a)
     q[P + c] = q2 / q1 - a2 / a1;
     q[2 * P + c] = q3 / q1 - a3 / a1;
     q[3 * P + c] = q4 / q1 - a4 / a1;
-}
-
-// diffuse.f:48-54: z ghosts of q(2:4) from the first/last interior layer.
-__global__ void k_zghost_uvw(Dims d, double *q)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
-    if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
-    for (int m = 1; m <= 3; m++) {
-        double *p = q + (long long)m * d.P;
-        const double lo = p[cid(d, i, j, 1)], hi = p[cid(d, i, j, d.mz)];
-        for (int kbc = 1; kbc <= d.mbc; kbc++) {
-            p[cid(d, i, j, 1 - kbc)] = lo;
-            p[cid(d, i, j, d.mz + kbc)] = hi;
-        }
-    }
-}
-
-// Interior thread mapping: x fastest.
-#define INTERIOR_IJK                                                          \
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;           \
-    const int j = (int)blockIdx.y + 1;                                        \
-    const int k = (int)blockIdx.z + 1;                                        \
-    if (i > d.mx) return;                                                     \
-    const long long c = cid(d, i, j, k);
-
-// diffuse.f:70-118: cross-derivative terms from u (copy of q(2:4) taken
-// after the z-ghost fill); writes q(2:4) on the interior.
-__global__ void k_diff_cross(Dims d, double *q, const double *u, const double *coef,
-                             double dxy, double dxz, double dyz)
-{
-    INTERIOR_IJK
-    const long long sx = 1, sy = d.NX, sz = (long long)d.NX * d.NY;
-    const double *u1 = u, *u2 = u + d.P, *u3 = u + 2 * d.P;
-    auto X = [&](const double *f, long long a, long long b) {
-        return f[c + a + b] - f[c - a + b] + f[c - a - b] - f[c + a - b];
-    };
-    const double ctxxy = X(u2, sx, sy), ctxxz = X(u3, sx, sz);
-    const double ctyxy = X(u1, sx, sy), ctyyz = X(u3, sy, sz);
-    const double ctzxz = X(u1, sx, sz), ctzyz = X(u2, sy, sz);
-    const double co = coef[c];
-    q[d.P + c] = u1[c] + co * (dxy * ctxxy + dxz * ctxxz);
-    q[2 * d.P + c] = u2[c] + co * (dxy * ctyxy + dyz * ctyyz);
-    q[3 * d.P + c] = u3[c] + co * (dxz * ctzxz + dyz * ctzyz);
-}
-
-// interior copy of planes src -> dst (n planes with stride P)
-__global__ void k_copy_interior(Dims d, double *dst, const double *src, int nplanes)
-{
-    INTERIOR_IJK
-    for (int m = 0; m < nplanes; m++) dst[m * d.P + c] = src[m * d.P + c];
-}
-
-// One explicit directional diffusion pass (diffuse.f:153-208, conduct.f:262-323):
-//   out = u + f*coef*(second difference of u along axis),
-// computed on the interior.  The interior values of u come from src and the
-// values on the ghost layers (which the reference copies once per call and
-// never refreshes between passes) from ghost; src may equal ghost.  The
-// second difference keeps the reference's operand order: (up-mid-mid+down)
-// along x and y, (down-mid-mid+up) along z.
-__global__ void k_pass(Dims d, double *out, const double *src, const double *ghost,
-                       const double *coef, double f, int axis)
-{
-    INTERIOR_IJK
-    long long s;
-    bool lo_in, hi_in;
-    if (axis == 0) { s = 1; lo_in = i > 1; hi_in = i < d.mx; }
-    else if (axis == 1) { s = d.NX; lo_in = j > 1; hi_in = j < d.my; }
-    else { s = (long long)d.NX * d.NY; lo_in = k > 1; hi_in = k < d.mz; }
-    const double aq = src[c];
-    const double am = lo_in ? src[c - s] : ghost[c - s];
-    const double ap = hi_in ? src[c + s] : ghost[c + s];
-    const double qt = axis == 2 ? (am - aq - aq + ap) : (ap - aq - aq + am);
-    out[c] = aq + f * coef[c] * qt;
-}
-
-// conduct.f:86-91: z ghosts of u, for j in [1-mbc, my+mbc] and i in
-// [1-mbc, mz+mbc] -- the reference bounds i by mz, not mx.  When mx > mz the
-// columns i > mz+mbc keep their stale z ghosts (which then enter the z
-// stencil of the interior cells of those columns).  When mz > mx the
-// out-of-row i addresses the array linearly: within a layer the writes land
-// on later rows of the same layer with those rows' own values (what the
-// in-row fill writes anyway), so this kernel fills whole rows and
-// k_conduct_zspill adds the writes that cross into the next layer.
-__global__ void k_conduct_zghost(Dims d, double *u)
-{
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
-    const int ilast = (d.mz < d.mx ? d.mz : d.mx) + d.mbc;
-    if (i > ilast || j > d.my + d.mbc) return;
-    const double lo = u[cid(d, i, j, 1)], hi = u[cid(d, i, j, d.mz)];
-    for (int kbc = 1; kbc <= d.mbc; kbc++) {
-        u[cid(d, i, j, 1 - kbc)] = lo;
-        u[cid(d, i, j, d.mz + kbc)] = hi;
-    }
-}
-
-// The layer-crossing writes of conduct.f:86-91 when mz > mx.  The rows at
-// the end of a layer address i up to mz+mbc, S = mz-mx elements past the
-// layer's last cell, so at in-layer offsets r < S the loop writes
-//   layer 2-kbc <- layer 2          (u(i,j,1-kbc) = u(i,j,1), kbc = 1..mbc)
-//   layer mz+1+kbc <- layer mz+1    (u(i,j,mz+kbc) = u(i,j,mz))
-// after the in-row fills of the rows those offsets fall in (the host checks
-// that the rows written lie before the rows that write them).  Writes past
-// the last layer leave the reference's array; they are dropped here.
-// Layers are Fortran k indices; r counts from cell (1-mbc, 1-mbc).
-__global__ void k_conduct_zspill(Dims d, double *u, int S)
-{
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (r >= S) return;
-    const long long NXY = (long long)d.NX * d.NY;
-    auto L = [&](int k) { return (long long)(k + d.mbc - 1) * NXY + r; };
-    const double lo = u[L(2)], hi = u[L(d.mz + 1)];
-    for (int kbc = 1; kbc <= d.mbc; kbc++) {
-        u[L(2 - kbc)] = lo;
-        if (d.mz + 1 + kbc <= d.mz + d.mbc) u[L(d.mz + 1 + kbc)] = hi;
-    }
 }
 
 // src3.f90:380-418 (reassembly), 430-445 / 514-607 (forcing), 616-626
@@ -320,5 +150,18 @@ __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, 
     if (c >= P) return;
     for (int m = 0; m < nc; m++) aos[c * nc + m] = planes[m * P + c];
 }
+
+// FP64 instantiations of the shared kernels
+template __global__ void k_bc_x<double>(Dims, double *, int, int, int);
+template __global__ void k_bc_y<double>(Dims, double *, int, int, int);
+template __global__ void k_clamp<double>(double *, long long, int);
+template __global__ void k_zghost_uvw<double>(Dims, double *);
+template __global__ void k_diff_cross<double, false>(Dims, double *, const double *, const double *, double,
+                                                      double, double);
+template __global__ void k_copy_interior<double>(Dims, double *, const double *, int);
+template __global__ void k_pass<double, false>(Dims, double *, const double *, const double *, const double *,
+                                               double, int);
+template __global__ void k_conduct_zghost<double>(Dims, double *);
+template __global__ void k_conduct_zspill<double>(Dims, double *, int);
 
 }  // namespace magic

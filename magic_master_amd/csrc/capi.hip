@@ -567,7 +567,7 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->u, c->q + P, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
+        k_diff_cross<double, false><<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
                                                  dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz));
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
@@ -581,9 +581,9 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdy2 = dtdy2 / fourthirds;
             }
             double *uc = c->u + (cc - 1) * P, *qc = c->q + cc * P;
-            k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0);
-            k_pass<<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1);
-            k_pass<<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2);
+            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0);
+            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1);
+            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2);
             TRY(launch_check());
         }
         // conduct: conduct.f:245-332.  ut = q(5) on every cell once; the
@@ -594,18 +594,18 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         double *q5 = c->q + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut, q5, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0);
-            k_pass<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1);
+            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0);
+            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1);
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut);
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S);
             }
             if (k < ndtrc) {
-                k_pass<<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2);
+                k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2);
                 k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1);
             } else {
-                k_pass<<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2);
+                k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2);
             }
             TRY(launch_check());
         }

@@ -1,0 +1,195 @@
+// Boundary, pre-step and diffusion/conduction stencil kernels shared by the
+// FP64 form (T = double, coefficients as 3-D planes) and the FP32
+// perturbation form (T = float, coefficients as background columns, COLC).
+// bc_src.hip and bc_src32.hip instantiate them under their own
+// floating-point contraction settings.
+//
+// Index convention: a cell with Fortran indices (i, j, k) in
+// [1-mbc, mx+mbc] x [1-mbc, my+mbc] x [1-mbc, mz+mbc] lives at
+// ((k+mbc-1)*NY + (j+mbc-1))*NX + (i+mbc-1) of every plane.
+#pragma once
+#include "bc_src.h"
+
+namespace magic {
+
+__device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
+{
+    return ((long long)(k + d.mbc - 1) * d.NY + (j + d.mbc - 1)) * d.NX + (i + d.mbc - 1);
+}
+
+// x faces: bc3_mpi.f:97-312.  One thread per (j, k) of the padded face.
+template <typename T>
+__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int k = (int)blockIdx.y + 1 - d.mbc;
+    if (j > d.my + d.mbc || k > d.mz + d.mbc) return;
+    const bool inner = j >= 1 && j <= d.my && k >= 1 && k <= d.mz;
+    for (int m = 0; m < nc; m++) {
+        T *p = q + (long long)m * d.P;
+        for (int ibc = 1; ibc <= d.mbc; ibc++) {
+            if (lo == 1) p[cid(d, 1 - ibc, j, k)] = p[cid(d, 1, j, k)];
+            else if (lo == 3) p[cid(d, 1 - ibc, j, k)] = (m == 1 ? T(-1.0) : T(1.0)) * p[cid(d, ibc, j, k)];
+            else if (lo == 2 && inner) p[cid(d, 1 - ibc, j, k)] = p[cid(d, d.mx - d.mbc + (d.mbc + 1 - ibc), j, k)];
+            if (hi == 1) p[cid(d, d.mx + ibc, j, k)] = p[cid(d, d.mx, j, k)];
+            else if (hi == 3) p[cid(d, d.mx + ibc, j, k)] = (m == 1 ? T(-1.0) : T(1.0)) * p[cid(d, d.mx + 1 - ibc, j, k)];
+            else if (hi == 2 && inner) p[cid(d, d.mx + ibc, j, k)] = p[cid(d, ibc, j, k)];
+        }
+    }
+}
+
+// y faces: bc3_mpi.f:320-536.  One thread per (i, k).
+template <typename T>
+__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int k = (int)blockIdx.y + 1 - d.mbc;
+    if (i > d.mx + d.mbc || k > d.mz + d.mbc) return;
+    const bool inner = k >= 1 && k <= d.mz;
+    for (int m = 0; m < nc; m++) {
+        T *p = q + (long long)m * d.P;
+        for (int jbc = 1; jbc <= d.mbc; jbc++) {
+            if (lo == 1) p[cid(d, i, 1 - jbc, k)] = p[cid(d, i, 1, k)];
+            else if (lo == 3) p[cid(d, i, 1 - jbc, k)] = (m == 2 ? T(-1.0) : T(1.0)) * p[cid(d, i, jbc, k)];
+            else if (lo == 2 && inner) p[cid(d, i, 1 - jbc, k)] = p[cid(d, i, d.my - d.mbc + (d.mbc + 1 - jbc), k)];
+            if (hi == 1) p[cid(d, i, d.my + jbc, k)] = p[cid(d, i, d.my, k)];
+            else if (hi == 3) p[cid(d, i, d.my + jbc, k)] = (m == 2 ? T(-1.0) : T(1.0)) * p[cid(d, i, d.my + 1 - jbc, k)];
+            else if (hi == 2 && inner) p[cid(d, i, d.my + jbc, k)] = p[cid(d, i, jbc, k)];
+        }
+    }
+}
+
+// b4step3.f:35-45: clamp tracers to [0, 1] on every cell.
+template <typename T>
+__global__ void k_clamp(T *q, long long P, int nc)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P) return;
+    for (int m = 5; m < nc; m++) {
+        const T v = q[m * P + c];
+        q[m * P + c] = fmaxD(T(0.0), fminD(v, T(1.0)));
+    }
+}
+
+// diffuse.f:48-54: z ghosts of q(2:4) from the first/last interior layer.
+template <typename T>
+__global__ void k_zghost_uvw(Dims d, T *q)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)blockIdx.y + 1 - d.mbc;
+    if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
+    for (int m = 1; m <= 3; m++) {
+        T *p = q + (long long)m * d.P;
+        const T lo = p[cid(d, i, j, 1)], hi = p[cid(d, i, j, d.mz)];
+        for (int kbc = 1; kbc <= d.mbc; kbc++) {
+            p[cid(d, i, j, 1 - kbc)] = lo;
+            p[cid(d, i, j, d.mz + kbc)] = hi;
+        }
+    }
+}
+
+// Interior thread mapping: x fastest.
+#define INTERIOR_IJK                                                          \
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;           \
+    const int j = (int)blockIdx.y + 1;                                        \
+    const int k = (int)blockIdx.z + 1;                                        \
+    if (i > d.mx) return;                                                     \
+    const long long c = cid(d, i, j, k);
+
+// diffuse.f:70-118: cross-derivative terms from u (copy of q(2:4) taken
+// after the z-ghost fill); writes q(2:4) on the interior.
+template <typename T, bool COLC>
+__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz)
+{
+    INTERIOR_IJK
+    const long long sx = 1, sy = d.NX, sz = (long long)d.NX * d.NY;
+    const T *u1 = u, *u2 = u + d.P, *u3 = u + 2 * d.P;
+    auto X = [&](const T *f, long long a, long long b) {
+        return f[c + a + b] - f[c - a + b] + f[c - a - b] - f[c + a - b];
+    };
+    const T ctxxy = X(u2, sx, sy), ctxxz = X(u3, sx, sz);
+    const T ctyxy = X(u1, sx, sy), ctyyz = X(u3, sy, sz);
+    const T ctzxz = X(u1, sx, sz), ctzyz = X(u2, sy, sz);
+    const T co = coef[COLC ? (long long)(k + d.mbc - 1) : c];
+    q[d.P + c] = u1[c] + co * (dxy * ctxxy + dxz * ctxxz);
+    q[2 * d.P + c] = u2[c] + co * (dxy * ctyxy + dyz * ctyyz);
+    q[3 * d.P + c] = u3[c] + co * (dxz * ctzxz + dyz * ctzyz);
+}
+
+// interior copy of planes src -> dst (n planes with stride P)
+template <typename T>
+__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes)
+{
+    INTERIOR_IJK
+    for (int m = 0; m < nplanes; m++) dst[m * d.P + c] = src[m * d.P + c];
+}
+
+// One explicit directional diffusion pass (diffuse.f:153-208, conduct.f:262-323):
+//   out = u + f*coef*(second difference of u along axis),
+// computed on the interior.  The interior values of u come from src and the
+// values on the ghost layers (which the reference copies once per call and
+// never refreshes between passes) from ghost; src may equal ghost.  The
+// second difference keeps the reference's operand order: (up-mid-mid+down)
+// along x and y, (down-mid-mid+up) along z.
+template <typename T, bool COLC>
+__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis)
+{
+    INTERIOR_IJK
+    long long s;
+    bool lo_in, hi_in;
+    if (axis == 0) { s = 1; lo_in = i > 1; hi_in = i < d.mx; }
+    else if (axis == 1) { s = d.NX; lo_in = j > 1; hi_in = j < d.my; }
+    else { s = (long long)d.NX * d.NY; lo_in = k > 1; hi_in = k < d.mz; }
+    const T aq = src[c];
+    const T am = lo_in ? src[c - s] : ghost[c - s];
+    const T ap = hi_in ? src[c + s] : ghost[c + s];
+    const T qt = axis == 2 ? (am - aq - aq + ap) : (ap - aq - aq + am);
+    out[c] = aq + f * coef[COLC ? (long long)(k + d.mbc - 1) : c] * qt;
+}
+
+// conduct.f:86-91: z ghosts of u, for j in [1-mbc, my+mbc] and i in
+// [1-mbc, mz+mbc] -- the reference bounds i by mz, not mx.  When mx > mz the
+// columns i > mz+mbc keep their stale z ghosts (which then enter the z
+// stencil of the interior cells of those columns).  When mz > mx the
+// out-of-row i addresses the array linearly: within a layer the writes land
+// on later rows of the same layer with those rows' own values (what the
+// in-row fill writes anyway), so this kernel fills whole rows and
+// k_conduct_zspill adds the writes that cross into the next layer.
+template <typename T>
+__global__ void k_conduct_zghost(Dims d, T *u)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)blockIdx.y + 1 - d.mbc;
+    const int ilast = (d.mz < d.mx ? d.mz : d.mx) + d.mbc;
+    if (i > ilast || j > d.my + d.mbc) return;
+    const T lo = u[cid(d, i, j, 1)], hi = u[cid(d, i, j, d.mz)];
+    for (int kbc = 1; kbc <= d.mbc; kbc++) {
+        u[cid(d, i, j, 1 - kbc)] = lo;
+        u[cid(d, i, j, d.mz + kbc)] = hi;
+    }
+}
+
+// The layer-crossing writes of conduct.f:86-91 when mz > mx.  The rows at
+// the end of a layer address i up to mz+mbc, S = mz-mx elements past the
+// layer's last cell, so at in-layer offsets r < S the loop writes
+//   layer 2-kbc <- layer 2          (u(i,j,1-kbc) = u(i,j,1), kbc = 1..mbc)
+//   layer mz+1+kbc <- layer mz+1    (u(i,j,mz+kbc) = u(i,j,mz))
+// after the in-row fills of the rows those offsets fall in (the host checks
+// that the rows written lie before the rows that write them).  Writes past
+// the last layer leave the reference's array; they are dropped here.
+// Layers are Fortran k indices; r counts from cell (1-mbc, 1-mbc).
+template <typename T>
+__global__ void k_conduct_zspill(Dims d, T *u, int S)
+{
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= S) return;
+    const long long NXY = (long long)d.NX * d.NY;
+    auto L = [&](int k) { return (long long)(k + d.mbc - 1) * NXY + r; };
+    const T lo = u[L(2)], hi = u[L(d.mz + 1)];
+    for (int kbc = 1; kbc <= d.mbc; kbc++) {
+        u[L(2 - kbc)] = lo;
+        if (d.mz + 1 + kbc <= d.mz + d.mbc) u[L(d.mz + 1 + kbc)] = hi;
+    }
+}
+
+}  // namespace magic

@@ -53,6 +53,21 @@ typedef struct {
 int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux,
                  double xlower, double ylower, double zlower,
                  double dx, double dy, double dz, magic_ctx **out);
+
+/* State forms.  MAGIC_FORM_FP64 reproduces the reference's FP64 step.
+ * MAGIC_FORM_FP32P stores q as FP32 perturbations from a background column
+ * at rest (aux slots 1-5, 11, 12, 17-19 independent of i and j, momenta 0)
+ * and runs the Riemann solves, limiters, transverse splits, increment
+ * accumulation and source pass in FP32; the Courant-number reduction and
+ * the background stay FP64.  Both forms take and return the reference's
+ * FP64 q and aux arrays.  No counterpart in the reference (a precision mode
+ * of the same operators). */
+#define MAGIC_FORM_FP64   0
+#define MAGIC_FORM_FP32P  1
+/* magic_create with a state form; magic_create(...) is form MAGIC_FORM_FP64. */
+int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int maux,
+                      double xlower, double ylower, double zlower,
+                      double dx, double dy, double dz, int form, magic_ctx **out);
 int magic_destroy(magic_ctx *ctx);
 const char *magic_strerror(int status);
 
@@ -63,7 +78,9 @@ int magic_set_params(magic_ctx *ctx, const magic_params *p);
 /* Upload the background aux (setaux output, maux slots) from host memory.
  * Only the slots the step reads are kept on the device (1-5, 11, 12, 17,
  * 18, 19).  Also precomputes the z-boundary density ratios of bc3
- * ((rho_bar_ghost/rho_bar_1)**0.5, bc3_mpi.f:560-573) in FP64. */
+ * ((rho_bar_ghost/rho_bar_1)**0.5, bc3_mpi.f:560-573) in FP64.
+ * MAGIC_FORM_FP32P keeps one column and returns MAGIC_EUNSUPPORTED when the
+ * background is not a column at rest. */
 int magic_set_aux(magic_ctx *ctx, const double *aux);
 /* Upload / download q (host, reference layout, ghost cells included). */
 int magic_set_q(magic_ctx *ctx, const double *q);

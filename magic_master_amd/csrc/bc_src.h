@@ -33,6 +33,25 @@ template <typename T>
 __global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi);
 template <typename T>
 __global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi);
+// Stephan & Alexander convective heating profile (src3.f90:534-572) for a
+// precipitation rate precipw > 0.1 at height zcell (m): K/s.
+__device__ __forceinline__ double sa_heating(double precipw, double zcell)
+{
+    const double pi = 3.141592653589793;
+    const double th = 10.79 + 0.05 * precipw, ph = 6.93 - 0.03 * precipw;
+    const double hamp = -0.0001 + 0.0025 * precipw;
+    const double tc = 1.12 + 0.10 * precipw, pc = 0.63 + 0.02 * precipw;
+    const double bc = 0.37 + 0.00 * precipw, camp = -0.0003 - 0.0005 * precipw;
+    const double z2 = zcell / 1000.0;
+    double heat;
+    if ((bc <= z2) && (z2 <= pc)) heat = camp * sin((2.0 * pi / (4.0 * (pc - bc))) * (z2 - bc));
+    else if ((pc <= z2) && (z2 <= tc)) heat = -camp * sin((2.0 * pi / (4.0 * (tc - pc))) * (z2 - tc));
+    else if ((pc <= z2) && (z2 <= ph)) heat = hamp * sin((2.0 * pi / (4.0 * (ph - tc))) * (z2 - tc));
+    else if ((ph <= z2) && (z2 <= th)) heat = -hamp * sin((2.0 * pi / (4.0 * (th - ph))) * (z2 - th));
+    else heat = 0.0;
+    return heat;
+}
+
 __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
                        const double *aux2, const double *aux3, const double *aux5,
                        const double *zratio);
@@ -53,6 +72,24 @@ template <typename T, bool COLC>
 __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis);
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
+
+// ---- FP32 perturbation form (bc_src32.hip) ----
+// Background columns at rest, indexed by k + mbc - 1: rho_bar, E_bar,
+// gamma (aux19), R (aux18).
+struct Col32 {
+    const float *rho, *E, *gam, *R;
+};
+// bc3 z-face factors (rho_bar_ghost/rho_bar_boundary)**0.5 per side and layer
+struct Zr32 {
+    float s[2][8];
+};
+__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr);
+__global__ void k32_src_convert(Dims d, float *q, Col32 col);
+__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f);
+__global__ void k32_aos_to_pert(Dims d, float *q, const double *aos, int nc, const double *bgrho,
+                                const double *bgE);
+__global__ void k32_pert_to_aos(Dims d, double *aos, const float *q, int nc, const double *bgrho,
+                                const double *bgE);
 __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, int nc);
 
 }  // namespace magic

@@ -109,19 +109,7 @@ __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f)
                           f.tgauss;
         const double precipw = f.amplitude * g2;
         if (precipw > 0.1) {
-            // Stephan & Alexander heating profile (src3.f90:534-572)
-            const double pi = 3.141592653589793;
-            const double th = 10.79 + 0.05 * precipw, ph = 6.93 - 0.03 * precipw;
-            const double hamp = -0.0001 + 0.0025 * precipw;
-            const double tc = 1.12 + 0.10 * precipw, pc = 0.63 + 0.02 * precipw;
-            const double bc = 0.37 + 0.00 * precipw, camp = -0.0003 - 0.0005 * precipw;
-            const double z2 = zcell / 1000.0;
-            double heat;
-            if ((bc <= z2) && (z2 <= pc)) heat = camp * sin((2.0 * pi / (4.0 * (pc - bc))) * (z2 - bc));
-            else if ((pc <= z2) && (z2 <= tc)) heat = -camp * sin((2.0 * pi / (4.0 * (tc - pc))) * (z2 - tc));
-            else if ((pc <= z2) && (z2 <= ph)) heat = hamp * sin((2.0 * pi / (4.0 * (ph - tc))) * (z2 - tc));
-            else if ((ph <= z2) && (z2 <= th)) heat = -hamp * sin((2.0 * pi / (4.0 * (th - ph))) * (z2 - th));
-            else heat = 0.0;
+            const double heat = sa_heating(precipw, zcell);
             const double qtemp = gamma1 * (q5 - 0.5 * (1.0 / q1) * (q2 * q2 + q3 * q3 + q4 * q4)) / (q1 * R);
             double t5 = qtemp - auxtemp;
             t5 = t5 + f.dt * heat;

@@ -1,0 +1,156 @@
+// Boundary and source-pass kernels of the FP32 perturbation form.
+//
+// The state is q' = q - q_bar on component planes (float), with a
+// background column at rest: q_bar = (rho_bar(k), 0, 0, 0, E_bar(k)) and
+// p_bar = (gamma-1) E_bar.  Tracers are stored as they are (their background
+// is 0).  Every formula below is the FP64 form's operator (bc_src.hip, the
+// reference's bc3_mpi.f / src3.f90) rewritten so that no FP32 operation
+// subtracts two background-sized numbers.  Products and sums may contract
+// into fused multiply-adds: this form's error bar is a fraction of the field
+// peak against the FP64 form.
+#pragma clang fp contract(fast)
+#include "magic_dev.h"
+#include "stencil_body.h"
+
+namespace magic {
+
+// FP32 instantiations of the shared kernels (coefficients as columns)
+template __global__ void k_bc_x<float>(Dims, float *, int, int, int);
+template __global__ void k_bc_y<float>(Dims, float *, int, int, int);
+template __global__ void k_clamp<float>(float *, long long, int);
+template __global__ void k_zghost_uvw<float>(Dims, float *);
+template __global__ void k_diff_cross<float, true>(Dims, float *, const float *, const float *, float, float,
+                                                   float);
+template __global__ void k_copy_interior<float>(Dims, float *, const float *, int);
+template __global__ void k_pass<float, true>(Dims, float *, const float *, const float *, const float *, float,
+                                             int);
+template __global__ void k_conduct_zghost<float>(Dims, float *);
+template __global__ void k_conduct_zspill<float>(Dims, float *, int);
+
+// z faces (bc3_mpi.f:544-809).  The FP64 form writes
+//   q_ghost = aux_ghost + (q_boundary - aux_boundary) * sc
+// for rho, the horizontal momenta and E, and q_ghost = q_boundary * sc for
+// the vertical momentum; with a background at rest every one of them is
+// q'_ghost = q'_boundary * sc.  sc depends on the layer only.
+__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)blockIdx.y + 1 - d.mbc;
+    if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
+    for (int side = 0; side < 2; side++) {
+        const int mth = side == 0 ? lo : hi;
+        if (mth == 1 || mth == 3) {
+            const int kb = side == 0 ? 1 : d.mz;
+            const long long cb = cid(d, i, j, kb);
+            for (int kbc = 1; kbc <= d.mbc; kbc++) {
+                const int kg = side == 0 ? 1 - kbc : d.mz + kbc;
+                const long long cg = cid(d, i, j, kg);
+                const float sc = zr.s[side][kbc - 1];
+                for (int m = 0; m < 5; m++) q[m * d.P + cg] = q[m * d.P + cb] * sc;
+                for (int m = 5; m < nc; m++) q[m * d.P + cg] = q[m * d.P + cb];
+            }
+            if (mth == 3)
+                for (int kbc = 1; kbc <= d.mbc; kbc++) {
+                    const int kg = side == 0 ? 1 - kbc : d.mz + kbc;
+                    const int km = side == 0 ? kbc : d.mz + 1 - kbc;
+                    q[3 * d.P + cid(d, i, j, kg)] = -q[3 * d.P + cid(d, i, j, km)];
+                }
+        } else if (mth == 2) {
+            for (int kbc = 1; kbc <= d.mbc; kbc++)
+                for (int m = 0; m < nc; m++) {
+                    float *p = q + (long long)m * d.P;
+                    if (side == 0) p[cid(d, i, j, kbc - d.mbc)] = p[cid(d, i, j, d.mz - d.mbc + kbc)];
+                    else p[cid(d, i, j, d.mz + kbc)] = p[cid(d, i, j, kbc)];
+                }
+        }
+    }
+}
+
+// All-cells thread mapping over the padded grid: x fastest.
+#define PADDED_IJK                                                            \
+    const int ip = (int)(blockIdx.x * blockDim.x + threadIdx.x);              \
+    if (ip >= d.NX) return;                                                   \
+    const int kk = (int)blockIdx.z;                                           \
+    const long long c = ((long long)kk * d.NY + blockIdx.y) * d.NX + ip;
+
+// conserved perturbation -> (rho', u, v, w, T') on every cell
+// (src3.f90:242-304 with u_bar = 0):
+//   T' = T - T_bar = (p' - rho' (gamma-1) E_bar / rho_bar) / (rho R),
+//   p' = (gamma-1) (E' - |m|^2 / (2 rho)),  rho = rho_bar + rho'.
+__global__ void k32_src_convert(Dims d, float *q, Col32 col)
+{
+    PADDED_IJK
+    const long long P = d.P;
+    const float rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0f, R = col.R[kk];
+    const float rp = q[c], rho = rb + rp;
+    const float m2 = q[P + c], m3 = q[2 * P + c], m4 = q[3 * P + c], ep = q[4 * P + c];
+    const float pp = g1 * (ep - 0.5f * (m2 * m2 + m3 * m3 + m4 * m4) / rho);
+    q[4 * P + c] = (pp - rp * (g1 * Eb / rb)) / (rho * R);
+    q[P + c] = m2 / rho;
+    q[2 * P + c] = m3 / rho;
+    q[3 * P + c] = m4 / rho;
+}
+
+// Reassembly, plume forcing and tracer clamp on the interior
+// (src3.f90:380-418, 430-445, 514-607, 616-626):
+//   m = u rho,  E' = E_bar rho'/rho_bar + T' rho R/(gamma-1) + |m|^2/(2 rho),
+// the heating adding dt*heat to T' first.  Positions and the forcing are
+// evaluated in FP64.
+__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f)
+{
+    INTERIOR_IJK
+    const long long P = d.P;
+    const int kk = k + d.mbc - 1;
+    const float rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0f, R = col.R[kk];
+    const float rp = q[c], rho = rb + rp;
+    const float m2 = q[P + c] * rho, m3 = q[2 * P + c] * rho;
+    float m4 = q[3 * P + c] * rho;
+    float tp = q[4 * P + c];
+    const double xcell = f.xlower + (i - 0.5) * f.dx - f.t * f.vsrcx;
+    const double ycell = f.ylower + (j - 0.5) * f.dy;
+    const double zcell = f.zlower + (k - 0.5) * f.dz;
+    if (f.forcemth == 1) {
+        const double gau = exp(-.5 * ((xcell - f.xpos) * (xcell - f.xpos) / (f.xwidth * f.xwidth))) *
+                           exp(-.5 * ((ycell - f.ypos) * (ycell - f.ypos) / (f.ywidth * f.ywidth))) *
+                           exp(-.5 * ((zcell - f.zpos) * (zcell - f.zpos) / (f.zwidth * f.zwidth))) *
+                           exp(-.5 * ((f.t - f.tcenter) * (f.t - f.tcenter) / (f.twidth * f.twidth)));
+        const double ffz = sin(f.omega * (f.t - f.tcenter) - f.propx * (xcell - f.xpos) -
+                               f.propz * (ycell - f.ypos));
+        m4 = m4 + (float)(f.dt * f.amplitude * gau * ffz) * rho;
+    } else if (f.forcemth == 3 && !(zcell > 15000.0)) {
+        const double g2 = exp(-.5 * ((xcell - f.xpos) * (xcell - f.xpos) / (f.xwidth * f.xwidth))) *
+                          exp(-.5 * ((ycell - f.ypos) * (ycell - f.ypos) / (f.ywidth * f.ywidth))) *
+                          f.tgauss;
+        const double precipw = f.amplitude * g2;
+        if (precipw > 0.1) tp = tp + (float)(f.dt * sa_heating(precipw, zcell));
+    }
+    q[P + c] = m2;
+    q[2 * P + c] = m3;
+    q[3 * P + c] = m4;
+    q[4 * P + c] = Eb * (rp / rb) + tp * rho * R / g1 + 0.5f * (m2 * m2 + m3 * m3 + m4 * m4) / rho;
+    for (int m = 5; m < nc; m++) q[m * P + c] = fmaxD(0.0f, fminD(q[m * P + c], 1.0f));
+}
+
+// reference AoS q(meqn, i, j, k) (FP64, full state) <-> perturbation planes:
+// the background is subtracted / added in FP64, then rounded.
+__global__ void k32_aos_to_pert(Dims d, float *q, const double *aos, int nc, const double *bgrho,
+                                const double *bgE)
+{
+    PADDED_IJK
+    for (int m = 0; m < nc; m++) {
+        const double bg = m == 0 ? bgrho[kk] : (m == 4 ? bgE[kk] : 0.0);
+        q[(long long)m * d.P + c] = (float)(aos[c * nc + m] - bg);
+    }
+}
+
+__global__ void k32_pert_to_aos(Dims d, double *aos, const float *q, int nc, const double *bgrho,
+                                const double *bgE)
+{
+    PADDED_IJK
+    for (int m = 0; m < nc; m++) {
+        const double bg = m == 0 ? bgrho[kk] : (m == 4 ? bgE[kk] : 0.0);
+        aos[c * nc + m] = bg + (double)q[(long long)m * d.P + c];
+    }
+}
+
+}  // namespace magic

@@ -45,6 +45,7 @@ struct TimedSpan {
 };
 
 struct magic_ctx {
+    int form = MAGIC_FORM_FP64;
     int mx, my, mz, meqn, mbc, maux;
     double xlower, ylower, zlower, dx, dy, dz;
     Dims d;
@@ -52,6 +53,12 @@ struct magic_ctx {
     hipStream_t stream = nullptr;
     double *q = nullptr, *qold = nullptr, *aux = nullptr, *zratio = nullptr;
     double *stage = nullptr, *u = nullptr, *qt = nullptr, *ut = nullptr;
+    // FP32 perturbation form: state planes and scratch, background columns
+    // (float: rho, E, gamma, R, aux17, z residual, nu, kappa; FP64: rho, E)
+    float *q32 = nullptr, *qold32 = nullptr, *u32 = nullptr, *qt32 = nullptr, *ut32 = nullptr;
+    float *col32 = nullptr;
+    double *bg64 = nullptr;
+    Zr32 zr32;
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     unsigned long long *hscal = nullptr;   // pinned mirror
     magic_params prm;
@@ -144,16 +151,26 @@ extern "C" const char *magic_strerror(int s)
     }
 }
 
+enum { C32_RHO = 0, C32_E, C32_GAM, C32_R, C32_VG, C32_RES, C32_NU, C32_KAP, NC32 };
+
 extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux, double xlower,
                             double ylower, double zlower, double dx, double dy, double dz,
                             magic_ctx **out)
 {
+    return magic_create_form(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz,
+                             MAGIC_FORM_FP64, out);
+}
+
+extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int maux, double xlower,
+                                 double ylower, double zlower, double dx, double dy, double dz, int form,
+                                 magic_ctx **out)
+{
     if (!out) return MAGIC_EARG;
     *out = nullptr;
+    if (form != MAGIC_FORM_FP64 && form != MAGIC_FORM_FP32P) return MAGIC_EARG;
+    if (form == MAGIC_FORM_FP32P && mbc > 8) return MAGIC_EUNSUPPORTED;
     if (mx < 1 || my < 1 || mz < 1 || mbc < 2 || meqn < 5 || meqn > 10 || maux < 19) return MAGIC_EARG;
     if (!sweep_supported(meqn - 5)) return MAGIC_EUNSUPPORTED;
-    const int nmax = mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
-    if (nmax + 3 > 512) return MAGIC_EUNSUPPORTED;   // one workgroup per pencil
     if (mz > mx) {
         // conduct.f:86-91 writes S = mz-mx elements past each layer; the
         // device replays those writes after the in-row fills, which is the
@@ -163,6 +180,7 @@ extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux,
         if (2 * ((S - 1) / NX) + 1 >= NY) return MAGIC_EUNSUPPORTED;
     }
     magic_ctx *c = new magic_ctx();
+    c->form = form;
     c->mx = mx; c->my = my; c->mz = mz; c->meqn = meqn; c->mbc = mbc; c->maux = maux;
     c->xlower = xlower; c->ylower = ylower; c->zlower = zlower; c->dx = dx; c->dy = dy; c->dz = dz;
     c->d.mx = mx; c->d.my = my; c->d.mz = mz; c->d.mbc = mbc;
@@ -171,16 +189,40 @@ extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux,
     const size_t P = (size_t)c->P;
     const size_t NXY = (size_t)c->d.NX * c->d.NY;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc(&c->stage, P * meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->dscal, 8 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipHostMalloc(&c->hscal, 8 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    if (ok && form == MAGIC_FORM_FP32P) {
+        const size_t NZ = (size_t)c->d.NZ;
+        ok = hipMalloc(&c->q32, P * meqn * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->qold32, P * meqn * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->u32, 3 * P * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->qt32, P * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->ut32, P * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->col32, NC32 * NZ * sizeof(float)) == hipSuccess &&
+             hipMalloc(&c->bg64, 2 * NZ * sizeof(double)) == hipSuccess;
+        ok = ok && hipMemsetAsync(c->q32, 0, P * meqn * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->qold32, 0, P * meqn * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->u32, 0, 3 * P * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->qt32, 0, P * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->ut32, 0, P * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->col32, 0, NC32 * NZ * sizeof(float), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->bg64, 0, 2 * NZ * sizeof(double), c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+        if (!ok) {
+            magic_destroy(c);
+            return MAGIC_ENOMEM;
+        }
+        *out = c;
+        return MAGIC_OK;
+    }
     ok = ok && hipMalloc(&c->q, P * meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->qold, P * meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->aux, P * NAUXP * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->zratio, 2 * mbc * NXY * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&c->stage, P * meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->u, 3 * P * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->qt, P * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->ut, P * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&c->dscal, 8 * sizeof(unsigned long long)) == hipSuccess;
-    ok = ok && hipHostMalloc(&c->hscal, 8 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
     if (ok) {
         ok = hipMemsetAsync(c->q, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
              hipMemsetAsync(c->qold, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
@@ -207,6 +249,8 @@ extern "C" int magic_destroy(magic_ctx *c)
     for (auto e : c->pool) hipEventDestroy(e);
     hipFree(c->q); hipFree(c->qold); hipFree(c->aux); hipFree(c->zratio); hipFree(c->stage);
     hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal);
+    hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
+    hipFree(c->col32); hipFree(c->bg64);
     if (c->hscal) hipHostFree(c->hscal);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -221,9 +265,73 @@ extern "C" int magic_set_params(magic_ctx *c, const magic_params *p)
     return MAGIC_OK;
 }
 
+// FP32 form: the background must be one column at rest.  Keeps float
+// columns of rho_bar, E_bar, gamma, R, aux17, nu, kappa, the background's own
+// z jump of the well-balanced f-wave (p_bar(k) - p_bar(k-1) - .5 (rho_bar(k)
+// + rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
+// evaluates it for a state at rest; zero to FP64 rounding), FP64 rho_bar and
+// E_bar for the conversions, and the bc3 z factors per layer.
+static int set_aux32(magic_ctx *c, const double *aux)
+{
+    static const int slot[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
+    const int NX = c->d.NX, NY = c->d.NY, NZ = c->d.NZ, mbc = c->mbc, mz = c->mz, maux = c->maux;
+    std::vector<double> col((size_t)NAUXP * NZ);
+    for (int kk = 0; kk < NZ; kk++) {
+        const size_t c0 = (size_t)kk * NY * NX;
+        for (int s = 0; s < NAUXP; s++) {
+            const double v0 = aux[c0 * maux + slot[s] - 1];
+            if ((s == AX_MX || s == AX_MY || s == AX_MZ) && v0 != 0.0) {
+                fprintf(stderr, "magic_hip: FP32 form needs a background at rest (aux slot %d = %g)\n",
+                        slot[s], v0);
+                return MAGIC_EUNSUPPORTED;
+            }
+            for (size_t e = 0; e < (size_t)NY * NX; e++)
+                if (aux[(c0 + e) * maux + slot[s] - 1] != v0) {
+                    fprintf(stderr, "magic_hip: FP32 form needs a background column (aux slot %d varies "
+                                    "in layer %d)\n", slot[s], kk + 1 - mbc);
+                    return MAGIC_EUNSUPPORTED;
+                }
+            col[(size_t)s * NZ + kk] = v0;
+        }
+    }
+    auto cv = [&](int s, int kk) { return col[(size_t)s * NZ + kk]; };
+    std::vector<float> h((size_t)NC32 * NZ, 0.0f);
+    std::vector<double> b64((size_t)2 * NZ);
+    for (int kk = 0; kk < NZ; kk++) {
+        h[C32_RHO * NZ + kk] = (float)cv(AX_RHO, kk);
+        h[C32_E * NZ + kk] = (float)cv(AX_E, kk);
+        h[C32_GAM * NZ + kk] = (float)cv(AX_GAM, kk);
+        h[C32_R * NZ + kk] = (float)cv(AX_R, kk);
+        h[C32_VG * NZ + kk] = (float)cv(AX_VG, kk);
+        h[C32_NU * NZ + kk] = (float)cv(AX_NU, kk);
+        h[C32_KAP * NZ + kk] = (float)cv(AX_KAP, kk);
+        if (kk > 0) {
+            const double pb = (cv(AX_GAM, kk) - 1.0) * (cv(AX_E, kk) - 0.5 * 0.0 / cv(AX_RHO, kk));
+            const double pa = (cv(AX_GAM, kk - 1) - 1.0) * (cv(AX_E, kk - 1) - 0.5 * 0.0 / cv(AX_RHO, kk - 1));
+            h[C32_RES * NZ + kk] = (float)((pb - pa) - 0.5 * (cv(AX_RHO, kk) + cv(AX_RHO, kk - 1)) * cv(AX_VG, kk));
+        }
+        b64[kk] = cv(AX_RHO, kk);
+        b64[NZ + kk] = cv(AX_E, kk);
+    }
+    for (int side = 0; side < 2; side++)
+        for (int kbc = 1; kbc <= mbc; kbc++) {
+            const int kg = side == 0 ? 1 - kbc : mz + kbc, kb = side == 0 ? 1 : mz;
+            c->zr32.s[side][kbc - 1] = (float)std::pow(cv(AX_RHO, kg + mbc - 1) / cv(AX_RHO, kb + mbc - 1), 0.5);
+        }
+    HIPCHECK(hipMemcpyAsync(c->col32, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->bg64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->aux11_top = cv(AX_NU, mz + mbc - 1);
+    c->aux12_top = cv(AX_KAP, mz + mbc - 1);
+    c->have_aux = true;
+    return MAGIC_OK;
+}
+
 extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
 {
     if (!c || !aux) return MAGIC_EARG;
+    if (c->form == MAGIC_FORM_FP32P) return set_aux32(c, aux);
     static const int slot[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
     const long long P = c->P;
     const int maux = c->maux;
@@ -261,10 +369,26 @@ extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
     return MAGIC_OK;
 }
 
+static dim3 padded_grid(const magic_ctx *c)
+{
+    return dim3(blocks(c->d.NX, 64), c->d.NY, c->d.NZ);
+}
+
 extern "C" int magic_set_q(magic_ctx *c, const double *q)
 {
     if (!c || !q) return MAGIC_EARG;
     const size_t n = (size_t)c->P * c->meqn;
+    if (c->form == MAGIC_FORM_FP32P) {
+        if (!c->have_aux) return MAGIC_EARG;   // the perturbation needs the background
+        const int NZ = c->d.NZ;
+        HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        k32_aos_to_pert<<<padded_grid(c), 64, 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn, c->bg64,
+                                                               c->bg64 + NZ);
+        TRY(launch_check());
+        HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return MAGIC_OK;
+    }
     HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     k_aos_to_planes<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->stage, c->P, c->meqn);
     TRY(launch_check());
@@ -277,6 +401,15 @@ extern "C" int magic_get_q(magic_ctx *c, double *q)
 {
     if (!c || !q) return MAGIC_EARG;
     const size_t n = (size_t)c->P * c->meqn;
+    if (c->form == MAGIC_FORM_FP32P) {
+        const int NZ = c->d.NZ;
+        k32_pert_to_aos<<<padded_grid(c), 64, 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn, c->bg64,
+                                                               c->bg64 + NZ);
+        TRY(launch_check());
+        HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        return MAGIC_OK;
+    }
     k_planes_to_aos<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->stage, c->q, c->P, c->meqn);
     TRY(launch_check());
     HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -293,6 +426,24 @@ static int bc3_launch(magic_ctx *c, const int *mthbc)
     const Dims &d = c->d;
     const double *A = c->aux;
     auto lohi = [](int m) { return m < 0 ? 0 : m; };
+    if (c->form == MAGIC_FORM_FP32P) {
+        if (mthbc[0] >= 0 || mthbc[1] >= 0) {
+            k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
+                                                                     lohi(mthbc[1]));
+            TRY(launch_check());
+        }
+        if (mthbc[2] >= 0 || mthbc[3] >= 0) {
+            k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
+                                                                     lohi(mthbc[3]));
+            TRY(launch_check());
+        }
+        if (mthbc[4] >= 0 || mthbc[5] >= 0) {
+            k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
+                                                                       lohi(mthbc[5]), c->zr32);
+            TRY(launch_check());
+        }
+        return MAGIC_OK;
+    }
     if (mthbc[0] >= 0 || mthbc[1] >= 0) {
         k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, c->stream>>>(d, c->q, c->meqn, lohi(mthbc[0]),
                                                                  lohi(mthbc[1]));
@@ -325,7 +476,8 @@ static int b4step3_launch(magic_ctx *c)
 {
     if (c->meqn > 5) {
         Span sp(c, TC_OTHER, 1);
-        k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->P, c->meqn);
+        if (c->form == MAGIC_FORM_FP32P) k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q32, c->P, c->meqn);
+        else k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->P, c->meqn);
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -339,18 +491,25 @@ extern "C" int magic_b4step3(magic_ctx *c)
     return MAGIC_OK;
 }
 
-static int copy_q(magic_ctx *c, double *dst, const double *src)
+// copyq3: q -> backup (to_backup) or backup -> q
+static int copy_state(magic_ctx *c, bool to_backup)
 {
     Span sp(c, TC_OTHER, 1);
-    HIPCHECK(hipMemcpyAsync(dst, src, (size_t)c->P * c->meqn * sizeof(double), hipMemcpyDeviceToDevice,
-                            c->stream));
+    const size_t n = (size_t)c->P * c->meqn;
+    if (c->form == MAGIC_FORM_FP32P) {
+        HIPCHECK(hipMemcpyAsync(to_backup ? c->qold32 : c->q32, to_backup ? c->q32 : c->qold32,
+                                n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        HIPCHECK(hipMemcpyAsync(to_backup ? c->qold : c->q, to_backup ? c->q : c->qold, n * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+    }
     return MAGIC_OK;
 }
 
 extern "C" int magic_backup(magic_ctx *c)
 {
     if (!c) return MAGIC_EARG;
-    TRY(copy_q(c, c->qold, c->q));
+    TRY(copy_state(c, true));
     HIPCHECK(hipStreamSynchronize(c->stream));
     return MAGIC_OK;
 }
@@ -358,7 +517,7 @@ extern "C" int magic_backup(magic_ctx *c)
 extern "C" int magic_restore(magic_ctx *c)
 {
     if (!c) return MAGIC_EARG;
-    TRY(copy_q(c, c->q, c->qold));
+    TRY(copy_state(c, false));
     HIPCHECK(hipStreamSynchronize(c->stream));
     return MAGIC_OK;
 }
@@ -372,6 +531,7 @@ __global__ void k_zero_scalars(unsigned long long *s)
 template <int NTR>
 static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
 {
+    if (a.n + 3 > 512) return MAGIC_EUNSUPPORTED;   // one workgroup per pencil
     const int NI = a.n + 3;
     const int threads = ((NI + 63) / 64) * 64;
     const size_t smem = (size_t)(7 + NTR) * NI * sizeof(double) + (size_t)NI * sizeof(int);
@@ -387,8 +547,8 @@ static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
     return MAGIC_OK;
 }
 
-template <int NTR>
-static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g, int *nl)
+template <typename T, bool PERT, int NTR>
+static int launch_tile(magic_ctx *c, SweepArgsT<T> a, bool pm, int a_is_g, int *nl)
 {
     const int TA = pm ? 4 : 16;
     const int ntiles = (a.nA + 2 + TA - 1) / TA;
@@ -399,21 +559,22 @@ static int launch_tile(magic_ctx *c, SweepArgs a, bool pm, int a_is_g, int *nl)
             const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
             const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
             if (gx == 0) continue;
-            if (pm) tile_sweep<double, false, NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            else tile_sweep<double, false, NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            if (pm) tile_sweep<T, PERT, NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            else tile_sweep<T, PERT, NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
             TRY(launch_check());
             ++*nl;
         }
     return MAGIC_OK;
 }
 
-static int tile_dispatch(magic_ctx *c, const SweepArgs &a, bool pm, int a_is_g, int *nl)
+template <typename T, bool PERT>
+static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, bool pm, int a_is_g, int *nl)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<0>(c, a, pm, a_is_g, nl);
-    case 1: return launch_tile<1>(c, a, pm, a_is_g, nl);
-    case 2: return launch_tile<2>(c, a, pm, a_is_g, nl);
-    case 5: return launch_tile<5>(c, a, pm, a_is_g, nl);
+    case 0: return launch_tile<T, PERT, 0>(c, a, pm, a_is_g, nl);
+    case 1: return launch_tile<T, PERT, 1>(c, a, pm, a_is_g, nl);
+    case 2: return launch_tile<T, PERT, 2>(c, a, pm, a_is_g, nl);
+    case 5: return launch_tile<T, PERT, 5>(c, a, pm, a_is_g, nl);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -435,24 +596,14 @@ static int sweep_dispatch(magic_ctx *c, const SweepArgs &a, int *nl)
     }
 }
 
-static int step3_launch(magic_ctx *c, double dt)
+// The three sweeps of step3 (lib/step3.f:121-621) on argument block a,
+// whose state pointers and form-specific fields are already set.
+template <typename T, bool PERT>
+static int step3_sweeps(magic_ctx *c, SweepArgsT<T> a, double dt)
 {
     const Dims &d = c->d;
     const long long NXY = (long long)d.NX * d.NY;
-    k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);
-    TRY(launch_check());
-    SweepArgs a;
-    memset(&a, 0, sizeof(a));
-    a.qold = c->qold;
-    a.qnew = c->q;
-    a.gam = c->aux + AX_GAM * c->P;
-    a.vgrav = c->aux + AX_VG * c->P;
-    a.P = c->P;
-    a.mbc = c->mbc;
-    a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
-    a.cfl_bits = c->dscal;
-    a.status = (int *)(c->dscal + 1);
-    for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+    const bool tile = PERT || use_tile();
     const double dtdx = dt / c->dx, dtdy = dt / c->dy, dtdz = dt / c->dz;
     {   // x-sweeps: lib/step3.f:121-280 (g = y, h = z)
         Span sp(c, TC_SWX, 0);
@@ -461,8 +612,8 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = 1; a.sA = d.NX; a.sB = NXY;
         a.gravity = 0; a.sweep = 1;
         a.dtdn = dtdx; a.fg = dtdy; a.fh = dtdz;
-        if (use_tile()) TRY(tile_dispatch(c, a, true, 1, &sp.n));   // tile axis y = g
-        else TRY(sweep_dispatch(c, a, &sp.n));
+        if (tile) TRY((tile_dispatch<T, PERT>(c, a, true, 1, &sp.n)));   // tile axis y = g
+        else if constexpr (!PERT) TRY(sweep_dispatch(c, a, &sp.n));
     }
     {   // y-sweeps: lib/step3.f:289-450 (g = z, h = x)
         Span sp(c, TC_SWY, 0);
@@ -471,11 +622,11 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = d.NX; a.sA = NXY; a.sB = 1;
         a.gravity = 0; a.sweep = 2;
         a.dtdn = dtdy; a.fg = dtdz; a.fh = dtdx;
-        if (use_tile()) {                                     // tile axis x = h
-            SweepArgs t = a;
+        if (tile) {                                           // tile axis x = h
+            SweepArgsT<T> t = a;
             t.nA = c->mx; t.nB = c->mz; t.sA = 1; t.sB = NXY;
-            TRY(tile_dispatch(c, t, false, 0, &sp.n));
-        } else {
+            TRY((tile_dispatch<T, PERT>(c, t, false, 0, &sp.n)));
+        } else if constexpr (!PERT) {
             TRY(sweep_dispatch(c, a, &sp.n));
         }
     }
@@ -486,8 +637,49 @@ static int step3_launch(magic_ctx *c, double dt)
         a.sN = NXY; a.sA = 1; a.sB = d.NX;
         a.gravity = 1; a.sweep = 3;
         a.dtdn = dtdz; a.fg = dtdx; a.fh = dtdy;
-        if (use_tile()) TRY(tile_dispatch(c, a, false, 1, &sp.n));  // tile axis x = g
-        else TRY(sweep_dispatch(c, a, &sp.n));
+        if (tile) TRY((tile_dispatch<T, PERT>(c, a, false, 1, &sp.n)));  // tile axis x = g
+        else if constexpr (!PERT) TRY(sweep_dispatch(c, a, &sp.n));
+    }
+    return MAGIC_OK;
+}
+
+static int step3_launch(magic_ctx *c, double dt)
+{
+    k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);
+    TRY(launch_check());
+    const int nspec = c->have_params ? c->prm.nspecadv : 0;
+    if (c->form == MAGIC_FORM_FP32P) {
+        SweepArgsT<float> a;
+        memset(&a, 0, sizeof(a));
+        a.qold = c->qold32;
+        a.qnew = c->q32;
+        a.P = c->P;
+        a.mbc = c->mbc;
+        a.nspecadv = nspec;
+        a.cfl_bits = c->dscal;
+        a.status = (int *)(c->dscal + 1);
+        for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+        const int NZ = c->d.NZ;
+        a.crho = c->col32 + C32_RHO * NZ;
+        a.cE = c->col32 + C32_E * NZ;
+        a.cgam = c->col32 + C32_GAM * NZ;
+        a.cvg = c->col32 + C32_VG * NZ;
+        a.cres = c->col32 + C32_RES * NZ;
+        TRY((step3_sweeps<float, true>(c, a, dt)));
+    } else {
+        SweepArgs a;
+        memset(&a, 0, sizeof(a));
+        a.qold = c->qold;
+        a.qnew = c->q;
+        a.gam = c->aux + AX_GAM * c->P;
+        a.vgrav = c->aux + AX_VG * c->P;
+        a.P = c->P;
+        a.mbc = c->mbc;
+        a.nspecadv = nspec;
+        a.cfl_bits = c->dscal;
+        a.status = (int *)(c->dscal + 1);
+        for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+        TRY((step3_sweeps<double, false>(c, a, dt)));
     }
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));
@@ -518,6 +710,83 @@ extern "C" int magic_step3(magic_ctx *c, double dt, double *cfl)
 }
 
 // ------------------------------------------------------------- src3 ------
+// The source pass of the FP32 form: the FP64 form's passes (below) on the
+// perturbation planes, with nu and kappa as background columns.
+static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, int ndtrc, double dtrc,
+                         const Forcing &f)
+{
+    const Dims &d = c->d;
+    const long long P = c->P;
+    const int NZ = d.NZ;
+    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gface(blocks(d.NX, 64), d.NY);
+    const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
+    const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
+                       c->col32 + C32_R * NZ};
+    {
+        Span sp(c, TC_SRC, 1);
+        k32_src_convert<<<padded_grid(c), 64, 0, c->stream>>>(d, c->q32, col);
+        TRY(launch_check());
+    }
+    for (int nn = 1; nn <= ndtr; nn++) {
+        TRY(bc3_launch(c, mthbc));
+        Span sp(c, TC_SRC, 13 + 6 * (ndtrc > 0 ? ndtrc : 0));
+        const double fourthirds = 4.0 / 3.0;
+        double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
+        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q32);
+        TRY(launch_check());
+        HIPCHECK(hipMemcpyAsync(c->u32, c->q32 + P, 3 * P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        const double dtr32 = dtr / 6.0;
+        k_diff_cross<float, true><<<gint, 64, 0, c->stream>>>(d, c->q32, c->u32, nu,
+                                                             (float)(dtr32 / (c->dx * c->dy)),
+                                                             (float)(dtr32 / (c->dx * c->dz)),
+                                                             (float)(dtr32 / (c->dy * c->dz)));
+        TRY(launch_check());
+        for (int cc = 1; cc <= 3; cc++) {
+            if (cc == 1) {
+                dtdx2 = fourthirds * dtdx2;
+            } else if (cc == 2) {
+                dtdy2 = fourthirds * dtdy2;
+                dtdx2 = dtdx2 / fourthirds;
+            } else {
+                dtdz2 = fourthirds * dtdz2;
+                dtdy2 = dtdy2 / fourthirds;
+            }
+            float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
+            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0);
+            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1);
+            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2);
+            TRY(launch_check());
+        }
+        const float cx = (float)((dtrc) / (c->dx * c->dx)), cy = (float)((dtrc) / (c->dy * c->dy)),
+                    cz = (float)((dtrc) / (c->dz * c->dz));
+        float *q5 = c->q32 + 4 * P;
+        HIPCHECK(hipMemcpyAsync(c->ut32, q5, P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        for (int k = 1; k <= ndtrc; k++) {
+            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cx, 0);
+            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, cy, 1);
+            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut32);
+            if (c->mz > c->mx) {
+                const int S = c->mz - c->mx;
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut32, S);
+            }
+            if (k < ndtrc) {
+                k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cz, 2);
+                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, 1);
+            } else {
+                k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, q5, c->ut32, c->ut32, kap, cz, 2);
+            }
+            TRY(launch_check());
+        }
+    }
+    {
+        Span sp(c, TC_SRC, 1);
+        k32_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q32, c->meqn, col, f);
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
+}
+
 static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, int *ndtr_out,
                        int *ndtrc_out)
 {
@@ -548,6 +817,17 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     const dim3 gcell(blocks(P, 256));
     const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
     const dim3 gface(blocks(d.NX, 64), d.NY);
+    Forcing f;
+    f.forcemth = fm;
+    f.t = t; f.dt = dt;
+    f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
+    f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+    f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
+    f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
+    f.tcenter = p.tcenter; f.twidth = p.twidth;
+    f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
+    f.tgauss = std::exp(-.5 * ((t - p.tcenter) * (t - p.tcenter) / (p.twidth * p.twidth)));
+    if (c->form == MAGIC_FORM_FP32P) return src3_passes32(c, mthbc, ndtr, dtr, ndtrc, dtrc, f);
     const double *nu = A + AX_NU * P, *kap = A + AX_KAP * P;
     {
         Span sp(c, TC_SRC, 1);
@@ -612,16 +892,6 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     }
     {
         Span sp(c, TC_SRC, 1);
-        Forcing f;
-        f.forcemth = fm;
-        f.t = t; f.dt = dt;
-        f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
-        f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
-        f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
-        f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
-        f.tcenter = p.tcenter; f.twidth = p.twidth;
-        f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
-        f.tgauss = std::exp(-.5 * ((t - p.tcenter) * (t - p.tcenter) / (p.twidth * p.twidth)));
         k_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q, c->meqn, ax, f);
         TRY(launch_check());
     }
@@ -657,7 +927,7 @@ extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv
         for (;;) {
             if ((st = bc3_launch(c, mthbc))) goto out;
             if ((st = b4step3_launch(c))) goto out;
-            if ((st = copy_q(c, c->qold, c->q))) goto out;
+            if ((st = copy_state(c, true))) goto out;
             if ((st = step3_launch(c, dt))) goto out;
             if ((st = step3_finish(c, &cfl))) goto out;
             t = told + dt;
@@ -672,7 +942,7 @@ extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv
                 break;
             }
             t = told;
-            if ((st = copy_q(c, c->q, c->qold))) goto out;
+            if ((st = copy_state(c, false))) goto out;
             dt = dtv[1] < dt * cflv[1] / cfl ? dtv[1] : dt * cflv[1] / cfl;
         }
         {

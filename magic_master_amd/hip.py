@@ -30,7 +30,10 @@ STATUS = {0: "ok", 1: "rho <= 0 in a Riemann problem", 2: "a**2 <= 0 in a Roe av
           3: "boundary condition needs MPI or a user routine", 4: "unsupported option",
           5: "invalid argument", 6: "HIP runtime error", 7: "device allocation failed"}
 
-EXPORTS = ["magic_create", "magic_destroy", "magic_strerror", "magic_set_params",
+FORM_FP64 = 0     # MAGIC_FORM_FP64
+FORM_FP32P = 1    # MAGIC_FORM_FP32P: FP32 perturbation from a background column at rest
+
+EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror", "magic_set_params",
            "magic_set_aux", "magic_set_q", "magic_get_q", "magic_bc3", "magic_b4step3",
            "magic_backup", "magic_restore", "magic_step3", "magic_src3", "magic_claw3",
            "magic_error_location", "magic_timing_enable", "magic_timing_read",
@@ -77,6 +80,7 @@ def load_library(path: str = LIB_PATH):
     L = C.CDLL(path)
     vp = C.c_void_p
     L.magic_create.argtypes = [C.c_int] * 6 + [C.c_double] * 6 + [C.POINTER(vp)]
+    L.magic_create_form.argtypes = [C.c_int] * 6 + [C.c_double] * 6 + [C.c_int, C.POINTER(vp)]
     L.magic_destroy.argtypes = [vp]
     L.magic_strerror.restype = C.c_char_p
     L.magic_strerror.argtypes = [C.c_int]
@@ -115,14 +119,17 @@ class StepLog:
 class MagicHIP:
     """One grid on the GPU (a magic_ctx)."""
 
-    def __init__(self, g: Grid, p: ProbData, aux: np.ndarray, q: Optional[np.ndarray] = None):
+    def __init__(self, g: Grid, p: ProbData, aux: np.ndarray, q: Optional[np.ndarray] = None,
+                 form: int = FORM_FP64):
         self.L = load_library()
         self.g = g
         self.p = p
+        self.form = form
         h = C.c_void_p()
-        self._check(self.L.magic_create(g.mx, g.my, g.mz, g.meqn, g.mbc, g.maux, g.xlower,
-                                        g.ylower, g.zlower, g.dx, g.dy, g.dz, C.byref(h)),
-                    "magic_create")
+        self._check(self.L.magic_create_form(g.mx, g.my, g.mz, g.meqn, g.mbc, g.maux, g.xlower,
+                                             g.ylower, g.zlower, g.dx, g.dy, g.dz, form,
+                                             C.byref(h)),
+                    "magic_create_form")
         self.h = h
         self._check(self.L.magic_set_params(self.h, C.byref(CParams.of(p))), "magic_set_params")
         aux = np.ascontiguousarray(aux, dtype=np.float64)

@@ -42,6 +42,21 @@ def build_hip() -> str:
     return HIP_LIB
 
 
+H5_LIB = os.path.join(HERE, "libmagic_h5.so")
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
+
+
+def build_h5() -> str:
+    """Host-only HDF5 frame writer/reader (include/magic_h5.h) on the HDF5 C
+    library of the image."""
+    tmp = H5_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", tmp, os.path.join(CSRC, "h5out.cpp"),
+          "-I" + os.path.join(HDF5_ROOT, "include"), "-L" + os.path.join(HDF5_ROOT, "lib"), "-lhdf5",
+          "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib")])
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 FC = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
 FORTRAN = os.path.join(HERE, "fortran")
 BIN = os.path.join(HERE, "bin")
@@ -54,13 +69,13 @@ def build_fortran() -> str:
     mod = os.path.join(BIN, "mod")
     os.makedirs(mod, exist_ok=True)
     objs = []
-    for src in ("magic_hip.f90", "claw3_hip.f90", "claw3ez_hip.f90"):
+    for src in ("magic_hip.f90", "claw3_hip.f90", "claw3ez_hip.f90", "out3_h5.f90"):
         obj = os.path.join(mod, src.replace(".f90", ".o"))
         _run([FC, "-c", "-O2", "-fPIC", "-module-dir", mod, "-I", mod,
               os.path.join(FORTRAN, src), "-o", obj])
         objs.append(obj)
     exe = os.path.join(BIN, "claw3ez_hip")
-    _run([FC, "-o", exe] + objs + ["-L" + HERE, "-lmagic_hip", "-Wl,-rpath,$ORIGIN/.."])
+    _run([FC, "-o", exe] + objs + ["-L" + HERE, "-lmagic_hip", "-lmagic_h5", "-Wl,-rpath,$ORIGIN/.."])
     # the drop-in operators as a library a reference-style caller links against
     lib = os.path.join(BIN, "libmagic_f77.so")
     _run([FC, "-shared", "-o", lib, objs[0], objs[1], "-L" + HERE, "-lmagic_hip",
@@ -88,7 +103,7 @@ def build_reference(ref_root: str = "/root/reference"):
 
 
 def build_all():
-    out = [build_hip(), build_fortran(), build_oracle()]
+    out = [build_hip(), build_h5(), build_fortran(), build_oracle()]
     ref = build_reference()
     if ref:
         out.append(ref)

@@ -12,8 +12,8 @@
 !      (the formulas of setaux.f:55-87)
 !   2  the standard temperature column integrated hydrostatically
 !      (same as magic_master_amd/problem.py:setaux_column)
-! Frames are written as raw REAL*8 q(meqn, 1:mx, 1:my, 1:mz) in Fortran order
-! to fort.qXXXX.bin with a text header fort.tXXXX (time, meqn, mx, my, mz).
+! Frames are written by out3 as the reference's HDF5 files
+! fort.qqXXXXid0000.h5 (mpilib/out3_mpi_h5sep.f90; magic_master_amd/fortran/out3_h5.f90).
 program claw3ez_hip
   use iso_c_binding
   use magic_hip
@@ -226,15 +226,7 @@ contains
   subroutine write_frame(iframe, t)
     integer, intent(in) :: iframe
     double precision, intent(in) :: t
-    character(len=32) :: fb, ft
-    write(fb, '(A,I4.4,A)') 'fort.q', iframe, '.bin'
-    write(ft, '(A,I4.4)') 'fort.t', iframe
-    open(21, file=ft, status='replace', form='formatted')
-    write(21, '(ES24.16,4I8)') t, meqn, mx, my, mz
-    close(21)
-    open(22, file=fb, status='replace', access='stream', form='unformatted')
-    write(22) q(1:meqn, 1:mx, 1:my, 1:mz)
-    close(22)
+    call out3(meqn, mbc, mx, my, mz, xlower, ylower, zlower, dx, dy, dz, q, t, iframe)
   end subroutine write_frame
 
 end program claw3ez_hip

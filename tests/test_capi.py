@@ -1,5 +1,5 @@
-"""The C-ABI library loads without a GPU and exports exactly the entry points
-include/magic_hip.h declares (no compute calls here)."""
+"""The C-ABI libraries load without a GPU and export exactly the entry points
+include/magic_hip.h and include/magic_h5.h declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -40,4 +40,21 @@ def test_create_rejects_bad_arguments_without_touching_the_gpu():
     # mbc < 2 and meqn out of range are rejected before any HIP call
     assert lib.magic_create(8, 8, 8, 6, 1, 32, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, ctypes.byref(h)) == 5
     assert lib.magic_create(8, 8, 8, 11, 2, 32, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, ctypes.byref(h)) == 5
+    # mz far above mx: conduct.f:86-91's out-of-row z-ghost writes would land
+    # in rows written after them, an order the device does not replay
     assert lib.magic_create(8, 8, 600, 6, 2, 32, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, ctypes.byref(h)) == 4
+    # unknown state form; FP32 form with mbc beyond its per-layer table
+    assert lib.magic_create_form(8, 8, 8, 6, 2, 32, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 7, ctypes.byref(h)) == 5
+    assert lib.magic_create_form(8, 8, 8, 6, 9, 32, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, hip.FORM_FP32P,
+                                 ctypes.byref(h)) == 4
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from magic_master_amd import h5
+    text = open(os.path.join(ROOT, "include", "magic_h5.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(magic_[a-z0-9_]+)\s*\(", text)))
+    assert names == ["magic_h5_frame_name", "magic_in3_h5", "magic_out3_h5"]
+    lib = h5.load_library()
+    for name in names:
+        assert hasattr(lib, name), name

@@ -1,7 +1,7 @@
 """The Fortran host on the GPU: (1) a caller that uses the drop-in claw3 with
 the reference's argument list (magic_master_amd/bin/dropin_check), and (2)
-the claw3ez_hip driver reading claw3ez.data/setprob.data; both against the C
-oracle on the same inputs."""
+the claw3ez_hip driver reading claw3ez.data/setprob.data and writing the
+reference's HDF5 frames; both against the C oracle on the same inputs."""
 import os
 import subprocess
 
@@ -55,10 +55,11 @@ def test_claw3ez_driver_matches_oracle(tmp_path):
     p.write(tmp_path / "setprob.data")
     out = _run([os.path.join(BIN, "claw3ez_hip")], tmp_path)
     assert "Frame    1" in out
-    hdr = open(tmp_path / "fort.t0001").read().split()
-    t = float(hdr[0])
-    assert [int(x) for x in hdr[1:]] == [g.meqn, g.mx, g.my, g.mz]
-    qg = np.fromfile(tmp_path / "fort.q0001.bin").reshape(g.mz, g.my, g.mx, g.meqn)
+    from magic_master_amd import h5
+    f = h5.in3(str(tmp_path / h5.frame_name(1)))
+    t = float(f["params"][2])
+    assert f["dims"] == (g.mz, g.my, g.mx, g.meqn)
+    qg = f["data"]
     O = COracle()
     qo = q.copy()
     dtv = list(c.dtv)

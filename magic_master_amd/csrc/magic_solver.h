@@ -43,6 +43,12 @@ __device__ __forceinline__ T cflsupbee(T theta, T beta, T r, T cfl, T cflup)
 template <typename T>
 __device__ __forceinline__ T suppow(T r, T cfl, T cflup)
 {
+    if constexpr (sizeof(T) == 4) {
+        // FP32: a denormal wave strength can make the ratio r overflow to
+        // +-inf, and (1-|r|)/(1+|r|) is then NaN.  For |r| >= 1e30 the
+        // result is already 0 in both precisions (rfrac rounds to 1).
+        r = fminF(fmaxF(r, T(-1e30)), T(1e30));
+    }
     T cfle = fmaxF(T(0.001), fminF(T(0.999), cfl));
     T cflupe = fmaxF(T(0.001), fminF(T(0.999), cflup));
     T s2 = (T(1.0) + cfle) / T(3.0);

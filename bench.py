@@ -35,16 +35,21 @@ HBM_PEAK_GBS = 8000.0
 # 19 (96 + 72 B).
 HYP_BYTES_PER_CELL = 112.0
 SRC_BYTES_PER_CELL = 168.0
-SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
-    "sweep_x": "magic::tile_sweep<double, false, {ntr}, true>",
-    "sweep_y": "magic::tile_sweep<double, false, {ntr}, false>",
-    "sweep_z": "magic::tile_sweep<double, false, {ntr}, false>",
+# FP32 perturbation form (background kept as columns): q' read + write in
+# each pass, 2 x 4 B x meqn = 48 B at meqn = 6
+HYP_BYTES_PER_CELL_FP32 = 48.0
+SRC_BYTES_PER_CELL_FP32 = 48.0
+SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers, acc = accumulation variant)
+    "sweep_x": "magic::tile_sweep<{t}, {pert}, {ntr}, true, {acc}>",
+    "sweep_y": "magic::tile_sweep<{t}, {pert}, {ntr}, false, {acc}>",
+    "sweep_z": "magic::tile_sweep<{t}, {pert}, {ntr}, false, {acc}>",
 }
+ACCUM = {"atomic": 1, "atomic2": 2}.get(os.environ.get("MAGIC_ACCUM", ""), 0)
 
 
-def run_gpu(workload, steps, warmup):
+def run_gpu(workload, steps, warmup, form="fp64"):
     from magic_master_amd import problem as P
-    from magic_master_amd.hip import MagicHIP
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
 
     if workload == "full":
         c, p, g, aux, q = P.config_full()
@@ -53,7 +58,7 @@ def run_gpu(workload, steps, warmup):
     else:
         c, p, g, aux, q = P.config_plain()
         name = "plain isothermal 250 K 64x64x48 (4x4x2 km), plume 20 mm/10min at t=600 s"
-    dev = MagicHIP(g, p, aux, q)
+    dev = MagicHIP(g, p, aux, q, form=FORM_FP32P if form == "fp32" else FORM_FP64)
     t0 = 0.0
     t, dtv, log = dev.run_steps(c, warmup, t0=t0)
     dev.timing(True)
@@ -121,6 +126,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="full", choices=["full", "plain"])
+    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"],
+                    help="state form: fp64 (the reference's arithmetic) or fp32 (perturbation form)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -134,7 +141,7 @@ def main():
         print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "
               "supported (the domain is not decomposed across GPUs)", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.form)
     g = r["g"]
     K = a.steps
     cells = g.ncell
@@ -144,7 +151,8 @@ def main():
     per_kernel = {}
     for cls, kname in SWEEP_KERNELS.items():
         ms, n = r["times"][cls]
-        k = kname.format(ntr=g.meqn - 5)
+        k = kname.format(t="float" if a.form == "fp32" else "double",
+                         pert="true" if a.form == "fp32" else "false", ntr=g.meqn - 5, acc=ACCUM)
         acc = per_kernel.setdefault(k, [0.0, 0])
         acc[0] += ms
         acc[1] += n
@@ -152,7 +160,9 @@ def main():
     sw_n = sum(r["times"][k][1] for k in SWEEP_KERNELS)
     launches_per_step3 = sw_n / max(r["timed_attempts"], 1)
     avg_launch_s = k_ms / max(k_n, 1) / 1e3
-    bytes_per_launch = HYP_BYTES_PER_CELL * cells / launches_per_step3
+    hyp_b, src_b = ((HYP_BYTES_PER_CELL_FP32, SRC_BYTES_PER_CELL_FP32) if a.form == "fp32"
+                    else (HYP_BYTES_PER_CELL, SRC_BYTES_PER_CELL))
+    bytes_per_launch = hyp_b * cells / launches_per_step3
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     st = np.array(r["step_ms"])
     out = {
@@ -166,10 +176,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if a.form == "fp32" else "f64",
         "data": "synthetic (analytic background column, fluid at rest, plume forcing)",
         "config": {"workload": r["name"], "cells": [g.mx, g.my, g.mz], "meqn": g.meqn,
-                   "precision": "fp64", "parallelism": "single GPU",
+                   "precision": a.form, "parallelism": "single GPU", "accumulation": ACCUM,
                    "attempted_steps": r["attempts"], "rejected_steps": r["rejected"],
                    "model_time_end_s": round(r["t"], 3)},
         "latency": {"unit": "ms per accepted step", "p50": round(float(np.percentile(st, 50)), 4),
@@ -180,9 +190,8 @@ def main():
                      "kernel": kname, "avg_launch_ms": round(avg_launch_s * 1e3, 5),
                      "launches": k_n, "sweep_launches_per_step3": launches_per_step3,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "bytes_per_cell_update": HYP_BYTES_PER_CELL + SRC_BYTES_PER_CELL,
-                     "step_hbm_frac": round(value * 1e6 * (HYP_BYTES_PER_CELL + SRC_BYTES_PER_CELL)
-                                            / (HBM_PEAK_GBS * 1e9), 5)},
+                     "bytes_per_cell_update": hyp_b + src_b,
+                     "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5)},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_budget)

@@ -36,8 +36,19 @@ def _run(cmd, cwd=None):
 
 
 def build_hip() -> str:
+    """Each source compiles to its own object (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, [HIPCC] + compile_flags + ["-c", src, "-o", obj], CSRC)
+                  for src, obj in zip(HIP_SOURCES, objs)]:
+            f.result()
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + HIP_SOURCES, cwd=CSRC)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

@@ -39,12 +39,11 @@ SRC_BYTES_PER_CELL = 168.0
 # each pass, 2 x 4 B x meqn = 48 B at meqn = 6
 HYP_BYTES_PER_CELL_FP32 = 48.0
 SRC_BYTES_PER_CELL_FP32 = 48.0
-SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers, acc = accumulation variant)
-    "sweep_x": "magic::tile_sweep<{t}, {pert}, {ntr}, true, {acc}>",
-    "sweep_y": "magic::tile_sweep<{t}, {pert}, {ntr}, false, {acc}>",
-    "sweep_z": "magic::tile_sweep<{t}, {pert}, {ntr}, false, {acc}>",
+SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
+    "sweep_x": "magic::tile_sweep<{t}, {pert}, {ntr}, true>",
+    "sweep_y": "magic::tile_sweep<{t}, {pert}, {ntr}, false>",
+    "sweep_z": "magic::tile_sweep<{t}, {pert}, {ntr}, false>",
 }
-ACCUM = {"atomic": 1, "atomic2": 2}.get(os.environ.get("MAGIC_ACCUM", ""), 0)
 
 
 def run_gpu(workload, steps, warmup, form="fp64"):
@@ -152,7 +151,7 @@ def main():
     for cls, kname in SWEEP_KERNELS.items():
         ms, n = r["times"][cls]
         k = kname.format(t="float" if a.form == "fp32" else "double",
-                         pert="true" if a.form == "fp32" else "false", ntr=g.meqn - 5, acc=ACCUM)
+                         pert="true" if a.form == "fp32" else "false", ntr=g.meqn - 5)
         acc = per_kernel.setdefault(k, [0.0, 0])
         acc[0] += ms
         acc[1] += n
@@ -179,7 +178,7 @@ def main():
         "dtype": "f32" if a.form == "fp32" else "f64",
         "data": "synthetic (analytic background column, fluid at rest, plume forcing)",
         "config": {"workload": r["name"], "cells": [g.mx, g.my, g.mz], "meqn": g.meqn,
-                   "precision": a.form, "parallelism": "single GPU", "accumulation": ACCUM,
+                   "precision": a.form, "parallelism": "single GPU",
                    "attempted_steps": r["attempts"], "rejected_steps": r["rejected"],
                    "model_time_end_s": round(r["t"], 3)},
         "latency": {"unit": "ms per accepted step", "p50": round(float(np.percentile(st, 50)), 4),

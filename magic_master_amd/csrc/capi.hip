@@ -46,10 +46,6 @@ struct TimedSpan {
 
 struct magic_ctx {
     int form = MAGIC_FORM_FP64;
-    // 9-point increment accumulation of the tile sweep: 0 registers
-    // (deterministic, default), 1 shared-memory atomics (MAGIC_ACCUM=atomic),
-    // 2 atomics compiled for two waves per SIMD (MAGIC_ACCUM=atomic2)
-    int accum = 0;
     int mx, my, mz, meqn, mbc, maux;
     double xlower, ylower, zlower, dx, dy, dz;
     Dims d;
@@ -185,10 +181,6 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
     }
     magic_ctx *c = new magic_ctx();
     c->form = form;
-    {
-        const char *e = getenv("MAGIC_ACCUM");
-        c->accum = !e ? 0 : strcmp(e, "atomic") == 0 ? 1 : strcmp(e, "atomic2") == 0 ? 2 : 0;
-    }
     c->mx = mx; c->my = my; c->mz = mz; c->meqn = meqn; c->mbc = mbc; c->maux = maux;
     c->xlower = xlower; c->ylower = ylower; c->zlower = zlower; c->dx = dx; c->dy = dy; c->dz = dz;
     c->d.mx = mx; c->d.my = my; c->d.mz = mz; c->d.mbc = mbc;
@@ -567,16 +559,8 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, bool pm, int a_is_g, int *
             const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
             const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
             if (gx == 0) continue;
-            if (c->accum == 2) {
-                if (pm) tile_sweep<T, PERT, NTR, true, 2><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-                else tile_sweep<T, PERT, NTR, false, 2><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            } else if (c->accum == 1) {
-                if (pm) tile_sweep<T, PERT, NTR, true, 1><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-                else tile_sweep<T, PERT, NTR, false, 1><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            } else {
-                if (pm) tile_sweep<T, PERT, NTR, true, 0><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-                else tile_sweep<T, PERT, NTR, false, 0><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            }
+            if (pm) tile_sweep<T, PERT, NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
+            else tile_sweep<T, PERT, NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
             TRY(launch_check());
             ++*nl;
         }

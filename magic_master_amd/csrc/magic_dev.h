@@ -44,10 +44,8 @@ __global__ void sweep_kernel(SweepArgs a);
 // the tile axis a (y- and z-sweeps).  a_is_g says whether the tile axis is
 // the sweep's g direction; sA/sB/nA/nB then describe the tile axis a and the
 // coloured axis b.  (T, PERT) = (double, false): FP64 form; (float, true):
-// FP32 perturbation form.  ACC = 0: deterministic register accumulation of
-// the 9-point increments; ACC = 1, 2: shared-memory atomic accumulation (2:
-// compiled for two waves per SIMD).
-template <typename T, bool PERT, int NTR, bool PM, int ACC>
+// FP32 perturbation form.
+template <typename T, bool PERT, int NTR, bool PM>
 __global__ void tile_sweep(SweepArgsT<T> a, int a_is_g);
 
 }  // namespace magic

@@ -144,32 +144,6 @@ struct Acc {
     __device__ __forceinline__ void c11(int m, T x) { D[1][1][m] += x; }
 };
 
-// The same folding into any sink add(gi, hi, m, x) (gi, hi = offset + 1):
-// used to add each increment straight into a shared-memory tile.
-template <int NC, typename T, typename F>
-struct AccSink {
-    F add;
-    __device__ __forceinline__ void g(int side, int off, T c, const T (&v)[NC])
-    {
-#pragma unroll
-        for (int m = 0; m < NC; m++) {
-            const T x = c * v[m];
-            if (side == 1) { add(0, off + 1, m, -x); add(1, off + 1, m, x); }
-            else           { add(2, off + 1, m, x); add(1, off + 1, m, -x); }
-        }
-    }
-    __device__ __forceinline__ void h(int side, int off, T c, const T (&v)[NC])
-    {
-#pragma unroll
-        for (int m = 0; m < NC; m++) {
-            const T x = c * v[m];
-            if (side == 1) { add(off + 1, 0, m, -x); add(off + 1, 1, m, x); }
-            else           { add(off + 1, 2, m, x); add(off + 1, 1, m, -x); }
-        }
-    }
-    __device__ __forceinline__ void c11(int m, T x) { add(1, 1, m, x); }
-};
-
 template <int NC, typename T>
 __device__ __forceinline__ void axpy3(T (&out)[NC], const T (&a)[NC], T c, const T (&b)[NC])
 {

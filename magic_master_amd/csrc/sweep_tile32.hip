@@ -6,29 +6,13 @@
 
 namespace magic {
 
-template __global__ void tile_sweep<float, true, 0, true, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 0, true, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 0, true, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 0, false, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 0, false, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 0, false, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, true, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, true, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, true, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, false, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, false, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 1, false, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, true, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, true, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, true, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, false, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, false, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 2, false, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, true, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, true, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, true, 2>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, false, 0>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, false, 1>(SweepArgsT<float>, int);
-template __global__ void tile_sweep<float, true, 5, false, 2>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 0, true>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 0, false>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 1, true>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 1, false>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 2, true>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 2, false>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 5, true>(SweepArgsT<float>, int);
+template __global__ void tile_sweep<float, true, 5, false>(SweepArgsT<float>, int);
 
 }  // namespace magic

@@ -51,9 +51,8 @@ __device__ __forceinline__ void report_t(const AT &A, int code, int i, int pg, i
     }
 }
 
-template <typename T, bool PERT, int NTR, bool PM, int ACC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC == 2 ? 2 : 1)))
-void tile_sweep(SweepArgsT<T> A, int a_is_g)
+template <typename T, bool PERT, int NTR, bool PM>
+__global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
 {
     constexpr int NC = 5 + NTR;
     constexpr int TA = TileShape<PM>::TA, CH = TileShape<PM>::CH, NT = TA * CH;
@@ -353,13 +352,12 @@ void tile_sweep(SweepArgsT<T> A, int a_is_g)
         // ---------------- part 3: transverse terms, two phases
         // Each phase recomputes the two cqxx splits it needs, in an order that
         // keeps at most four split vectors alive next to the accumulator.
-        // ACC = 0: each lane folds its increments into a register accumulator,
-        // added to the tile in three barrier-separated sub-steps (one per a
-        // offset, so no two lanes add to one entry at once): deterministic.
-        // ACC = 1, 2: each increment is added to the tile at once with a
-        // shared-memory atomic add: no accumulator registers and no sub-step
-        // barriers, summation order (rounding) not reproducible run to run;
-        // ACC = 2 also asks the compiler for two waves per SIMD.
+        // Each lane folds its increments into a register accumulator, added
+        // to the tile in three barrier-separated sub-steps (one per a offset,
+        // so no two lanes add to one entry at once): deterministic.  (Adding
+        // each increment to the tile with shared-memory atomics instead, with
+        // or without a two-waves-per-SIMD register budget, measured 3-5 %
+        // slower in FP64 and 7x slower in FP32 on the full-domain workload.)
         for (int phase = 0; phase < 2; phase++) {
             // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
             const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
@@ -408,41 +406,31 @@ void tile_sweep(SweepArgsT<T> A, int a_is_g)
                 for (int m = 0; m < NC; m++)
                     acc.c11(m, phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
             };
-            if constexpr (ACC == 0) {
-                Acc<NC, T> acc;
-                acc.zero();
-                if (on) transverse(acc);
-                // one sub-step per offset along a: lanes then add to distinct T entries
+            Acc<NC, T> acc;
+            acc.zero();
+            if (on) transverse(acc);
+            // one sub-step per offset along a: lanes then add to distinct T entries
 #pragma unroll
-                for (int da = 0; da < 3; da++) {
-                    if (on) {
-                        // the accumulator is indexed [g offset][h offset]; both
-                        // branches keep every index a compile-time constant so
-                        // it stays in registers
-                        if (a_is_g) {
+            for (int da = 0; da < 3; da++) {
+                if (on) {
+                    // the accumulator is indexed [g offset][h offset]; both
+                    // branches keep every index a compile-time constant so
+                    // it stays in registers
+                    if (a_is_g) {
 #pragma unroll
-                            for (int db = 0; db < 3; db++)
+                        for (int db = 0; db < 3; db++)
 #pragma unroll
-                                for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
-                        } else {
+                            for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
+                    } else {
 #pragma unroll
-                            for (int db = 0; db < 3; db++)
+                        for (int db = 0; db < 3; db++)
 #pragma unroll
-                                for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
-                        }
+                            for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
                     }
-                    __syncthreads();
                 }
-            } else {
-                auto add = [&](int gi, int hi, int m, T x) {
-                    const int da = a_is_g ? gi : hi, db = a_is_g ? hi : gi;
-                    atomicAdd(&LT[tix(m, db, pa + da, ci)], x);
-                };
-                AccSink<NC, T, decltype(add)> acc{add};
-                if (on) transverse(acc);
+                __syncthreads();
             }
         }
-        if constexpr (ACC != 0) __syncthreads();   // every increment is in the tile
 
         // ---------------- flush the tile to q (one read-modify-write per row)
         for (int e = t; e < NC * 3 * TROW * NCELL; e += NT) {

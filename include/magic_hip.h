@@ -82,6 +82,12 @@ int magic_set_params(magic_ctx *ctx, const magic_params *p);
  * MAGIC_FORM_FP32P keeps one column and returns MAGIC_EUNSUPPORTED when the
  * background is not a column at rest. */
 int magic_set_aux(magic_ctx *ctx, const double *aux);
+/* A background that depends on z only: col holds aux(1:maux, k) for
+ * k = 1-mbc .. mz+mbc (maux * (mz+2mbc) doubles, k slowest), the value of
+ * every column of the grid.  Same effect as magic_set_aux on the aux array
+ * that repeats it; no counterpart in the reference (the FP32 form keeps
+ * only the column anyway). */
+int magic_set_aux_column(magic_ctx *ctx, const double *col);
 /* Upload / download q (host, reference layout, ghost cells included). */
 int magic_set_q(magic_ctx *ctx, const double *q);
 int magic_get_q(magic_ctx *ctx, double *q);
@@ -126,6 +132,34 @@ int magic_error_location(magic_ctx *ctx, int *sweep, int *i, int *j, int *k);
 int magic_timing_enable(magic_ctx *ctx, int on);
 int magic_timing_read(magic_ctx *ctx, double ms[6], long long launches[6]);
 int magic_timing_reset(magic_ctx *ctx);
+
+/* ---- Batched driver (FP32 form): many grids advanced together ----------
+ * Each scenario is a grid with its own state (a magic_ctx reachable through
+ * magic_batch_ctx for parameters, aux and q); its step control is claw3's
+ * (claw3_mpi.f:736-927) but lives on the device, and one attempted step of
+ * every scenario is one replay of a HIP graph captured at the first
+ * magic_batch_run.  The sweeps of all scenarios run as one persistent launch
+ * per direction and tile colour over a work list of tiles.  No counterpart in
+ * the reference. */
+typedef struct magic_batch magic_batch;
+/* dims: (mx, my, mz) per scenario; geom: (xlower, ylower, zlower, dx, dy, dz)
+ * per scenario; form must be MAGIC_FORM_FP32P. */
+int magic_batch_create(int nscen, const int *dims, int meqn, int mbc, int maux, const double *geom,
+                       int form, magic_batch **out);
+int magic_batch_destroy(magic_batch *b);
+/* The scenario's context (owned by the batch: do not destroy it). */
+int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx);
+/* Start an interval of scenario s: from t to tend (clipped like claw3 when
+ * t < tend) with first dt, dtv(2) = dtv[1] as the largest dt, cflv(1:2),
+ * at most maxsteps accepted steps, boundary conditions mthbc. */
+int magic_batch_start(magic_batch *b, int s, double t, double tend, double dt, const double dtv[5],
+                      const double cflv[4], int maxsteps, const int mthbc[6]);
+/* Replay attempts until no started scenario is active (checked every
+ * check_every attempts) or max_attempts were made. */
+int magic_batch_run(magic_batch *b, int max_attempts, int check_every, int *attempts_done);
+/* real_out: t, dt (next), cfl (last attempt), cflmax, tend;
+ * int_out: accepted steps, attempts, rejected attempts, active, status, ndtr. */
+int magic_batch_state(magic_batch *b, int s, double real_out[5], int int_out[6]);
 
 #ifdef __cplusplus
 }

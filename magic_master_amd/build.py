@@ -14,7 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "sweep_tile32.hip", "bc_src.hip", "bc_src32.hip", "capi.hip"]
+HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "sweep_tile32.hip", "bc_src.hip", "bc_src32.hip", "capi.hip",
+               "batch.hip"]
 HIP_LIB = os.path.join(HERE, "libmagic_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,0 +1,636 @@
+// Batched driver: many grids (scenarios) of the FP32 perturbation form
+// advanced together on one stream, with claw3's step control on the device
+// (batch.h) and one attempted step of every scenario captured once as a HIP
+// graph and replayed.
+//
+// One attempt, for every scenario whose control block says it runs:
+//   prepare  (dt clip, per-grid sweep factors, source-pass substep counts)
+//   bc3, b4step3, copyq3 backup                 (per scenario, gated)
+//   3 directions x 6 tile colours of the sweep  (one persistent launch each
+//                                                over all scenarios' tiles)
+//   decide   (accept / reject from the Courant number)
+//   restore of rejected scenarios, src3 of accepted ones (gated)
+//   finalize (next dt, step count, end of interval)
+// No host synchronisation inside an attempt; the host reads the control
+// blocks back every few attempts to see whether all scenarios are done.
+#include <algorithm>
+
+#include "batch.h"
+#include "ctx.h"
+
+using namespace magic;
+
+struct magic_batch {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    std::vector<magic_ctx *> sc;
+    std::vector<ScenCtl> ctl;            // host copy of the control blocks
+    std::vector<int> mthbc;              // 6 per scenario
+    ScenCtl *ctl_d = nullptr;
+    ScenCtl *ctl_pin = nullptr;          // pinned read-back buffer
+    SweepArgsT<float> *args_d = nullptr; // 18 per scenario: direction x colour
+    const int **run_d = nullptr;         // per argument block: &ctl_d[s].run
+    int4 *items_d = nullptr;
+    int item_off[19] = {0};              // work list of direction d, colour k at [item_off[6d+k], ..)
+    int *counters_d = nullptr;           // 18 work counters
+    int grid_pm = 0, grid_tm = 0;        // persistent grid sizes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;                   // control blocks changed on the host
+    bool graph_ok = false;
+};
+
+// ------------------------------------------------------- control kernels --
+__global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, int *counters)
+{
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s < 18) counters[s] = 0;
+    if (s >= n) return;
+    ScenCtl &c = ctl[s];
+    c.accepted = 0;
+    c.rejected = 0;
+    c.run = c.active != 0 && c.status == 0;
+    if (!c.run) return;
+    if (c.newstep) {                                   // claw3_mpi.f:741
+        c.told = c.t;
+        if (c.clip && c.told + c.dt > c.tend) c.dt = c.tend - c.told;
+    }
+    c.cfl_bits = 0ull;
+    for (int e = 0; e < 6; e++) c.err[e] = 0;
+    const double dt = c.dt;
+    const double dtdx = dt / c.dx, dtdy = dt / c.dy, dtdz = dt / c.dz;
+    for (int k = 0; k < 6; k++) {
+        SweepArgsT<float> &ax = args[s * 18 + 0 * 6 + k], &ay = args[s * 18 + 1 * 6 + k],
+                          &az = args[s * 18 + 2 * 6 + k];
+        ax.dtdn = dtdx; ax.fg = dtdy; ax.fh = dtdz;
+        ay.dtdn = dtdy; ay.fg = dtdz; ay.fh = dtdx;
+        az.dtdn = dtdz; az.fg = dtdx; az.fh = dtdy;
+    }
+    // src3.f90:314-322
+    const int ndtr = (int)ceil((dt * c.difmax / (c.dxyzmin2)) / 0.45);
+    const double dtr = dt / ndtr;
+    const int ndtrc = (int)ceil((dtr * c.kapmax / (c.dxyzmin2)) / 0.45);
+    const double dtrc = dtr / ndtrc;
+    if (ndtr > c.ndtr_cap || ndtrc > c.ndtrc_cap) {   // beyond the captured substeps
+        c.status = MAGIC_EUNSUPPORTED;
+        c.run = 0;
+        return;
+    }
+    c.ndtr = ndtr;
+    c.ndtrc = ndtrc;
+    c.dtr = dtr;
+    c.dtrc = dtrc;
+    // factors of the diffusion passes in the order the host form evaluates them
+    const double fourthirds = 4.0 / 3.0;
+    double dtdx2 = (dtr) / (c.dx * c.dx), dtdy2 = (dtr) / (c.dy * c.dy), dtdz2 = (dtr) / (c.dz * c.dz);
+    const double dtr32 = dtr / 6.0;
+    c.fac[FAC_CROSS + 0] = dtr32 / (c.dx * c.dy);
+    c.fac[FAC_CROSS + 1] = dtr32 / (c.dx * c.dz);
+    c.fac[FAC_CROSS + 2] = dtr32 / (c.dy * c.dz);
+    for (int cc = 1; cc <= 3; cc++) {
+        if (cc == 1) {
+            dtdx2 = fourthirds * dtdx2;
+        } else if (cc == 2) {
+            dtdy2 = fourthirds * dtdy2;
+            dtdx2 = dtdx2 / fourthirds;
+        } else {
+            dtdz2 = fourthirds * dtdz2;
+            dtdy2 = dtdy2 / fourthirds;
+        }
+        c.fac[FAC_DIFF + 3 * (cc - 1) + 0] = dtdx2;
+        c.fac[FAC_DIFF + 3 * (cc - 1) + 1] = dtdy2;
+        c.fac[FAC_DIFF + 3 * (cc - 1) + 2] = dtdz2;
+    }
+    c.fac[FAC_COND + 0] = (dtrc) / (c.dx * c.dx);
+    c.fac[FAC_COND + 1] = (dtrc) / (c.dy * c.dy);
+    c.fac[FAC_COND + 2] = (dtrc) / (c.dz * c.dz);
+    c.tgauss = exp(-.5 * ((c.told - c.tcenter) * (c.told - c.tcenter) / (c.twidth * c.twidth)));
+}
+
+// claw3_mpi.f:843-886: accept or reject the attempt
+__global__ void k_batch_decide(ScenCtl *ctl, int n)
+{
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= n) return;
+    ScenCtl &c = ctl[s];
+    if (!c.run) return;
+    const double cfl = __longlong_as_double((long long)c.cfl_bits);
+    c.cfl = cfl;
+    c.attempts++;
+    if (c.err[0] != 0) {
+        c.status = c.err[0] == 1 ? MAGIC_ERHO : MAGIC_EA2;
+        c.run = 0;
+        c.active = 0;
+        return;
+    }
+    if (cfl <= c.cflv1) {
+        c.accepted = 1;
+        c.t = c.told + c.dt;
+        c.cflmax = c.cflmax > cfl ? c.cflmax : cfl;
+    } else {
+        c.rejected = 1;
+        c.rejects++;
+        c.t = c.told;
+        c.dt = c.dtmax < c.dt * c.cflv2 / cfl ? c.dtmax : c.dt * c.cflv2 / cfl;
+        c.newstep = 0;
+    }
+}
+
+// claw3_mpi.f:910-927: next dt after an accepted step, end of the interval
+__global__ void k_batch_finalize(ScenCtl *ctl, int n)
+{
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= n) return;
+    ScenCtl &c = ctl[s];
+    if (!c.accepted) return;
+    if (c.cfl == 0.0) c.dt = c.dtmax;
+    else c.dt = c.dtmax < c.dt * c.cflv2 / c.cfl ? c.dtmax : c.dt * c.cflv2 / c.cfl;
+    c.nsteps++;
+    c.newstep = 1;
+    if (c.t >= c.tend || c.nsteps >= c.maxsteps) c.active = 0;
+}
+
+// --------------------------------------------------------------- helpers --
+static int src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
+{
+    double dmin = c->dx < c->dy ? c->dx : c->dy;
+    dmin = dmin < c->dz ? dmin : c->dz;
+    const double d2 = dmin * dmin;
+    const double difmax = (4.0 / 3.0) * c->aux11_top;
+    *ndtr = (int)std::ceil((dtbound * difmax / d2) / 0.45);
+    if (*ndtr < 1) *ndtr = 1;
+    // the conduction count is largest for the largest dtr, i.e. dt / 1
+    *ndtrc = (int)std::ceil((dtbound * c->aux12_top / d2) / 0.45);
+    if (*ndtrc < 1) *ndtrc = 1;
+    return MAGIC_OK;
+}
+
+template <int NTR>
+static void launch_lists(magic_batch *b, int dir, int k)
+{
+    const int lo = b->item_off[6 * dir + k], hi = b->item_off[6 * dir + k + 1];
+    if (hi <= lo) return;
+    int *cnt = b->counters_d + 6 * dir + k;
+    if (dir == 0)
+        tile_sweep_list<float, true, NTR, true><<<b->grid_pm, 256, 0, b->stream>>>(b->args_d, b->items_d + lo, hi - lo,
+                                                                                b->run_d, cnt);
+    else
+        tile_sweep_list<float, true, NTR, false><<<b->grid_tm, 256, 0, b->stream>>>(b->args_d, b->items_d + lo, hi - lo,
+                                                                                 b->run_d, cnt);
+}
+
+static int launch_sweeps(magic_batch *b)
+{
+    const int ntr = b->sc[0]->meqn - 5;
+    for (int dir = 0; dir < 3; dir++)
+        for (int k = 0; k < 6; k++) {
+            switch (ntr) {
+            case 0: launch_lists<0>(b, dir, k); break;
+            case 1: launch_lists<1>(b, dir, k); break;
+            case 2: launch_lists<2>(b, dir, k); break;
+            case 5: launch_lists<5>(b, dir, k); break;
+            default: return MAGIC_EUNSUPPORTED;
+            }
+            TRY(launch_check());
+        }
+    return MAGIC_OK;
+}
+
+static StepGate gate(const magic_batch *b, int s, int mode, int nn = 0, int kk = 0, int fslot = -1)
+{
+    StepGate g;
+    g.c = b->ctl_d + s;
+    g.mode = mode;
+    g.nn = nn;
+    g.kk = kk;
+    g.fslot = fslot;
+    return g;
+}
+
+static int launch_bc(magic_batch *b, int s, const int *mthbc, StepGate g)
+{
+    magic_ctx *c = b->sc[s];
+    const Dims &d = c->d;
+    auto lohi = [](int m) { return m < 0 ? 0 : m; };
+    if (mthbc[0] >= 0 || mthbc[1] >= 0)
+        k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
+                                                                 lohi(mthbc[1]), g);
+    if (mthbc[2] >= 0 || mthbc[3] >= 0)
+        k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
+                                                                 lohi(mthbc[3]), g);
+    if (mthbc[4] >= 0 || mthbc[5] >= 0)
+        k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
+                                                                   lohi(mthbc[5]), c->zr32, g);
+    return launch_check();
+}
+
+static int launch_copy(magic_batch *b, float *dst, const float *src, long long n, StepGate g)
+{
+    long long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    k_copy<float, StepGate><<<(unsigned)nb, 256, 0, b->stream>>>(dst, src, n, g);
+    return launch_check();
+}
+
+// src3 of scenario s with the captured substep caps (src3_passes32's sequence)
+static int launch_src(magic_batch *b, int s)
+{
+    magic_ctx *c = b->sc[s];
+    const ScenCtl &h = b->ctl[s];
+    const Dims &d = c->d;
+    const long long P = c->P;
+    const int NZ = d.NZ;
+    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gface(blocks(d.NX, 64), d.NY);
+    const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
+    const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
+                       c->col32 + C32_R * NZ};
+    int mthbc[6];
+    memcpy(mthbc, &b->mthbc[6 * s], sizeof(mthbc));
+    mthbc[4] = -1;
+    mthbc[5] = -1;
+    k32_src_convert<<<dim3(blocks(d.NX, 64), d.NY, d.NZ), 64, 0, b->stream>>>(d, c->q32, col,
+                                                                             gate(b, s, GATE_ACCEPTED));
+    TRY(launch_check());
+    for (int nn = 1; nn <= h.ndtr_cap; nn++) {
+        const StepGate gd = gate(b, s, GATE_DIFF, nn);
+        TRY(launch_bc(b, s, mthbc, gd));
+        k_zghost_uvw<<<gface, 64, 0, b->stream>>>(d, c->q32, gd);
+        TRY(launch_copy(b, c->u32, c->q32 + P, 3 * P, gd));
+        k_diff_cross<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->q32, c->u32, nu, 0.f, 0.f, 0.f,
+                                                                       gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
+        TRY(launch_check());
+        for (int cc = 1; cc <= 3; cc++) {
+            float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
+            const int f0 = FAC_DIFF + 3 * (cc - 1);
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->qt32, qc, uc, nu, 0.f, 0,
+                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 0));
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, uc, c->qt32, uc, nu, 0.f, 1,
+                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 1));
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, qc, uc, uc, nu, 0.f, 2,
+                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 2));
+            TRY(launch_check());
+        }
+        float *q5 = c->q32 + 4 * P;
+        TRY(launch_copy(b, c->ut32, q5, P, gd));
+        for (int kk = 1; kk <= h.ndtrc_cap; kk++) {
+            const StepGate gk = gate(b, s, GATE_COND, nn, kk);
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
+                                                                     gate(b, s, GATE_COND, nn, kk, FAC_COND + 0));
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
+                                                                     gate(b, s, GATE_COND, nn, kk, FAC_COND + 1));
+            k_conduct_zghost<<<gface, 64, 0, b->stream>>>(d, c->ut32, gk);
+            if (c->mz > c->mx) {
+                const int S = c->mz - c->mx;
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->stream>>>(d, c->ut32, S, gk);
+            }
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(
+                d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
+            k_copy_interior<<<gint, 64, 0, b->stream>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
+            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(
+                d, q5, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_LAST, nn, kk, FAC_COND + 2));
+            TRY(launch_check());
+        }
+    }
+    const magic_params &p = c->prm;
+    Forcing f;
+    f.forcemth = (int)p.forcemth;
+    f.t = 0.0; f.dt = 0.0; f.tgauss = 0.0;   // from the control block at run time
+    f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
+    f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+    f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
+    f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
+    f.tcenter = p.tcenter; f.twidth = p.twidth;
+    f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
+    k32_src_finish<<<gint, 64, 0, b->stream>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
+    return launch_check();
+}
+
+// one attempt of every scenario, as launched into the capture
+static int record_attempt(magic_batch *b)
+{
+    const unsigned nb = blocks(b->n > 18 ? b->n : 18, 256);
+    k_batch_prepare<<<nb, 256, 0, b->stream>>>(b->ctl_d, b->n, b->args_d, b->counters_d);
+    TRY(launch_check());
+    for (int s = 0; s < b->n; s++) {
+        magic_ctx *c = b->sc[s];
+        const StepGate g = gate(b, s, GATE_RUN);
+        TRY(launch_bc(b, s, &b->mthbc[6 * s], g));
+        if (c->meqn > 5) {
+            k_clamp<<<blocks(c->P, 256), 256, 0, b->stream>>>(c->q32, c->P, c->meqn, g);
+            TRY(launch_check());
+        }
+        TRY(launch_copy(b, c->qold32, c->q32, c->P * c->meqn, g));
+    }
+    TRY(launch_sweeps(b));
+    k_batch_decide<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
+    TRY(launch_check());
+    for (int s = 0; s < b->n; s++) {
+        magic_ctx *c = b->sc[s];
+        TRY(launch_copy(b, c->q32, c->qold32, c->P * c->meqn, gate(b, s, GATE_REJECTED)));
+        TRY(launch_src(b, s));
+    }
+    k_batch_finalize<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
+    return launch_check();
+}
+
+// sweep argument blocks and tile work lists (geometry only)
+static int build_lists(magic_batch *b)
+{
+    std::vector<SweepArgsT<float>> args((size_t)b->n * 18);
+    std::vector<const int *> run((size_t)b->n * 18);
+    std::vector<std::vector<int4>> lists(18);
+    // tiles of longer pencils first (they take longest)
+    std::vector<int> order(b->n);
+    for (int s = 0; s < b->n; s++) order[s] = s;
+    for (int dir = 0; dir < 3; dir++) {
+        auto len = [&](int s) { return dir == 0 ? b->sc[s]->mx : dir == 1 ? b->sc[s]->my : b->sc[s]->mz; };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len(x) > len(y); });
+        for (int s : order) {
+            magic_ctx *c = b->sc[s];
+            const Dims &d = c->d;
+            const long long NXY = (long long)d.NX * d.NY;
+            const int NZ = d.NZ;
+            SweepArgsT<float> a;
+            memset(&a, 0, sizeof(a));
+            a.qold = c->qold32;
+            a.qnew = c->q32;
+            a.P = c->P;
+            a.mbc = c->mbc;
+            a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
+            a.cfl_bits = &b->ctl_d[s].cfl_bits;
+            a.status = &b->ctl_d[s].err[0];
+            for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+            a.crho = c->col32 + C32_RHO * NZ;
+            a.cE = c->col32 + C32_E * NZ;
+            a.cgam = c->col32 + C32_GAM * NZ;
+            a.cvg = c->col32 + C32_VG * NZ;
+            a.cres = c->col32 + C32_RES * NZ;
+            int a_is_g;
+            bool pm;
+            if (dir == 0) {          // step3.f:121-280 (g = y, h = z), tile axis y = g
+                a.comp[0] = 0; a.comp[1] = 1; a.comp[2] = 2; a.comp[3] = 3; a.comp[4] = 4;
+                a.n = c->mx; a.nA = c->my; a.nB = c->mz;
+                a.sN = 1; a.sA = d.NX; a.sB = NXY;
+                a.gravity = 0; a.sweep = 1;
+                a_is_g = 1; pm = true;
+            } else if (dir == 1) {   // step3.f:289-450 (g = z, h = x), tile axis x = h
+                a.comp[0] = 0; a.comp[1] = 2; a.comp[2] = 3; a.comp[3] = 1; a.comp[4] = 4;
+                a.n = c->my; a.nA = c->mx; a.nB = c->mz;
+                a.sN = d.NX; a.sA = 1; a.sB = NXY;
+                a.gravity = 0; a.sweep = 2;
+                a_is_g = 0; pm = false;
+            } else {                 // step3.f:459-621 (g = x, h = y), tile axis x = g
+                a.comp[0] = 0; a.comp[1] = 3; a.comp[2] = 1; a.comp[3] = 2; a.comp[4] = 4;
+                a.n = c->mz; a.nA = c->mx; a.nB = c->my;
+                a.sN = NXY; a.sA = 1; a.sB = d.NX;
+                a.gravity = 1; a.sweep = 3;
+                a_is_g = 1; pm = false;
+            }
+            const int TA = pm ? 4 : 16;
+            const int ntiles = (a.nA + 2 + TA - 1) / TA;
+            for (int ca = 0; ca < 2; ca++)
+                for (int cb = 0; cb < 3; cb++) {
+                    const int k = ca * 3 + cb, ai = s * 18 + dir * 6 + k;
+                    SweepArgsT<float> t = a;
+                    t.cA = ca;
+                    t.cB = cb;
+                    args[ai] = t;
+                    run[ai] = &b->ctl_d[s].run;
+                    const int gx = (ntiles - ca + 1) / 2, gy = (a.nB + 1 - cb) / 3 + 1;
+                    for (int by = 0; by < gy; by++)
+                        for (int bx = 0; bx < gx; bx++) lists[dir * 6 + k].push_back(make_int4(ai, a_is_g, bx, by));
+                }
+        }
+    }
+    size_t total = 0;
+    for (int k = 0; k < 18; k++) {
+        b->item_off[k] = (int)total;
+        total += lists[k].size();
+    }
+    b->item_off[18] = (int)total;
+    std::vector<int4> items;
+    items.reserve(total);
+    for (auto &l : lists) items.insert(items.end(), l.begin(), l.end());
+    hipFree(b->items_d);
+    b->items_d = nullptr;
+    HIPCHECK(hipMalloc(&b->items_d, (total ? total : 1) * sizeof(int4)));
+    HIPCHECK(hipMemcpyAsync(b->items_d, items.data(), total * sizeof(int4), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->args_d, args.data(), args.size() * sizeof(args[0]), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->run_d, run.data(), run.size() * sizeof(run[0]), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return MAGIC_OK;
+}
+
+template <int NTR>
+static int occupancy(int *pm, int *tm)
+{
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        pm, reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, true>), 256, 0));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        tm, reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, false>), 256, 0));
+    return MAGIC_OK;
+}
+
+static int build_graph(magic_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_ok = false;
+    int dev = 0, ncu = 0, opm = 1, otm = 1;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    switch (b->sc[0]->meqn - 5) {
+    case 0: TRY(occupancy<0>(&opm, &otm)); break;
+    case 1: TRY(occupancy<1>(&opm, &otm)); break;
+    case 2: TRY(occupancy<2>(&opm, &otm)); break;
+    case 5: TRY(occupancy<5>(&opm, &otm)); break;
+    default: return MAGIC_EUNSUPPORTED;
+    }
+    b->grid_pm = ncu * (opm > 0 ? opm : 1);
+    b->grid_tm = ncu * (otm > 0 ? otm : 1);
+    TRY(build_lists(b));
+    HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int st = record_attempt(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != MAGIC_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    HIPCHECK(e);
+    b->graph = g;
+    HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_ok = true;
+    return MAGIC_OK;
+}
+
+// ------------------------------------------------------------------ ABI --
+extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc, int maux, const double *geom,
+                                  int form, magic_batch **out)
+{
+    if (!out || nscen < 1 || !dims || !geom) return MAGIC_EARG;
+    *out = nullptr;
+    if (form != MAGIC_FORM_FP32P) return MAGIC_EUNSUPPORTED;
+    magic_batch *b = new magic_batch();
+    b->n = nscen;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return MAGIC_EHIP;
+    }
+    int st = MAGIC_OK;
+    for (int s = 0; s < nscen && st == MAGIC_OK; s++) {
+        magic_ctx *c = nullptr;
+        const double *g = geom + 6 * s;
+        st = magic_create_form(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2], meqn, mbc, maux, g[0], g[1], g[2],
+                               g[3], g[4], g[5], form, &c);
+        if (st == MAGIC_OK) {
+            hipStreamSynchronize(c->stream);
+            hipStreamDestroy(c->stream);
+            c->stream = b->stream;
+            c->own_stream = false;
+            b->sc.push_back(c);
+        }
+    }
+    b->ctl.assign(nscen, ScenCtl());
+    b->mthbc.assign(6 * nscen, 1);
+    for (int s = 0; s < nscen; s++) memset(&b->ctl[s], 0, sizeof(ScenCtl));   // inactive until started
+    bool ok = st == MAGIC_OK &&
+              hipMalloc(&b->ctl_d, nscen * sizeof(ScenCtl)) == hipSuccess &&
+              hipHostMalloc(&b->ctl_pin, nscen * sizeof(ScenCtl), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&b->args_d, (size_t)nscen * 18 * sizeof(SweepArgsT<float>)) == hipSuccess &&
+              hipMalloc(&b->run_d, (size_t)nscen * 18 * sizeof(int *)) == hipSuccess &&
+              hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess;
+    if (!ok) {
+        magic_batch_destroy(b);
+        return st != MAGIC_OK ? st : MAGIC_ENOMEM;
+    }
+    *out = b;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_destroy(magic_batch *b)
+{
+    if (!b) return MAGIC_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto c : b->sc) magic_destroy(c);
+    hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
+    if (b->ctl_pin) hipHostFree(b->ctl_pin);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx)
+{
+    if (!b || !ctx || s < 0 || s >= b->n) return MAGIC_EARG;
+    *ctx = b->sc[s];
+    b->graph_ok = false;   // the caller may change parameters, aux or q
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_start(magic_batch *b, int s, double t, double tend, double dt, const double dtv[5],
+                                 const double cflv[4], int maxsteps, const int mthbc[6])
+{
+    if (!b || s < 0 || s >= b->n || !dtv || !cflv || !mthbc || maxsteps < 1) return MAGIC_EARG;
+    magic_ctx *c = b->sc[s];
+    if (!c->have_aux || !c->have_params) return MAGIC_EARG;
+    if (cflv[1] > cflv[0]) return MAGIC_EARG;
+    for (int f = 0; f < 6; f++)
+        if (mthbc[f] == 0 || mthbc[f] >= 4) return MAGIC_EBC;
+    const magic_params &p = c->prm;
+    if (p.omega1 != 0.0 || !(p.forcemth == 0.0 || p.forcemth == 1.0 || p.forcemth == 3.0))
+        return MAGIC_EUNSUPPORTED;
+    ScenCtl &h = b->ctl[s];
+    if (memcmp(&b->mthbc[6 * s], mthbc, 6 * sizeof(int)) != 0) b->graph_ok = false;
+    memcpy(&b->mthbc[6 * s], mthbc, 6 * sizeof(int));
+    h.t = t;
+    h.told = t;
+    h.tend = tend;
+    h.clip = t < tend;
+    h.dt = dt;
+    h.dtmax = dtv[1];
+    h.cflv1 = cflv[0];
+    h.cflv2 = cflv[1];
+    h.cfl = 0.0;
+    h.cflmax = 0.0;
+    h.active = 1;
+    h.newstep = 1;
+    h.nsteps = 0;
+    h.attempts = 0;
+    h.rejects = 0;
+    h.maxsteps = maxsteps;
+    h.status = 0;
+    h.dx = c->dx; h.dy = c->dy; h.dz = c->dz;
+    double dmin = c->dx < c->dy ? c->dx : c->dy;
+    dmin = dmin < c->dz ? dmin : c->dz;
+    h.dxyzmin2 = dmin * dmin;
+    h.difmax = (4.0 / 3.0) * c->aux11_top;
+    h.kapmax = c->aux12_top;
+    h.tcenter = p.tcenter;
+    h.twidth = p.twidth;
+    int a, bb;
+    src_caps(c, dt > dtv[1] ? dt : dtv[1], &a, &bb);
+    if (a != h.ndtr_cap || bb != h.ndtrc_cap) b->graph_ok = false;
+    h.ndtr_cap = a;
+    h.ndtrc_cap = bb;
+    b->dirty = true;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_run(magic_batch *b, int max_attempts, int check_every, int *attempts_done)
+{
+    if (!b || max_attempts < 1 || check_every < 1) return MAGIC_EARG;
+    if (!b->graph_ok) {
+        // the captured kernels read the control blocks; upload before building
+        HIPCHECK(hipMemcpyAsync(b->ctl_d, b->ctl.data(), b->n * sizeof(ScenCtl), hipMemcpyHostToDevice, b->stream));
+        TRY(build_graph(b));
+        b->dirty = false;
+    }
+    if (b->dirty) {
+        HIPCHECK(hipMemcpyAsync(b->ctl_d, b->ctl.data(), b->n * sizeof(ScenCtl), hipMemcpyHostToDevice, b->stream));
+        b->dirty = false;
+    }
+    int done = 0;
+    bool any = true;
+    while (done < max_attempts && any) {
+        const int k = std::min(check_every, max_attempts - done);
+        for (int i = 0; i < k; i++) HIPCHECK(hipGraphLaunch(b->exec, b->stream));
+        done += k;
+        HIPCHECK(hipMemcpyAsync(b->ctl_pin, b->ctl_d, b->n * sizeof(ScenCtl), hipMemcpyDeviceToHost, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        memcpy(b->ctl.data(), b->ctl_pin, b->n * sizeof(ScenCtl));
+        any = false;
+        for (int s = 0; s < b->n; s++) any = any || (b->ctl[s].active && b->ctl[s].status == 0);
+    }
+    if (attempts_done) *attempts_done = done;
+    for (int s = 0; s < b->n; s++)
+        if (b->ctl[s].status != 0) return b->ctl[s].status;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_state(magic_batch *b, int s, double real_out[5], int int_out[6])
+{
+    if (!b || s < 0 || s >= b->n) return MAGIC_EARG;
+    const ScenCtl &h = b->ctl[s];
+    if (real_out) {
+        real_out[0] = h.t;
+        real_out[1] = h.dt;
+        real_out[2] = h.cfl;
+        real_out[3] = h.cflmax;
+        real_out[4] = h.tend;
+    }
+    if (int_out) {
+        int_out[0] = h.nsteps;
+        int_out[1] = h.attempts;
+        int_out[2] = h.rejects;
+        int_out[3] = h.active;
+        int_out[4] = h.status;
+        int_out[5] = h.ndtr;
+    }
+    return MAGIC_OK;
+}

@@ -29,10 +29,19 @@ struct Forcing {
     double tgauss;        // exp(-.5*((t-tcenter)**2/twidth**2)), host FP64
 };
 
-template <typename T>
-__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi);
-template <typename T>
-__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi);
+// Gate of the shared kernels for a single grid: always on; kernel
+// arguments are used as passed.
+struct NoGate {
+    __device__ __forceinline__ bool on() const { return true; }
+    template <typename T>
+    __device__ __forceinline__ T f(T x, int) const { return x; }
+    __device__ __forceinline__ void forcing(double &, double &, double &) const {}
+};
+
+template <typename T, class G>
+__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi, G gate);
+template <typename T, class G>
+__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi, G gate);
 // Stephan & Alexander convective heating profile (src3.f90:534-572) for a
 // precipitation rate precipw > 0.1 at height zcell (m): K/s.
 __device__ __forceinline__ double sa_heating(double precipw, double zcell)
@@ -55,21 +64,21 @@ __device__ __forceinline__ double sa_heating(double precipw, double zcell)
 __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
                        const double *aux2, const double *aux3, const double *aux5,
                        const double *zratio);
-template <typename T>
-__global__ void k_clamp(T *q, long long P, int nc);
+template <typename T, class G>
+__global__ void k_clamp(T *q, long long P, int nc, G gate);
 __global__ void k_src_convert(long long P, double *q, const Aux a);
-template <typename T>
-__global__ void k_zghost_uvw(Dims d, T *q);
-template <typename T, bool COLC>
-__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz);
-template <typename T>
-__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes);
-template <typename T>
-__global__ void k_conduct_zghost(Dims d, T *u);
-template <typename T>
-__global__ void k_conduct_zspill(Dims d, T *u, int S);
-template <typename T, bool COLC>
-__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis);
+template <typename T, class G>
+__global__ void k_zghost_uvw(Dims d, T *q, G gate);
+template <typename T, bool COLC, class G>
+__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz, G gate);
+template <typename T, class G>
+__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes, G gate);
+template <typename T, class G>
+__global__ void k_conduct_zghost(Dims d, T *u, G gate);
+template <typename T, class G>
+__global__ void k_conduct_zspill(Dims d, T *u, int S, G gate);
+template <typename T, bool COLC, class G>
+__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate);
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
 
@@ -83,9 +92,14 @@ struct Col32 {
 struct Zr32 {
     float s[2][8];
 };
-__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr);
-__global__ void k32_src_convert(Dims d, float *q, Col32 col);
-__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f);
+template <class G>
+__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate);
+template <class G>
+__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate);
+template <class G>
+__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G gate);
+template <typename T, class G>
+__global__ void k_copy(T *dst, const T *src, long long n, G gate);
 __global__ void k32_aos_to_pert(Dims d, float *q, const double *aos, int nc, const double *bgrho,
                                 const double *bgE);
 __global__ void k32_pert_to_aos(Dims d, double *aos, const float *q, int nc, const double *bgrho,

@@ -140,16 +140,16 @@ __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, 
 }
 
 // FP64 instantiations of the shared kernels
-template __global__ void k_bc_x<double>(Dims, double *, int, int, int);
-template __global__ void k_bc_y<double>(Dims, double *, int, int, int);
-template __global__ void k_clamp<double>(double *, long long, int);
-template __global__ void k_zghost_uvw<double>(Dims, double *);
-template __global__ void k_diff_cross<double, false>(Dims, double *, const double *, const double *, double,
-                                                      double, double);
-template __global__ void k_copy_interior<double>(Dims, double *, const double *, int);
-template __global__ void k_pass<double, false>(Dims, double *, const double *, const double *, const double *,
-                                               double, int);
-template __global__ void k_conduct_zghost<double>(Dims, double *);
-template __global__ void k_conduct_zspill<double>(Dims, double *, int);
+template __global__ void k_bc_x<double, NoGate>(Dims, double *, int, int, int, NoGate);
+template __global__ void k_bc_y<double, NoGate>(Dims, double *, int, int, int, NoGate);
+template __global__ void k_clamp<double, NoGate>(double *, long long, int, NoGate);
+template __global__ void k_zghost_uvw<double, NoGate>(Dims, double *, NoGate);
+template __global__ void k_diff_cross<double, false, NoGate>(Dims, double *, const double *, const double *, double, double, double,
+                                                        NoGate);
+template __global__ void k_copy_interior<double, NoGate>(Dims, double *, const double *, int, NoGate);
+template __global__ void k_pass<double, false, NoGate>(Dims, double *, const double *, const double *, const double *, double, int,
+                                                  NoGate);
+template __global__ void k_conduct_zghost<double, NoGate>(Dims, double *, NoGate);
+template __global__ void k_conduct_zspill<double, NoGate>(Dims, double *, int, NoGate);
 
 }  // namespace magic

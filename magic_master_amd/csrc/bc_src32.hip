@@ -9,31 +9,34 @@
 // into fused multiply-adds: this form's error bar is a fraction of the field
 // peak against the FP64 form.
 #pragma clang fp contract(fast)
+#include "batch.h"
 #include "magic_dev.h"
 #include "stencil_body.h"
 
 namespace magic {
 
 // FP32 instantiations of the shared kernels (coefficients as columns)
-template __global__ void k_bc_x<float>(Dims, float *, int, int, int);
-template __global__ void k_bc_y<float>(Dims, float *, int, int, int);
-template __global__ void k_clamp<float>(float *, long long, int);
-template __global__ void k_zghost_uvw<float>(Dims, float *);
-template __global__ void k_diff_cross<float, true>(Dims, float *, const float *, const float *, float, float,
-                                                   float);
-template __global__ void k_copy_interior<float>(Dims, float *, const float *, int);
-template __global__ void k_pass<float, true>(Dims, float *, const float *, const float *, const float *, float,
-                                             int);
-template __global__ void k_conduct_zghost<float>(Dims, float *);
-template __global__ void k_conduct_zspill<float>(Dims, float *, int);
+template __global__ void k_bc_x<float, NoGate>(Dims, float *, int, int, int, NoGate);
+template __global__ void k_bc_y<float, NoGate>(Dims, float *, int, int, int, NoGate);
+template __global__ void k_clamp<float, NoGate>(float *, long long, int, NoGate);
+template __global__ void k_zghost_uvw<float, NoGate>(Dims, float *, NoGate);
+template __global__ void k_diff_cross<float, true, NoGate>(Dims, float *, const float *, const float *, float, float, float,
+                                                        NoGate);
+template __global__ void k_copy_interior<float, NoGate>(Dims, float *, const float *, int, NoGate);
+template __global__ void k_pass<float, true, NoGate>(Dims, float *, const float *, const float *, const float *, float, int,
+                                                  NoGate);
+template __global__ void k_conduct_zghost<float, NoGate>(Dims, float *, NoGate);
+template __global__ void k_conduct_zspill<float, NoGate>(Dims, float *, int, NoGate);
 
 // z faces (bc3_mpi.f:544-809).  The FP64 form writes
 //   q_ghost = aux_ghost + (q_boundary - aux_boundary) * sc
 // for rho, the horizontal momenta and E, and q_ghost = q_boundary * sc for
 // the vertical momentum; with a background at rest every one of them is
 // q'_ghost = q'_boundary * sc.  sc depends on the layer only.
-__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr)
+template <class G>
+__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate)
 {
+    if (!gate.on()) return;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
     const int j = (int)blockIdx.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
@@ -77,8 +80,10 @@ __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr)
 // (src3.f90:242-304 with u_bar = 0):
 //   T' = T - T_bar = (p' - rho' (gamma-1) E_bar / rho_bar) / (rho R),
 //   p' = (gamma-1) (E' - |m|^2 / (2 rho)),  rho = rho_bar + rho'.
-__global__ void k32_src_convert(Dims d, float *q, Col32 col)
+template <class G>
+__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate)
 {
+    if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
     const float rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0f, R = col.R[kk];
@@ -96,8 +101,11 @@ __global__ void k32_src_convert(Dims d, float *q, Col32 col)
 //   m = u rho,  E' = E_bar rho'/rho_bar + T' rho R/(gamma-1) + |m|^2/(2 rho),
 // the heating adding dt*heat to T' first.  Positions and the forcing are
 // evaluated in FP64.
-__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f)
+template <class G>
+__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G gate)
 {
+    if (!gate.on()) return;
+    gate.forcing(f.t, f.dt, f.tgauss);
     INTERIOR_IJK
     const long long P = d.P;
     const int kk = k + d.mbc - 1;
@@ -130,6 +138,36 @@ __global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f)
     q[4 * P + c] = Eb * (rp / rb) + tp * rho * R / g1 + 0.5f * (m2 * m2 + m3 * m3 + m4 * m4) / rho;
     for (int m = 5; m < nc; m++) q[m * P + c] = fmaxD(0.0f, fminD(q[m * P + c], 1.0f));
 }
+
+// plain copy of n elements (copyq3 and the working copies of the source pass)
+template <typename T, class G>
+__global__ void k_copy(T *dst, const T *src, long long n, G gate)
+{
+    if (!gate.on()) return;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+// instantiations: one grid (NoGate) and the batched driver (StepGate)
+template __global__ void k32_bc_z<NoGate>(Dims, float *, int, int, int, Zr32, NoGate);
+template __global__ void k32_src_convert<NoGate>(Dims, float *, Col32, NoGate);
+template __global__ void k32_src_finish<NoGate>(Dims, float *, int, Col32, Forcing, NoGate);
+template __global__ void k32_bc_z<StepGate>(Dims, float *, int, int, int, Zr32, StepGate);
+template __global__ void k32_src_convert<StepGate>(Dims, float *, Col32, StepGate);
+template __global__ void k32_src_finish<StepGate>(Dims, float *, int, Col32, Forcing, StepGate);
+template __global__ void k_copy<float, StepGate>(float *, const float *, long long, StepGate);
+template __global__ void k_bc_x<float, StepGate>(Dims, float *, int, int, int, StepGate);
+template __global__ void k_bc_y<float, StepGate>(Dims, float *, int, int, int, StepGate);
+template __global__ void k_clamp<float, StepGate>(float *, long long, int, StepGate);
+template __global__ void k_zghost_uvw<float, StepGate>(Dims, float *, StepGate);
+template __global__ void k_diff_cross<float, true, StepGate>(Dims, float *, const float *, const float *, float,
+                                                            float, float, StepGate);
+template __global__ void k_copy_interior<float, StepGate>(Dims, float *, const float *, int, StepGate);
+template __global__ void k_pass<float, true, StepGate>(Dims, float *, const float *, const float *, const float *,
+                                                      float, int, StepGate);
+template __global__ void k_conduct_zghost<float, StepGate>(Dims, float *, StepGate);
+template __global__ void k_conduct_zspill<float, StepGate>(Dims, float *, int, StepGate);
 
 // reference AoS q(meqn, i, j, k) (FP64, full state) <-> perturbation planes:
 // the background is subtracted / added in FP64, then rounded.

@@ -5,136 +5,7 @@
 // the bc3 z-boundary density ratios, and the source-pass scratch planes.
 // Everything runs on one HIP stream of default priority; every entry point
 // returns after its work is complete.
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <vector>
-
-#include "../../include/magic_hip.h"
-#include "bc_src.h"
-#include "magic_dev.h"
-
-using namespace magic;
-
-#define HIPCHECK(x)                                                                   \
-    do {                                                                              \
-        hipError_t e_ = (x);                                                          \
-        if (e_ != hipSuccess) {                                                       \
-            fprintf(stderr, "magic_hip: %s failed: %s (%s:%d)\n", #x,                 \
-                    hipGetErrorString(e_), __FILE__, __LINE__);                       \
-            return MAGIC_EHIP;                                                        \
-        }                                                                             \
-    } while (0)
-
-#define TRY(x)                        \
-    do {                              \
-        int s_ = (x);                 \
-        if (s_ != MAGIC_OK) return s_; \
-    } while (0)
-
-enum { AX_RHO = 0, AX_MX, AX_MY, AX_MZ, AX_E, AX_NU, AX_KAP, AX_VG, AX_R, AX_GAM, NAUXP };
-enum { TC_SWX = 0, TC_SWY, TC_SWZ, TC_BC, TC_SRC, TC_OTHER, NTC };
-
-struct TimedSpan {
-    int cls;
-    hipEvent_t a, b;
-    int nlaunch;
-};
-
-struct magic_ctx {
-    int form = MAGIC_FORM_FP64;
-    int mx, my, mz, meqn, mbc, maux;
-    double xlower, ylower, zlower, dx, dy, dz;
-    Dims d;
-    long long P;
-    hipStream_t stream = nullptr;
-    double *q = nullptr, *qold = nullptr, *aux = nullptr, *zratio = nullptr;
-    double *stage = nullptr, *u = nullptr, *qt = nullptr, *ut = nullptr;
-    // FP32 perturbation form: state planes and scratch, background columns
-    // (float: rho, E, gamma, R, aux17, z residual, nu, kappa; FP64: rho, E)
-    float *q32 = nullptr, *qold32 = nullptr, *u32 = nullptr, *qt32 = nullptr, *ut32 = nullptr;
-    float *col32 = nullptr;
-    double *bg64 = nullptr;
-    Zr32 zr32;
-    unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
-    unsigned long long *hscal = nullptr;   // pinned mirror
-    magic_params prm;
-    bool have_params = false, have_aux = false;
-    double aux11_top = 0.0, aux12_top = 0.0;
-    int errloc[4] = {0, 0, 0, 0};
-    bool timing = false;
-    double ms[NTC] = {0};
-    long long launches[NTC] = {0};
-    std::vector<TimedSpan> pending;
-    std::vector<hipEvent_t> pool;
-};
-
-// ------------------------------------------------------------ timing -----
-static hipEvent_t ev_get(magic_ctx *c)
-{
-    hipEvent_t e;
-    if (!c->pool.empty()) {
-        e = c->pool.back();
-        c->pool.pop_back();
-    } else if (hipEventCreate(&e) != hipSuccess) {
-        return nullptr;
-    }
-    return e;
-}
-
-struct Span {
-    magic_ctx *c;
-    int cls, n;
-    hipEvent_t a = nullptr;
-    Span(magic_ctx *c_, int cls_, int n_) : c(c_), cls(cls_), n(n_)
-    {
-        if (c->timing && (a = ev_get(c))) hipEventRecord(a, c->stream);
-    }
-    ~Span()
-    {
-        if (!c->timing) {
-            return;
-        }
-        hipEvent_t b = ev_get(c);
-        if (a && b) {
-            hipEventRecord(b, c->stream);
-            c->pending.push_back({cls, a, b, n});
-        }
-    }
-};
-
-static void drain_timing(magic_ctx *c)
-{
-    for (auto &s : c->pending) {
-        float ms = 0.f;
-        if (hipEventSynchronize(s.b) == hipSuccess && hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
-            c->ms[s.cls] += ms;
-            c->launches[s.cls] += s.nlaunch;
-        }
-        c->pool.push_back(s.a);
-        c->pool.push_back(s.b);
-    }
-    c->pending.clear();
-}
-
-// ---------------------------------------------------------- helpers ------
-static inline unsigned blocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
-
-static int launch_check() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        fprintf(stderr, "magic_hip: kernel launch failed: %s\n", hipGetErrorString(e));
-        return MAGIC_EHIP;
-    }
-    return MAGIC_OK;
-}
-
-static int ntracer(const magic_ctx *c) { return c->meqn - 5; }
-
-static bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
+#include "ctx.h"
 
 extern "C" const char *magic_strerror(int s)
 {
@@ -151,7 +22,6 @@ extern "C" const char *magic_strerror(int s)
     }
 }
 
-enum { C32_RHO = 0, C32_E, C32_GAM, C32_R, C32_VG, C32_RES, C32_NU, C32_KAP, NC32 };
 
 extern "C" int magic_create(int mx, int my, int mz, int meqn, int mbc, int maux, double xlower,
                             double ylower, double zlower, double dx, double dy, double dz,
@@ -252,7 +122,7 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
     hipFree(c->col32); hipFree(c->bg64);
     if (c->hscal) hipHostFree(c->hscal);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return MAGIC_OK;
 }
@@ -271,26 +141,36 @@ extern "C" int magic_set_params(magic_ctx *c, const magic_params *p)
 // + rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
 // evaluates it for a state at rest; zero to FP64 rounding), FP64 rho_bar and
 // E_bar for the conversions, and the bc3 z factors per layer.
-static int set_aux32(magic_ctx *c, const double *aux)
+static const int AUX_SLOT[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
+
+// FP32 form: the background must be one column at rest.  Keeps float
+// columns of rho_bar, E_bar, gamma, R, aux17, nu, kappa, the background's own
+// z jump of the well-balanced f-wave (p_bar(k) - p_bar(k-1) - .5 (rho_bar(k)
+// + rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
+// evaluates it for a state at rest; zero to FP64 rounding), FP64 rho_bar and
+// E_bar for the conversions, and the bc3 z factors per layer.  get(slot
+// index, cell) returns the background value of a padded cell.
+template <class F>
+static int set_aux32(magic_ctx *c, F get, bool check_columns)
 {
-    static const int slot[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
-    const int NX = c->d.NX, NY = c->d.NY, NZ = c->d.NZ, mbc = c->mbc, mz = c->mz, maux = c->maux;
+    const int NX = c->d.NX, NY = c->d.NY, NZ = c->d.NZ, mbc = c->mbc, mz = c->mz;
     std::vector<double> col((size_t)NAUXP * NZ);
     for (int kk = 0; kk < NZ; kk++) {
-        const size_t c0 = (size_t)kk * NY * NX;
+        const long long c0 = (long long)kk * NY * NX;
         for (int s = 0; s < NAUXP; s++) {
-            const double v0 = aux[c0 * maux + slot[s] - 1];
+            const double v0 = get(s, c0);
             if ((s == AX_MX || s == AX_MY || s == AX_MZ) && v0 != 0.0) {
                 fprintf(stderr, "magic_hip: FP32 form needs a background at rest (aux slot %d = %g)\n",
-                        slot[s], v0);
+                        AUX_SLOT[s], v0);
                 return MAGIC_EUNSUPPORTED;
             }
-            for (size_t e = 0; e < (size_t)NY * NX; e++)
-                if (aux[(c0 + e) * maux + slot[s] - 1] != v0) {
-                    fprintf(stderr, "magic_hip: FP32 form needs a background column (aux slot %d varies "
-                                    "in layer %d)\n", slot[s], kk + 1 - mbc);
-                    return MAGIC_EUNSUPPORTED;
-                }
+            if (check_columns)
+                for (long long e = 0; e < (long long)NY * NX; e++)
+                    if (get(s, c0 + e) != v0) {
+                        fprintf(stderr, "magic_hip: FP32 form needs a background column (aux slot %d varies "
+                                        "in layer %d)\n", AUX_SLOT[s], kk + 1 - mbc);
+                        return MAGIC_EUNSUPPORTED;
+                    }
             col[(size_t)s * NZ + kk] = v0;
         }
     }
@@ -328,18 +208,15 @@ static int set_aux32(magic_ctx *c, const double *aux)
     return MAGIC_OK;
 }
 
-extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
+// FP64 form: the ten slots as planes, the bc3 z factors per column.
+template <class F>
+static int set_aux64(magic_ctx *c, F get)
 {
-    if (!c || !aux) return MAGIC_EARG;
-    if (c->form == MAGIC_FORM_FP32P) return set_aux32(c, aux);
-    static const int slot[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
     const long long P = c->P;
-    const int maux = c->maux;
     std::vector<double> h((size_t)P * NAUXP);
     for (int s = 0; s < NAUXP; s++) {
         double *dst = h.data() + (size_t)s * P;
-        const double *src = aux + (slot[s] - 1);
-        for (long long k = 0; k < P; k++) dst[k] = src[k * maux];
+        for (long long k = 0; k < P; k++) dst[k] = get(s, k);
     }
     HIPCHECK(hipMemcpyAsync(c->aux, h.data(), (size_t)P * NAUXP * sizeof(double), hipMemcpyHostToDevice,
                             c->stream));
@@ -363,10 +240,29 @@ extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
     HIPCHECK(hipStreamSynchronize(c->stream));
     // substep counts read the top cell of column (1,1): src3.f90:315, 319
     const long long c11 = ((long long)(mz + mbc - 1) * NY + (1 + mbc - 1)) * NX + (1 + mbc - 1);
-    c->aux11_top = aux[c11 * maux + 10];
-    c->aux12_top = aux[c11 * maux + 11];
+    c->aux11_top = get(AX_NU, c11);
+    c->aux12_top = get(AX_KAP, c11);
     c->have_aux = true;
     return MAGIC_OK;
+}
+
+extern "C" int magic_set_aux(magic_ctx *c, const double *aux)
+{
+    if (!c || !aux) return MAGIC_EARG;
+    const int maux = c->maux;
+    auto get = [&](int s, long long cell) { return aux[cell * maux + AUX_SLOT[s] - 1]; };
+    if (c->form == MAGIC_FORM_FP32P) return set_aux32(c, get, true);
+    return set_aux64(c, get);
+}
+
+extern "C" int magic_set_aux_column(magic_ctx *c, const double *col)
+{
+    if (!c || !col) return MAGIC_EARG;
+    const int maux = c->maux;
+    const long long NXY = (long long)c->d.NX * c->d.NY;
+    auto get = [&](int s, long long cell) { return col[(cell / NXY) * maux + AUX_SLOT[s] - 1]; };
+    if (c->form == MAGIC_FORM_FP32P) return set_aux32(c, get, false);
+    return set_aux64(c, get);
 }
 
 static dim3 padded_grid(const magic_ctx *c)
@@ -429,29 +325,29 @@ static int bc3_launch(magic_ctx *c, const int *mthbc)
     if (c->form == MAGIC_FORM_FP32P) {
         if (mthbc[0] >= 0 || mthbc[1] >= 0) {
             k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
-                                                                     lohi(mthbc[1]));
+                                                                     lohi(mthbc[1]), NoGate{});
             TRY(launch_check());
         }
         if (mthbc[2] >= 0 || mthbc[3] >= 0) {
             k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
-                                                                     lohi(mthbc[3]));
+                                                                     lohi(mthbc[3]), NoGate{});
             TRY(launch_check());
         }
         if (mthbc[4] >= 0 || mthbc[5] >= 0) {
             k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, c->stream>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
-                                                                       lohi(mthbc[5]), c->zr32);
+                                                                       lohi(mthbc[5]), c->zr32, NoGate{});
             TRY(launch_check());
         }
         return MAGIC_OK;
     }
     if (mthbc[0] >= 0 || mthbc[1] >= 0) {
         k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, c->stream>>>(d, c->q, c->meqn, lohi(mthbc[0]),
-                                                                 lohi(mthbc[1]));
+                                                                 lohi(mthbc[1]), NoGate{});
         TRY(launch_check());
     }
     if (mthbc[2] >= 0 || mthbc[3] >= 0) {
         k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, c->stream>>>(d, c->q, c->meqn, lohi(mthbc[2]),
-                                                                 lohi(mthbc[3]));
+                                                                 lohi(mthbc[3]), NoGate{});
         TRY(launch_check());
     }
     if (mthbc[4] >= 0 || mthbc[5] >= 0) {
@@ -476,8 +372,8 @@ static int b4step3_launch(magic_ctx *c)
 {
     if (c->meqn > 5) {
         Span sp(c, TC_OTHER, 1);
-        if (c->form == MAGIC_FORM_FP32P) k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q32, c->P, c->meqn);
-        else k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->P, c->meqn);
+        if (c->form == MAGIC_FORM_FP32P) k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q32, c->P, c->meqn, NoGate{});
+        else k_clamp<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->P, c->meqn, NoGate{});
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -725,7 +621,7 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
                        c->col32 + C32_R * NZ};
     {
         Span sp(c, TC_SRC, 1);
-        k32_src_convert<<<padded_grid(c), 64, 0, c->stream>>>(d, c->q32, col);
+        k32_src_convert<<<padded_grid(c), 64, 0, c->stream>>>(d, c->q32, col, NoGate{});
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -733,14 +629,14 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         Span sp(c, TC_SRC, 13 + 6 * (ndtrc > 0 ? ndtrc : 0));
         const double fourthirds = 4.0 / 3.0;
         double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
-        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q32);
+        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q32, NoGate{});
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->u32, c->q32 + P, 3 * P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<float, true><<<gint, 64, 0, c->stream>>>(d, c->q32, c->u32, nu,
+        k_diff_cross<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->q32, c->u32, nu,
                                                              (float)(dtr32 / (c->dx * c->dy)),
                                                              (float)(dtr32 / (c->dx * c->dz)),
-                                                             (float)(dtr32 / (c->dy * c->dz)));
+                                                             (float)(dtr32 / (c->dy * c->dz)), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             if (cc == 1) {
@@ -753,9 +649,9 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
                 dtdy2 = dtdy2 / fourthirds;
             }
             float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
-            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0);
-            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1);
-            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2);
+            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0, NoGate{});
+            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1, NoGate{});
+            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2, NoGate{});
             TRY(launch_check());
         }
         const float cx = (float)((dtrc) / (c->dx * c->dx)), cy = (float)((dtrc) / (c->dy * c->dy)),
@@ -763,25 +659,25 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         float *q5 = c->q32 + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut32, q5, P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cx, 0);
-            k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, cy, 1);
-            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut32);
+            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cx, 0, NoGate{});
+            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, cy, 1, NoGate{});
+            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut32, NoGate{});
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
-                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut32, S);
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut32, S, NoGate{});
             }
             if (k < ndtrc) {
-                k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cz, 2);
-                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, 1);
+                k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cz, 2, NoGate{});
+                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, 1, NoGate{});
             } else {
-                k_pass<float, true><<<gint, 64, 0, c->stream>>>(d, q5, c->ut32, c->ut32, kap, cz, 2);
+                k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, q5, c->ut32, c->ut32, kap, cz, 2, NoGate{});
             }
             TRY(launch_check());
         }
     }
     {
         Span sp(c, TC_SRC, 1);
-        k32_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q32, c->meqn, col, f);
+        k32_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q32, c->meqn, col, f, NoGate{});
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -843,12 +739,12 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         // output and whose ghost input is u (stale, as in the reference).
         const double fourthirds = 4.0 / 3.0;
         double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
-        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q);
+        k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q, NoGate{});
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->u, c->q + P, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<double, false><<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
-                                                 dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz));
+        k_diff_cross<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
+                                                 dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             if (cc == 1) {
@@ -861,9 +757,9 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdy2 = dtdy2 / fourthirds;
             }
             double *uc = c->u + (cc - 1) * P, *qc = c->q + cc * P;
-            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0);
-            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1);
-            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2);
+            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0, NoGate{});
+            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1, NoGate{});
+            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2, NoGate{});
             TRY(launch_check());
         }
         // conduct: conduct.f:245-332.  ut = q(5) on every cell once; the
@@ -874,18 +770,18 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         double *q5 = c->q + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut, q5, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0);
-            k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1);
-            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut);
+            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0, NoGate{});
+            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1, NoGate{});
+            k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut, NoGate{});
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
-                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S);
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S, NoGate{});
             }
             if (k < ndtrc) {
-                k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2);
-                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1);
+                k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2, NoGate{});
+                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1, NoGate{});
             } else {
-                k_pass<double, false><<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2);
+                k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2, NoGate{});
             }
             TRY(launch_check());
         }

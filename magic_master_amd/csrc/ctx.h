@@ -1,0 +1,136 @@
+// Internal declarations shared by the C ABI (capi.hip) and the batched
+// driver (batch.hip): the per-grid context, timing spans and host helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/magic_hip.h"
+#include "bc_src.h"
+#include "magic_dev.h"
+
+using namespace magic;
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            fprintf(stderr, "magic_hip: %s failed: %s (%s:%d)\n", #x,                 \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                       \
+            return MAGIC_EHIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define TRY(x)                        \
+    do {                              \
+        int s_ = (x);                 \
+        if (s_ != MAGIC_OK) return s_; \
+    } while (0)
+
+enum { AX_RHO = 0, AX_MX, AX_MY, AX_MZ, AX_E, AX_NU, AX_KAP, AX_VG, AX_R, AX_GAM, NAUXP };
+enum { TC_SWX = 0, TC_SWY, TC_SWZ, TC_BC, TC_SRC, TC_OTHER, NTC };
+
+struct TimedSpan {
+    int cls;
+    hipEvent_t a, b;
+    int nlaunch;
+};
+
+struct magic_ctx {
+    int form = MAGIC_FORM_FP64;
+    int mx, my, mz, meqn, mbc, maux;
+    double xlower, ylower, zlower, dx, dy, dz;
+    Dims d;
+    long long P;
+    hipStream_t stream = nullptr;
+    double *q = nullptr, *qold = nullptr, *aux = nullptr, *zratio = nullptr;
+    double *stage = nullptr, *u = nullptr, *qt = nullptr, *ut = nullptr;
+    // FP32 perturbation form: state planes and scratch, background columns
+    // (float: rho, E, gamma, R, aux17, z residual, nu, kappa; FP64: rho, E)
+    float *q32 = nullptr, *qold32 = nullptr, *u32 = nullptr, *qt32 = nullptr, *ut32 = nullptr;
+    float *col32 = nullptr;
+    double *bg64 = nullptr;
+    Zr32 zr32;
+    bool own_stream = true;   // false when a batch lends its stream
+    unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
+    unsigned long long *hscal = nullptr;   // pinned mirror
+    magic_params prm;
+    bool have_params = false, have_aux = false;
+    double aux11_top = 0.0, aux12_top = 0.0;
+    int errloc[4] = {0, 0, 0, 0};
+    bool timing = false;
+    double ms[NTC] = {0};
+    long long launches[NTC] = {0};
+    std::vector<TimedSpan> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+// ------------------------------------------------------------ timing -----
+static inline hipEvent_t ev_get(magic_ctx *c)
+{
+    hipEvent_t e;
+    if (!c->pool.empty()) {
+        e = c->pool.back();
+        c->pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    return e;
+}
+
+struct Span {
+    magic_ctx *c;
+    int cls, n;
+    hipEvent_t a = nullptr;
+    Span(magic_ctx *c_, int cls_, int n_) : c(c_), cls(cls_), n(n_)
+    {
+        if (c->timing && (a = ev_get(c))) hipEventRecord(a, c->stream);
+    }
+    ~Span()
+    {
+        if (!c->timing) {
+            return;
+        }
+        hipEvent_t b = ev_get(c);
+        if (a && b) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({cls, a, b, n});
+        }
+    }
+};
+
+static inline void drain_timing(magic_ctx *c)
+{
+    for (auto &s : c->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(s.b) == hipSuccess && hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            c->ms[s.cls] += ms;
+            c->launches[s.cls] += s.nlaunch;
+        }
+        c->pool.push_back(s.a);
+        c->pool.push_back(s.b);
+    }
+    c->pending.clear();
+}
+
+// ---------------------------------------------------------- helpers ------
+static inline unsigned blocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static inline int launch_check() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "magic_hip: kernel launch failed: %s\n", hipGetErrorString(e));
+        return MAGIC_EHIP;
+    }
+    return MAGIC_OK;
+}
+
+static inline int ntracer(const magic_ctx *c) { return c->meqn - 5; }
+
+static inline bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
+
+enum { C32_RHO = 0, C32_E, C32_GAM, C32_R, C32_VG, C32_RES, C32_NU, C32_KAP, NC32 };

@@ -47,5 +47,11 @@ __global__ void sweep_kernel(SweepArgs a);
 // FP32 perturbation form.
 template <typename T, bool PERT, int NTR, bool PM>
 __global__ void tile_sweep(SweepArgsT<T> a, int a_is_g);
+// Persistent variant over a work list of tiles of several grids (items:
+// {argument block, a_is_g, bx, by}; run[argument block] points at the
+// grid's "runs in this attempt" flag; next is the work counter, 0 at launch).
+template <typename T, bool PERT, int NTR, bool PM>
+__global__ void tile_sweep_list(const SweepArgsT<T> *args, const int4 *items, int nitems, const int *const *run,
+                                int *next);
 
 }  // namespace magic

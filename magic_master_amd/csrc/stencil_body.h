@@ -10,6 +10,11 @@
 #pragma once
 #include "bc_src.h"
 
+// Every kernel below takes a gate G: NoGate (always on, factors as passed)
+// for one grid; the batched driver passes a gate that reads its scenario's
+// step-control block on the device (skip when the scenario is idle in this
+// stage, factors that depend on the scenario's dt).
+
 namespace magic {
 
 __device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
@@ -18,9 +23,10 @@ __device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
 }
 
 // x faces: bc3_mpi.f:97-312.  One thread per (j, k) of the padded face.
-template <typename T>
-__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi)
+template <typename T, class G>
+__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi, G gate)
 {
+    if (!gate.on()) return;
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
     const int k = (int)blockIdx.y + 1 - d.mbc;
     if (j > d.my + d.mbc || k > d.mz + d.mbc) return;
@@ -39,9 +45,10 @@ __global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi)
 }
 
 // y faces: bc3_mpi.f:320-536.  One thread per (i, k).
-template <typename T>
-__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi)
+template <typename T, class G>
+__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi, G gate)
 {
+    if (!gate.on()) return;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
     const int k = (int)blockIdx.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || k > d.mz + d.mbc) return;
@@ -60,9 +67,10 @@ __global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi)
 }
 
 // b4step3.f:35-45: clamp tracers to [0, 1] on every cell.
-template <typename T>
-__global__ void k_clamp(T *q, long long P, int nc)
+template <typename T, class G>
+__global__ void k_clamp(T *q, long long P, int nc, G gate)
 {
+    if (!gate.on()) return;
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P) return;
     for (int m = 5; m < nc; m++) {
@@ -72,9 +80,10 @@ __global__ void k_clamp(T *q, long long P, int nc)
 }
 
 // diffuse.f:48-54: z ghosts of q(2:4) from the first/last interior layer.
-template <typename T>
-__global__ void k_zghost_uvw(Dims d, T *q)
+template <typename T, class G>
+__global__ void k_zghost_uvw(Dims d, T *q, G gate)
 {
+    if (!gate.on()) return;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
     const int j = (int)blockIdx.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
@@ -98,9 +107,13 @@ __global__ void k_zghost_uvw(Dims d, T *q)
 
 // diffuse.f:70-118: cross-derivative terms from u (copy of q(2:4) taken
 // after the z-ghost fill); writes q(2:4) on the interior.
-template <typename T, bool COLC>
-__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz)
+template <typename T, bool COLC, class G>
+__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz, G gate)
 {
+    if (!gate.on()) return;
+    dxy = gate.f(dxy, 0);
+    dxz = gate.f(dxz, 1);
+    dyz = gate.f(dyz, 2);
     INTERIOR_IJK
     const long long sx = 1, sy = d.NX, sz = (long long)d.NX * d.NY;
     const T *u1 = u, *u2 = u + d.P, *u3 = u + 2 * d.P;
@@ -117,9 +130,10 @@ __global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T d
 }
 
 // interior copy of planes src -> dst (n planes with stride P)
-template <typename T>
-__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes)
+template <typename T, class G>
+__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes, G gate)
 {
+    if (!gate.on()) return;
     INTERIOR_IJK
     for (int m = 0; m < nplanes; m++) dst[m * d.P + c] = src[m * d.P + c];
 }
@@ -131,9 +145,11 @@ __global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes)
 // never refreshes between passes) from ghost; src may equal ghost.  The
 // second difference keeps the reference's operand order: (up-mid-mid+down)
 // along x and y, (down-mid-mid+up) along z.
-template <typename T, bool COLC>
-__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis)
+template <typename T, bool COLC, class G>
+__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate)
 {
+    if (!gate.on()) return;
+    f = gate.f(f, 0);
     INTERIOR_IJK
     long long s;
     bool lo_in, hi_in;
@@ -155,9 +171,10 @@ __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *co
 // on later rows of the same layer with those rows' own values (what the
 // in-row fill writes anyway), so this kernel fills whole rows and
 // k_conduct_zspill adds the writes that cross into the next layer.
-template <typename T>
-__global__ void k_conduct_zghost(Dims d, T *u)
+template <typename T, class G>
+__global__ void k_conduct_zghost(Dims d, T *u, G gate)
 {
+    if (!gate.on()) return;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
     const int j = (int)blockIdx.y + 1 - d.mbc;
     const int ilast = (d.mz < d.mx ? d.mz : d.mx) + d.mbc;
@@ -178,9 +195,10 @@ __global__ void k_conduct_zghost(Dims d, T *u)
 // that the rows written lie before the rows that write them).  Writes past
 // the last layer leave the reference's array; they are dropped here.
 // Layers are Fortran k indices; r counts from cell (1-mbc, 1-mbc).
-template <typename T>
-__global__ void k_conduct_zspill(Dims d, T *u, int S)
+template <typename T, class G>
+__global__ void k_conduct_zspill(Dims d, T *u, int S, G gate)
 {
+    if (!gate.on()) return;
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (r >= S) return;
     const long long NXY = (long long)d.NX * d.NY;

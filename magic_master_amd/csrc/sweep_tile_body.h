@@ -51,8 +51,10 @@ __device__ __forceinline__ void report_t(const AT &A, int code, int i, int pg, i
     }
 }
 
+// One tile (bx along the tile axis, by along the coloured axis) of one
+// colour launch; called by every lane of a 256-lane workgroup.
 template <typename T, bool PERT, int NTR, bool PM>
-__global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
+__device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is_g, int bx, int by)
 {
     constexpr int NC = 5 + NTR;
     constexpr int TA = TileShape<PM>::TA, CH = TileShape<PM>::CH, NT = TA * CH;
@@ -74,9 +76,9 @@ __global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
     const int c = PM ? (t % CH) : (t / TA);     // interface offset within the chunk
     const int nb = PM ? 1 : TA;                 // lane distance between neighbouring interfaces
     const int n = A.n;
-    const int A0 = (2 * (int)blockIdx.x + A.cA) * TA;
+    const int A0 = (2 * bx + A.cA) * TA;
     const int a = A0 + pa;                      // tile-axis coordinate of this lane's pencil
-    const int b = A.cB + 3 * (int)blockIdx.y;   // other transverse coordinate
+    const int b = A.cB + 3 * by;                // other transverse coordinate
     const bool pen = a <= A.nA + 1;
     const int pg = a_is_g ? a : b, ph = a_is_g ? b : a;   // (g, h) coordinates
     const long long pbase = (long long)(a + A.mbc - 1) * A.sA + (long long)(b + A.mbc - 1) * A.sB;
@@ -463,5 +465,32 @@ __global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
     if ((threadIdx.x & 63) == 0) atomicMax(A.cfl_bits, (unsigned long long)__double_as_longlong(cflmax));
 }
 
+
+template <typename T, bool PERT, int NTR, bool PM>
+__global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
+{
+    tile_sweep_body<T, PERT, NTR, PM>(A, a_is_g, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// Persistent form for a batch of grids: the workgroups take tiles from a
+// work list (item = {argument block, a_is_g, bx, by}) through an atomic
+// counter until it runs out; tiles of grids whose control flag (*run) is 0
+// in this attempt are skipped.
+template <typename T, bool PERT, int NTR, bool PM>
+__global__ __launch_bounds__(256) void tile_sweep_list(const SweepArgsT<T> *args, const int4 *items, int nitems,
+                                                        const int *const *run, int *next)
+{
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();   // the previous tile's readers of s_item and of the tile arrays are done
+        if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= nitems) break;
+        const int4 w = items[it];
+        if (*run[w.x] == 0) continue;
+        tile_sweep_body<T, PERT, NTR, PM>(args[w.x], w.y, w.z, w.w);
+    }
+}
 
 }  // namespace magic

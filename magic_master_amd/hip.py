@@ -37,7 +37,8 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_set_aux", "magic_set_q", "magic_get_q", "magic_bc3", "magic_b4step3",
            "magic_backup", "magic_restore", "magic_step3", "magic_src3", "magic_claw3",
            "magic_error_location", "magic_timing_enable", "magic_timing_read",
-           "magic_timing_reset"]
+           "magic_timing_reset", "magic_set_aux_column", "magic_batch_create", "magic_batch_destroy",
+           "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state"]
 
 
 class MagicError(RuntimeError):
@@ -99,6 +100,13 @@ def load_library(path: str = LIB_PATH):
     L.magic_timing_enable.argtypes = [vp, C.c_int]
     L.magic_timing_read.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
     L.magic_timing_reset.argtypes = [vp]
+    L.magic_set_aux_column.argtypes = [vp, _dp]
+    L.magic_batch_create.argtypes = [C.c_int, _ip, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.POINTER(vp)]
+    L.magic_batch_destroy.argtypes = [vp]
+    L.magic_batch_ctx.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.magic_batch_start.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int, _ip]
+    L.magic_batch_run.argtypes = [vp, C.c_int, C.c_int, _ip]
+    L.magic_batch_state.argtypes = [vp, C.c_int, _dp, _ip]
     _LIB = L
     return L
 
@@ -245,3 +253,80 @@ class MagicHIP:
         self._check(self.L.magic_timing_read(self.h, ms, n), "timing")
         names = ["sweep_x", "sweep_y", "sweep_z", "bc3", "src3", "other"]
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
+
+
+class MagicBatch:
+    """Many grids of the FP32 perturbation form advanced together with the
+    step control on the device (magic_batch_*).  Each scenario takes its
+    parameters, a background column (aux(1:maux, k) for every k) and q."""
+
+    def __init__(self, grids, params, aux_columns, qs):
+        self.L = load_library()
+        n = len(grids)
+        g0 = grids[0]
+        if any((g.meqn, g.mbc, g.maux) != (g0.meqn, g0.mbc, g0.maux) for g in grids):
+            raise ValueError("all scenarios need the same meqn, mbc, maux")
+        dims = (C.c_int * (3 * n))(*[v for g in grids for v in (g.mx, g.my, g.mz)])
+        geom = (C.c_double * (6 * n))(*[v for g in grids for v in (g.xlower, g.ylower, g.zlower,
+                                                                      g.dx, g.dy, g.dz)])
+        h = C.c_void_p()
+        self._check(self.L.magic_batch_create(n, dims, g0.meqn, g0.mbc, g0.maux, geom, FORM_FP32P,
+                                              C.byref(h)), "magic_batch_create")
+        self.h = h
+        self.grids = list(grids)
+        self.n = n
+        self.ctx = []
+        for s in range(n):
+            c = C.c_void_p()
+            self._check(self.L.magic_batch_ctx(self.h, s, C.byref(c)), "magic_batch_ctx")
+            self.ctx.append(c)
+            self._check(self.L.magic_set_params(c, C.byref(CParams.of(params[s]))), "magic_set_params")
+            col = np.ascontiguousarray(aux_columns[s], dtype=np.float64)
+            if col.shape != (grids[s].mz + 2 * grids[s].mbc, grids[s].maux):
+                raise ValueError(f"aux column shape {col.shape}")
+            self._check(self.L.magic_set_aux_column(c, _ptr(col)), "magic_set_aux_column")
+            self.set_q(s, qs[s])
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.magic_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, where):
+        if st != 0:
+            raise MagicError(st, where)
+
+    def set_q(self, s, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != self.grids[s].shape_q:
+            raise ValueError(f"q shape {q.shape} != {self.grids[s].shape_q}")
+        self._check(self.L.magic_set_q(self.ctx[s], _ptr(q)), "magic_set_q")
+
+    def get_q(self, s):
+        q = np.empty(self.grids[s].shape_q)
+        self._check(self.L.magic_get_q(self.ctx[s], _ptr(q)), "magic_get_q")
+        return q
+
+    def start(self, s, t, tend, dt, dtv, cflv, maxsteps, mthbc):
+        self._check(self.L.magic_batch_start(self.h, s, t, tend, dt, (C.c_double * 5)(*dtv),
+                                             (C.c_double * 4)(*cflv), maxsteps,
+                                             (C.c_int * 6)(*mthbc)), "magic_batch_start")
+
+    def run(self, max_attempts=100000, check_every=8):
+        done = C.c_int()
+        self._check(self.L.magic_batch_run(self.h, max_attempts, check_every, C.byref(done)),
+                    "magic_batch_run")
+        return done.value
+
+    def state(self, s):
+        r = (C.c_double * 5)()
+        i = (C.c_int * 6)()
+        self._check(self.L.magic_batch_state(self.h, s, r, i), "magic_batch_state")
+        return {"t": r[0], "dt": r[1], "cfl": r[2], "cflmax": r[3], "tend": r[4], "nsteps": i[0],
+                "attempts": i[1], "rejects": i[2], "active": i[3], "status": i[4], "ndtr": i[5]}

@@ -79,6 +79,40 @@ def run_gpu(workload, steps, warmup, form="fp64"):
                 attempts=len(log.dt), rejected=sum(not a for a in log.accepted), timed_attempts=timed)
 
 
+def run_ensemble(steps, warmup, nscen=64, seed=2024):
+    """The batched FP32 ensemble: nscen seeded scenarios (problem.ensemble),
+    all advanced by the batched driver (device-resident step control, one
+    graph replay per attempted step of every scenario).  warmup accepted
+    steps (graph capture included), then steps accepted steps of every
+    scenario timed on the host clock around magic_batch_run."""
+    from magic_master_amd import problem as P
+    from magic_master_amd.hip import MagicBatch
+
+    ens = P.ensemble(nscen, seed)
+    b = MagicBatch([e[2] for e in ens], [e[1] for e in ens], [e[3] for e in ens], None)
+    try:
+        for s, (c, p, g, col) in enumerate(ens):
+            b.start(s, 0.0, 1e300, c.dtv[0], c.dtv, c.cflv, warmup, c.mthbc)
+        b.run(max_attempts=100 * warmup, check_every=max(1, warmup))
+        for s, (c, p, g, col) in enumerate(ens):
+            st = b.state(s)
+            b.start(s, st["t"], 1e300, st["dt"], c.dtv, c.cflv, steps, c.mthbc)
+        t0 = time.perf_counter()
+        attempts = b.run(max_attempts=100 * steps, check_every=4)
+        wall = time.perf_counter() - t0
+        states = [b.state(s) for s in range(nscen)]
+    finally:
+        b.close()
+    cells = sum(g.mx * g.my * g.mz for (c, p, g, col) in ens)
+    if any(st["status"] != 0 or st["nsteps"] != steps for st in states):
+        raise RuntimeError("ensemble: a scenario did not complete its steps")
+    return dict(value=cells * steps / wall / 1e6, wall=wall, cells=cells, attempts=attempts,
+                rejected=sum(st["rejects"] for st in states), nscen=nscen,
+                dims_range=[[min(e[2].mx for e in ens), max(e[2].mx for e in ens)],
+                            [min(e[2].my for e in ens), max(e[2].my for e in ens)],
+                            [min(e[2].mz for e in ens), max(e[2].mz for e in ens)]])
+
+
 def cpu_baseline(budget_s=20.0):
     """The C port (oracle) of the same step on one host core, on a slab of the
     full-domain workload: 250 x 16 x 300 cells, same column and physics."""
@@ -129,6 +163,9 @@ def main():
                     help="state form: fp64 (the reference's arithmetic) or fp32 (perturbation form)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the FP32 batched ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (read) + 1 GiB (write), 3 times: "
                          "the known-size kernel that HBM byte counters are scaled against")
@@ -192,6 +229,16 @@ def main():
                      "bytes_per_cell_update": hyp_b + src_b,
                      "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5)},
     }
+    if not a.no_ensemble:
+        e = run_ensemble(a.ensemble_steps, 2)
+        out["fp32_batched"] = {
+            "value": round(e["value"], 3), "unit": "Mcell-updates/s", "dtype": "f32",
+            "vs_fp64": round(e["value"] / value, 3),
+            "workload": (f"{e['nscen']} seeded scenarios (nx, ny, nz in {e['dims_range']}, "
+                         f"{e['cells'] / 1e6:.1f} M cells in all), FP32 perturbation form, batched "
+                         "driver with device-resident step control and one graph replay per attempt"),
+            "steps_per_scenario": a.ensemble_steps, "attempts": e["attempts"], "rejected": e["rejected"],
+            "wall_s": round(e["wall"], 4)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_budget)
     print(json.dumps(out), flush=True)

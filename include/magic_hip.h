@@ -90,6 +90,9 @@ int magic_set_aux(magic_ctx *ctx, const double *aux);
 int magic_set_aux_column(magic_ctx *ctx, const double *col);
 /* Upload / download q (host, reference layout, ghost cells included). */
 int magic_set_q(magic_ctx *ctx, const double *q);
+/* qinit (qinit.f:74-82) without noise: q = aux(1:5) on every cell (fluid at
+ * rest on the background), tracers 0.  Needs the aux first. */
+int magic_qinit_rest(magic_ctx *ctx);
 int magic_get_q(magic_ctx *ctx, double *q);
 
 /* bc3 (bc3_mpi.f:5-812): fill the ghost layers of q on the device.

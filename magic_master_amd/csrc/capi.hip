@@ -293,6 +293,24 @@ extern "C" int magic_set_q(magic_ctx *c, const double *q)
     return MAGIC_OK;
 }
 
+// qinit.f:74-82: q = aux(1:5) (fluid at rest on the background), tracers 0,
+// on every cell including ghosts.
+extern "C" int magic_qinit_rest(magic_ctx *c)
+{
+    if (!c || !c->have_aux) return MAGIC_EARG;
+    const size_t P = (size_t)c->P, n = P * c->meqn;
+    if (c->form == MAGIC_FORM_FP32P) {
+        HIPCHECK(hipMemsetAsync(c->q32, 0, n * sizeof(float), c->stream));
+        HIPCHECK(hipMemsetAsync(c->qold32, 0, n * sizeof(float), c->stream));
+    } else {
+        HIPCHECK(hipMemsetAsync(c->q, 0, n * sizeof(double), c->stream));
+        HIPCHECK(hipMemcpyAsync(c->q, c->aux, 5 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipMemcpyAsync(c->qold, c->q, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
 extern "C" int magic_get_q(magic_ctx *c, double *q)
 {
     if (!c || !q) return MAGIC_EARG;

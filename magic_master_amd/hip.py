@@ -37,7 +37,7 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_set_aux", "magic_set_q", "magic_get_q", "magic_bc3", "magic_b4step3",
            "magic_backup", "magic_restore", "magic_step3", "magic_src3", "magic_claw3",
            "magic_error_location", "magic_timing_enable", "magic_timing_read",
-           "magic_timing_reset", "magic_set_aux_column", "magic_batch_create", "magic_batch_destroy",
+           "magic_timing_reset", "magic_set_aux_column", "magic_qinit_rest", "magic_batch_create", "magic_batch_destroy",
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state"]
 
 
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
     L.magic_timing_read.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
     L.magic_timing_reset.argtypes = [vp]
     L.magic_set_aux_column.argtypes = [vp, _dp]
+    L.magic_qinit_rest.argtypes = [vp]
     L.magic_batch_create.argtypes = [C.c_int, _ip, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.POINTER(vp)]
     L.magic_batch_destroy.argtypes = [vp]
     L.magic_batch_ctx.argtypes = [vp, C.c_int, C.POINTER(vp)]
@@ -258,9 +259,10 @@ class MagicHIP:
 class MagicBatch:
     """Many grids of the FP32 perturbation form advanced together with the
     step control on the device (magic_batch_*).  Each scenario takes its
-    parameters, a background column (aux(1:maux, k) for every k) and q."""
+    parameters, a background column (aux(1:maux, k) for every k) and q
+    (None: the fluid at rest on the background, qinit without noise)."""
 
-    def __init__(self, grids, params, aux_columns, qs):
+    def __init__(self, grids, params, aux_columns, qs=None):
         self.L = load_library()
         n = len(grids)
         g0 = grids[0]
@@ -285,7 +287,10 @@ class MagicBatch:
             if col.shape != (grids[s].mz + 2 * grids[s].mbc, grids[s].maux):
                 raise ValueError(f"aux column shape {col.shape}")
             self._check(self.L.magic_set_aux_column(c, _ptr(col)), "magic_set_aux_column")
-            self.set_q(s, qs[s])
+            if qs is None or qs[s] is None:
+                self._check(self.L.magic_qinit_rest(c), "magic_qinit_rest")
+            else:
+                self.set_q(s, qs[s])
 
     def close(self):
         if getattr(self, "h", None):

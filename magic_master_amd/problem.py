@@ -307,16 +307,16 @@ def setaux_isothermal(g: Grid, p: ProbData) -> np.ndarray:
     return _fill_aux_from_column(g, col)
 
 
-def temperature_standard(z: float) -> float:
+def temperature_standard(z: float, ts: float = 288.0, lapse: float = 6.5) -> float:
     """Analytic temperature column (K) of the full-domain workload:
-    288 K at the surface, -6.5 K/km to 11 km, isothermal to 20 km, +1 K/km to
-    47 km, then a tanh rise to 1000 K that is continuous at 47 km and within
-    2% of 1000 K above 150 km.  Below the surface (ghost cells) the lapse
-    rate continues."""
+    ts (288 K) at the surface, -lapse (6.5 K/km) to 11 km, isothermal to
+    20 km, +1 K/km to 47 km, then a tanh rise to 1000 K that is continuous at
+    47 km and within 2% of 1000 K above 150 km.  Below the surface (ghost
+    cells) the lapse rate continues."""
     zk = z / 1000.0
     if zk <= 11.0:
-        return 288.0 - 6.5 * zk
-    t11 = 288.0 - 6.5 * 11.0
+        return ts - lapse * zk
+    t11 = ts - lapse * 11.0
     if zk <= 20.0:
         return t11
     if zk <= 47.0:
@@ -329,14 +329,15 @@ def temperature_standard(z: float) -> float:
 
 def setaux_column(g: Grid, gamma: float = 1.4, grav: float = 9.81, R: float = 287.0,
                   p0: float = 101325.0, mu: float = 1.8e-5, prandtl: float = 0.7,
-                  temperature=temperature_standard) -> np.ndarray:
+                  temperature=temperature_standard, as_column: bool = False) -> np.ndarray:
     """Horizontally uniform FP64 background from a temperature column T(z):
     pressure integrated with the trapezoidal hydrostatic balance
         p_k - p_{k-1} = -g dz (rho_k + rho_{k-1})/2,  rho = p/(R T),
     so that aux(17) = (p_k - p_{k-1}) / ((rho_k + rho_{k-1})/2) is -g dz to
     rounding (the well-balanced form rpn3 expects, rpn3euX.f:202-208).
     nu = mu/rho and kappa = gamma*mu/(prandtl*rho) (kinematic, growing with
-    height).  p0 is the pressure at z = 0."""
+    height).  p0 is the pressure at z = 0.  as_column returns only the
+    column, aux(1:maux, k) for every k, shape (mz+2mbc, maux)."""
     nzp = g.mz + 2 * g.mbc
     zs = [g.zcell(kk + 1 - g.mbc) for kk in range(nzp)]
     T = [temperature(z) for z in zs]
@@ -360,7 +361,18 @@ def setaux_column(g: Grid, gamma: float = 1.4, grav: float = 9.81, R: float = 28
         col["nu"][kk] = mu / rho
         col["kap"][kk] = gamma * mu / (prandtl * rho)
     col["vg"] = _vgrav(col)
+    if as_column:
+        return _column_array(g, col)
     return _fill_aux_from_column(g, col)
+
+
+def _column_array(g: Grid, col) -> np.ndarray:
+    """The column as aux(1:maux, k), k = 1-mbc..mz+mbc: shape (mz+2mbc, maux)."""
+    out = np.zeros((g.mz + 2 * g.mbc, g.maux))
+    for key, slot in (("rho", AUX_RHO), ("mx", AUX_MX), ("E", AUX_E), ("nu", AUX_NU),
+                      ("kap", AUX_KAPPA), ("vg", AUX_VGRAV), ("R", AUX_R), ("gam", AUX_GAMMA)):
+        out[:, slot - 1] = np.asarray(col[key])
+    return out
 
 
 def qinit(g: Grid, aux: np.ndarray, noise: float = 0.0, seed: int = 0,
@@ -432,3 +444,38 @@ def config_full(mx=250, my=200, mz=300, tracer=True):
     aux = setaux_column(g)
     q = qinit(g, aux, tracer_blob=dict(z0=10e3, width=2e3, amp=1.0) if tracer else None)
     return c, p, g, aux, q
+
+
+def ensemble(n: int = 64, seed: int = 2024, nmin=(64, 64, 96), nmax=(200, 200, 300)):
+    """Seeded scenarios of the batched ensemble workload: nx, ny uniform in
+    [64, 200], nz uniform in [96, 300]; dx = dy = 2 km, dz = 1 km; the
+    full-domain background with surface temperature uniform in [280, 305] K
+    and lapse rate uniform in [5, 8] K/km; plume centre uniform in the
+    interior 60 % of the domain, half-width uniform in [3, 10] km, time
+    half-width uniform in [300, 900] s, amplitude log-uniform in [2, 60]
+    mm/10 min, peak time uniform in [0, 600] s; extrapolation sides and top,
+    wall bottom; one tracer; the fluid starts at rest (q = aux, qinit without
+    noise).  Returns a list of (ClawData, ProbData, Grid, aux column
+    (mz+2mbc, maux))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        mx, my = (int(rng.integers(nmin[0], nmax[0] + 1)), int(rng.integers(nmin[1], nmax[1] + 1)))
+        mz = int(rng.integers(nmin[2], nmax[2] + 1))
+        ts = float(rng.uniform(280.0, 305.0))
+        lapse = float(rng.uniform(5.0, 8.0))
+        c = ClawData(mx=mx, my=my, mz=mz, outstyle=3, nstepout=300, nstop=300, nout=300,
+                     xupper=2e3 * mx, yupper=2e3 * my, zupper=1e3 * mz,
+                     mthbc=[1, 1, 1, 1, 3, 1], meqn=6)
+        c.method[3] = 0
+        fx, fy = rng.uniform(0.2, 0.8, size=2)
+        width = float(rng.uniform(3e3, 10e3))
+        p = ProbData(loadprof=2, forcemth=3.0, amplitude=float(np.exp(rng.uniform(np.log(2.0), np.log(60.0)))),
+                     xpos=float(fx * 2e3 * mx), ypos=float(fy * 2e3 * my), xwidth=width, ywidth=width,
+                     tcenter=float(rng.uniform(0.0, 600.0)), twidth=float(rng.uniform(300.0, 900.0)),
+                     dmolec0=0.0, dtherm0=0.0, pres=101325.0)
+        g = Grid.from_claw(c)
+        col = setaux_column(g, temperature=lambda z, ts=ts, lapse=lapse: temperature_standard(z, ts, lapse),
+                            as_column=True)
+        out.append((c, p, g, col))
+    return out

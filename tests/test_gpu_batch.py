@@ -5,7 +5,8 @@
   the FP32 form: the same dt of every attempt (a rejected first attempt
   included) and the same state bit for bit, with the plume forcing off (with
   it on, the forcing's time factor is evaluated by the host's exp for a
-  single grid and by the device's exp in a batch: within 1e-6 of the peak).
+  single grid and by the device's exp in a batch: rho', T', w within 1e-5
+  of their peaks).
 * claw3's interval semantics: a scenario started at t with tend > t stops
   exactly at tend (dt clipped, claw3_mpi.f:741).
 * A scenario's result is the same alone and in a mix of scenarios, bit for

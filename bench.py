@@ -113,6 +113,19 @@ def run_ensemble(steps, warmup, nscen=64, seed=2024):
                             [min(e[2].mz for e in ens), max(e[2].mz for e in ens)]])
 
 
+def run_serving(rate, window, t_end=(60.0, 240.0), cadence=60.0, seed=7):
+    """Continuous-batching serving loop (magic_master_amd.serve): Poisson
+    arrivals of ensemble-drawn scenarios, admission against a 250 GB budget,
+    snapshots every cadence model seconds copied to the host."""
+    from magic_master_amd import serve
+    rq = serve.poisson_requests(rate, window, seed=seed, t_end=t_end, cadence=cadence)
+    res = serve.Server(budget_bytes=250e9, check_every=4, snapshots="download").run(rq)
+    bad = [o.rid for o in res["outcomes"] if o.status != 0]
+    if bad:
+        raise RuntimeError(f"serving: requests {bad} failed")
+    return res
+
+
 def cpu_baseline(budget_s=20.0):
     """The C port (oracle) of the same step on one host core, on a slab of the
     full-domain workload: 250 x 16 x 300 cells, same column and physics."""
@@ -166,6 +179,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the FP32 batched ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-window", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (read) + 1 GiB (write), 3 times: "
                          "the known-size kernel that HBM byte counters are scaled against")
@@ -239,6 +255,17 @@ def main():
                          "driver with device-resident step control and one graph replay per attempt"),
             "steps_per_scenario": a.ensemble_steps, "attempts": e["attempts"], "rejected": e["rejected"],
             "wall_s": round(e["wall"], 4)}
+    if not a.no_serving:
+        r = run_serving(a.serve_rate, a.serve_window)
+        out["serving"] = {
+            "workload": (f"Poisson arrivals at {a.serve_rate} req/s over {a.serve_window} s, scenarios drawn "
+                         "like the ensemble, end time U[60, 240] s model time, snapshot every 60 s "
+                         "(copied to the host), 250 GB admission budget, FP32 form, continuous batching"),
+            "requests": r["requests"], "makespan_s": round(r["makespan_s"], 3),
+            "mcell_updates_per_s": round(r["mcell_updates_per_s"], 3),
+            "ttc_p50_s": round(r["ttc_p50_s"], 3), "ttc_p99_s": round(r["ttc_p99_s"], 3),
+            "queue_p50_s": round(r["queue_p50_s"], 3), "queue_p99_s": round(r["queue_p99_s"], 3),
+            "max_concurrent": r["max_concurrent"]}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_budget)
     print(json.dumps(out), flush=True)

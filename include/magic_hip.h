@@ -146,10 +146,15 @@ int magic_timing_reset(magic_ctx *ctx);
  * the reference. */
 typedef struct magic_batch magic_batch;
 /* dims: (mx, my, mz) per scenario; geom: (xlower, ylower, zlower, dx, dy, dz)
- * per scenario; form must be MAGIC_FORM_FP32P. */
+ * per scenario (nscen may be 0); form must be MAGIC_FORM_FP32P. */
 int magic_batch_create(int nscen, const int *dims, int meqn, int mbc, int maux, const double *geom,
                        int form, magic_batch **out);
 int magic_batch_destroy(magic_batch *b);
+/* Continuous batching: add a scenario (a new grid, inactive until started;
+ * may reuse the slot of a removed one) or remove one (its device memory is
+ * freed).  The next magic_batch_run captures the graph again. */
+int magic_batch_add(magic_batch *b, const int dims[3], const double geom[6], int *slot);
+int magic_batch_remove(magic_batch *b, int s);
 /* The scenario's context (owned by the batch: do not destroy it). */
 int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx);
 /* Start an interval of scenario s: from t to tend (clipped like claw3 when

@@ -21,9 +21,11 @@
 using namespace magic;
 
 struct magic_batch {
-    int n = 0;
+    int n = 0;                           // slots in use (a removed scenario leaves an empty slot)
+    int cap = 0;                         // slots the device arrays hold
+    int meqn = 0, mbc = 0, maux = 0, form = 0;
     hipStream_t stream = nullptr;
-    std::vector<magic_ctx *> sc;
+    std::vector<magic_ctx *> sc;         // nullptr: empty slot
     std::vector<ScenCtl> ctl;            // host copy of the control blocks
     std::vector<int> mthbc;              // 6 per scenario
     ScenCtl *ctl_d = nullptr;
@@ -181,7 +183,7 @@ static void launch_lists(magic_batch *b, int dir, int k)
 
 static int launch_sweeps(magic_batch *b)
 {
-    const int ntr = b->sc[0]->meqn - 5;
+    const int ntr = b->meqn - 5;
     for (int dir = 0; dir < 3; dir++)
         for (int k = 0; k < 6; k++) {
             switch (ntr) {
@@ -314,6 +316,7 @@ static int record_attempt(magic_batch *b)
     TRY(launch_check());
     for (int s = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
+        if (!c) continue;
         const StepGate g = gate(b, s, GATE_RUN);
         TRY(launch_bc(b, s, &b->mthbc[6 * s], g));
         if (c->meqn > 5) {
@@ -327,6 +330,7 @@ static int record_attempt(magic_batch *b)
     TRY(launch_check());
     for (int s = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
+        if (!c) continue;
         TRY(launch_copy(b, c->q32, c->qold32, c->P * c->meqn, gate(b, s, GATE_REJECTED)));
         TRY(launch_src(b, s));
     }
@@ -341,8 +345,9 @@ static int build_lists(magic_batch *b)
     std::vector<const int *> run((size_t)b->n * 18);
     std::vector<std::vector<int4>> lists(18);
     // tiles of longer pencils first (they take longest)
-    std::vector<int> order(b->n);
-    for (int s = 0; s < b->n; s++) order[s] = s;
+    std::vector<int> order;
+    for (int s = 0; s < b->n; s++)
+        if (b->sc[s]) order.push_back(s);
     for (int dir = 0; dir < 3; dir++) {
         auto len = [&](int s) { return dir == 0 ? b->sc[s]->mx : dir == 1 ? b->sc[s]->my : b->sc[s]->mz; };
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len(x) > len(y); });
@@ -442,7 +447,7 @@ static int build_graph(magic_batch *b)
     int dev = 0, ncu = 0, opm = 1, otm = 1;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    switch (b->sc[0]->meqn - 5) {
+    switch (b->meqn - 5) {
     case 0: TRY(occupancy<0>(&opm, &otm)); break;
     case 1: TRY(occupancy<1>(&opm, &otm)); break;
     case 2: TRY(occupancy<2>(&opm, &otm)); break;
@@ -468,44 +473,96 @@ static int build_graph(magic_batch *b)
 }
 
 // ------------------------------------------------------------------ ABI --
+// device arrays for at least want slots (contents are rebuilt from the host
+// copies before the next attempt)
+static int reserve(magic_batch *b, int want)
+{
+    if (want <= b->cap) return MAGIC_OK;
+    int cap = b->cap ? b->cap : 8;
+    while (cap < want) cap *= 2;
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d);
+    if (b->ctl_pin) hipHostFree(b->ctl_pin);
+    b->ctl_d = nullptr; b->args_d = nullptr; b->run_d = nullptr; b->ctl_pin = nullptr;
+    b->cap = 0;
+    if (hipMalloc(&b->ctl_d, cap * sizeof(ScenCtl)) != hipSuccess ||
+        hipHostMalloc(&b->ctl_pin, cap * sizeof(ScenCtl), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&b->args_d, (size_t)cap * 18 * sizeof(SweepArgsT<float>)) != hipSuccess ||
+        hipMalloc(&b->run_d, (size_t)cap * 18 * sizeof(int *)) != hipSuccess)
+        return MAGIC_ENOMEM;
+    b->cap = cap;
+    b->graph_ok = false;
+    b->dirty = true;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_add(magic_batch *b, const int dims[3], const double geom[6], int *slot)
+{
+    if (!b || !dims || !geom || !slot) return MAGIC_EARG;
+    magic_ctx *c = nullptr;
+    TRY(magic_create_form(dims[0], dims[1], dims[2], b->meqn, b->mbc, b->maux, geom[0], geom[1], geom[2],
+                          geom[3], geom[4], geom[5], b->form, &c));
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+    c->stream = b->stream;
+    c->own_stream = false;
+    int s = 0;
+    while (s < b->n && b->sc[s]) s++;
+    if (s == b->n) {
+        const int st = reserve(b, b->n + 1);
+        if (st != MAGIC_OK) {
+            magic_destroy(c);
+            return st;
+        }
+        b->n++;
+        b->sc.push_back(nullptr);
+        b->ctl.push_back(ScenCtl());
+        b->mthbc.insert(b->mthbc.end(), 6, 1);
+    }
+    b->sc[s] = c;
+    memset(&b->ctl[s], 0, sizeof(ScenCtl));   // inactive until started
+    b->graph_ok = false;
+    b->dirty = true;
+    *slot = s;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_remove(magic_batch *b, int s)
+{
+    if (!b || s < 0 || s >= b->n || !b->sc[s]) return MAGIC_EARG;
+    hipStreamSynchronize(b->stream);
+    magic_destroy(b->sc[s]);
+    b->sc[s] = nullptr;
+    memset(&b->ctl[s], 0, sizeof(ScenCtl));
+    b->graph_ok = false;
+    b->dirty = true;
+    return MAGIC_OK;
+}
+
 extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc, int maux, const double *geom,
                                   int form, magic_batch **out)
 {
-    if (!out || nscen < 1 || !dims || !geom) return MAGIC_EARG;
+    if (!out || nscen < 0 || (nscen > 0 && (!dims || !geom))) return MAGIC_EARG;
     *out = nullptr;
     if (form != MAGIC_FORM_FP32P) return MAGIC_EUNSUPPORTED;
+    if (mbc < 2 || mbc > 8 || meqn < 5 || meqn > 10 || maux < 19) return MAGIC_EARG;
+    if (!sweep_supported(meqn - 5)) return MAGIC_EUNSUPPORTED;
     magic_batch *b = new magic_batch();
-    b->n = nscen;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return MAGIC_EHIP;
-    }
+    b->meqn = meqn;
+    b->mbc = mbc;
+    b->maux = maux;
+    b->form = form;
     int st = MAGIC_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
+    if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
+    if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
     for (int s = 0; s < nscen && st == MAGIC_OK; s++) {
-        magic_ctx *c = nullptr;
-        const double *g = geom + 6 * s;
-        st = magic_create_form(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2], meqn, mbc, maux, g[0], g[1], g[2],
-                               g[3], g[4], g[5], form, &c);
-        if (st == MAGIC_OK) {
-            hipStreamSynchronize(c->stream);
-            hipStreamDestroy(c->stream);
-            c->stream = b->stream;
-            c->own_stream = false;
-            b->sc.push_back(c);
-        }
+        int slot;
+        st = magic_batch_add(b, dims + 3 * s, geom + 6 * s, &slot);
     }
-    b->ctl.assign(nscen, ScenCtl());
-    b->mthbc.assign(6 * nscen, 1);
-    for (int s = 0; s < nscen; s++) memset(&b->ctl[s], 0, sizeof(ScenCtl));   // inactive until started
-    bool ok = st == MAGIC_OK &&
-              hipMalloc(&b->ctl_d, nscen * sizeof(ScenCtl)) == hipSuccess &&
-              hipHostMalloc(&b->ctl_pin, nscen * sizeof(ScenCtl), hipHostMallocDefault) == hipSuccess &&
-              hipMalloc(&b->args_d, (size_t)nscen * 18 * sizeof(SweepArgsT<float>)) == hipSuccess &&
-              hipMalloc(&b->run_d, (size_t)nscen * 18 * sizeof(int *)) == hipSuccess &&
-              hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess;
-    if (!ok) {
+    if (st != MAGIC_OK) {
         magic_batch_destroy(b);
-        return st != MAGIC_OK ? st : MAGIC_ENOMEM;
+        return st;
     }
     *out = b;
     return MAGIC_OK;
@@ -517,7 +574,8 @@ extern "C" int magic_batch_destroy(magic_batch *b)
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    for (auto c : b->sc) magic_destroy(c);
+    for (auto c : b->sc)
+        if (c) magic_destroy(c);
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -527,7 +585,7 @@ extern "C" int magic_batch_destroy(magic_batch *b)
 
 extern "C" int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx)
 {
-    if (!b || !ctx || s < 0 || s >= b->n) return MAGIC_EARG;
+    if (!b || !ctx || s < 0 || s >= b->n || !b->sc[s]) return MAGIC_EARG;
     *ctx = b->sc[s];
     b->graph_ok = false;   // the caller may change parameters, aux or q
     return MAGIC_OK;
@@ -536,7 +594,7 @@ extern "C" int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx)
 extern "C" int magic_batch_start(magic_batch *b, int s, double t, double tend, double dt, const double dtv[5],
                                  const double cflv[4], int maxsteps, const int mthbc[6])
 {
-    if (!b || s < 0 || s >= b->n || !dtv || !cflv || !mthbc || maxsteps < 1) return MAGIC_EARG;
+    if (!b || s < 0 || s >= b->n || !b->sc[s] || !dtv || !cflv || !mthbc || maxsteps < 1) return MAGIC_EARG;
     magic_ctx *c = b->sc[s];
     if (!c->have_aux || !c->have_params) return MAGIC_EARG;
     if (cflv[1] > cflv[0]) return MAGIC_EARG;
@@ -585,6 +643,10 @@ extern "C" int magic_batch_start(magic_batch *b, int s, double t, double tend, d
 extern "C" int magic_batch_run(magic_batch *b, int max_attempts, int check_every, int *attempts_done)
 {
     if (!b || max_attempts < 1 || check_every < 1) return MAGIC_EARG;
+    if (attempts_done) *attempts_done = 0;
+    bool started = false;
+    for (int s = 0; s < b->n; s++) started = started || (b->sc[s] && b->ctl[s].active && b->ctl[s].status == 0);
+    if (!started) return MAGIC_OK;
     if (!b->graph_ok) {
         // the captured kernels read the control blocks; upload before building
         HIPCHECK(hipMemcpyAsync(b->ctl_d, b->ctl.data(), b->n * sizeof(ScenCtl), hipMemcpyHostToDevice, b->stream));
@@ -609,13 +671,13 @@ extern "C" int magic_batch_run(magic_batch *b, int max_attempts, int check_every
     }
     if (attempts_done) *attempts_done = done;
     for (int s = 0; s < b->n; s++)
-        if (b->ctl[s].status != 0) return b->ctl[s].status;
+        if (b->sc[s] && b->ctl[s].status != 0) return b->ctl[s].status;
     return MAGIC_OK;
 }
 
 extern "C" int magic_batch_state(magic_batch *b, int s, double real_out[5], int int_out[6])
 {
-    if (!b || s < 0 || s >= b->n) return MAGIC_EARG;
+    if (!b || s < 0 || s >= b->n || !b->sc[s]) return MAGIC_EARG;
     const ScenCtl &h = b->ctl[s];
     if (real_out) {
         real_out[0] = h.t;

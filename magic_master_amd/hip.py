@@ -38,7 +38,8 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_backup", "magic_restore", "magic_step3", "magic_src3", "magic_claw3",
            "magic_error_location", "magic_timing_enable", "magic_timing_read",
            "magic_timing_reset", "magic_set_aux_column", "magic_qinit_rest", "magic_batch_create", "magic_batch_destroy",
-           "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state"]
+           "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
+           "magic_batch_add", "magic_batch_remove"]
 
 
 class MagicError(RuntimeError):
@@ -108,6 +109,8 @@ def load_library(path: str = LIB_PATH):
     L.magic_batch_start.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int, _ip]
     L.magic_batch_run.argtypes = [vp, C.c_int, C.c_int, _ip]
     L.magic_batch_state.argtypes = [vp, C.c_int, _dp, _ip]
+    L.magic_batch_add.argtypes = [vp, _ip, _dp, _ip]
+    L.magic_batch_remove.argtypes = [vp, C.c_int]
     _LIB = L
     return L
 
@@ -262,35 +265,50 @@ class MagicBatch:
     parameters, a background column (aux(1:maux, k) for every k) and q
     (None: the fluid at rest on the background, qinit without noise)."""
 
-    def __init__(self, grids, params, aux_columns, qs=None):
+    def __init__(self, grids, params, aux_columns, qs=None, meqn=None, mbc=None, maux=None):
         self.L = load_library()
-        n = len(grids)
-        g0 = grids[0]
-        if any((g.meqn, g.mbc, g.maux) != (g0.meqn, g0.mbc, g0.maux) for g in grids):
-            raise ValueError("all scenarios need the same meqn, mbc, maux")
-        dims = (C.c_int * (3 * n))(*[v for g in grids for v in (g.mx, g.my, g.mz)])
-        geom = (C.c_double * (6 * n))(*[v for g in grids for v in (g.xlower, g.ylower, g.zlower,
-                                                                      g.dx, g.dy, g.dz)])
+        g0 = grids[0] if grids else None
+        meqn = g0.meqn if g0 else meqn
+        mbc = g0.mbc if g0 else mbc
+        maux = g0.maux if g0 else maux
         h = C.c_void_p()
-        self._check(self.L.magic_batch_create(n, dims, g0.meqn, g0.mbc, g0.maux, geom, FORM_FP32P,
-                                              C.byref(h)), "magic_batch_create")
+        self._check(self.L.magic_batch_create(0, None, meqn, mbc, maux, None, FORM_FP32P, C.byref(h)),
+                    "magic_batch_create")
         self.h = h
-        self.grids = list(grids)
-        self.n = n
-        self.ctx = []
-        for s in range(n):
-            c = C.c_void_p()
-            self._check(self.L.magic_batch_ctx(self.h, s, C.byref(c)), "magic_batch_ctx")
-            self.ctx.append(c)
-            self._check(self.L.magic_set_params(c, C.byref(CParams.of(params[s]))), "magic_set_params")
-            col = np.ascontiguousarray(aux_columns[s], dtype=np.float64)
-            if col.shape != (grids[s].mz + 2 * grids[s].mbc, grids[s].maux):
-                raise ValueError(f"aux column shape {col.shape}")
-            self._check(self.L.magic_set_aux_column(c, _ptr(col)), "magic_set_aux_column")
-            if qs is None or qs[s] is None:
-                self._check(self.L.magic_qinit_rest(c), "magic_qinit_rest")
-            else:
-                self.set_q(s, qs[s])
+        self.meqn, self.mbc, self.maux = meqn, mbc, maux
+        self.grids = {}
+        self.ctx = {}
+        for s, g in enumerate(grids):
+            self.add(g, params[s], aux_columns[s], None if qs is None else qs[s])
+
+    def add(self, g, p, aux_column, q=None) -> int:
+        """A new scenario (inactive until start); returns its slot."""
+        if (g.meqn, g.mbc, g.maux) != (self.meqn, self.mbc, self.maux):
+            raise ValueError("all scenarios need the same meqn, mbc, maux")
+        slot = C.c_int()
+        self._check(self.L.magic_batch_add(self.h, (C.c_int * 3)(g.mx, g.my, g.mz),
+                                           (C.c_double * 6)(g.xlower, g.ylower, g.zlower, g.dx, g.dy, g.dz),
+                                           C.byref(slot)), "magic_batch_add")
+        s = slot.value
+        c = C.c_void_p()
+        self._check(self.L.magic_batch_ctx(self.h, s, C.byref(c)), "magic_batch_ctx")
+        self.grids[s] = g
+        self.ctx[s] = c
+        self._check(self.L.magic_set_params(c, C.byref(CParams.of(p))), "magic_set_params")
+        col = np.ascontiguousarray(aux_column, dtype=np.float64)
+        if col.shape != (g.mz + 2 * g.mbc, g.maux):
+            raise ValueError(f"aux column shape {col.shape}")
+        self._check(self.L.magic_set_aux_column(c, _ptr(col)), "magic_set_aux_column")
+        if q is None:
+            self._check(self.L.magic_qinit_rest(c), "magic_qinit_rest")
+        else:
+            self.set_q(s, q)
+        return s
+
+    def remove(self, s):
+        self._check(self.L.magic_batch_remove(self.h, s), "magic_batch_remove")
+        del self.grids[s]
+        del self.ctx[s]
 
     def close(self):
         if getattr(self, "h", None):

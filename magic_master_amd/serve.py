@@ -1,0 +1,232 @@
+"""Serving loop for scenario requests: continuous batching on one GPU.
+
+Requests (a grid, a background column, forcing parameters, an end time in
+model seconds and a snapshot cadence) arrive over wall-clock time.  They are
+admitted first come first served whenever their device footprint fits the
+memory budget next to the scenarios already running; every admitted scenario
+is a slot of one MagicBatch (FP32 perturbation form), so all of them advance
+together in each graph replay, and a request that arrives while others run
+joins them at the next replay.  At every snapshot time a scenario stops its
+interval (claw3's clip to the interval end), its state is copied to the host
+and handed to a writer thread (HDF5 frames with the reference's out3 layout,
+or kept in memory), and the next interval starts; at its end time the
+scenario leaves the batch and its memory is freed.
+
+Reported per request: arrival, admission and completion times (queue wait
+and time to completion), accepted and rejected steps; for the run: p50/p99
+of both, and aggregate cell-updates per second over the run.
+"""
+from __future__ import annotations
+
+import os
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import problem as P
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float              # wall-clock seconds after the start of the run
+    claw: P.ClawData
+    prob: P.ProbData
+    grid: P.Grid
+    column: np.ndarray          # background aux(1:maux, k) for every k
+    t_end: float                # model time to reach (s)
+    cadence: float = 60.0       # model seconds between snapshots
+    q0: Optional[np.ndarray] = None   # initial q (None: at rest on the background)
+
+
+@dataclass
+class Outcome:
+    rid: int
+    arrival: float
+    admitted: float = float("nan")
+    completed: float = float("nan")
+    steps: int = 0
+    rejects: int = 0
+    snapshots: List[tuple] = field(default_factory=list)   # (model time, path or None)
+    final_q: Optional[np.ndarray] = None
+    status: int = 0
+
+    @property
+    def queue_wait(self):
+        return self.admitted - self.arrival
+
+    @property
+    def time_to_completion(self):
+        return self.completed - self.arrival
+
+
+def footprint_bytes(g: P.Grid) -> int:
+    """Device memory of one scenario of the FP32 form: q' and its backup
+    (2 x 4 B x meqn), the source pass's scratch (5 x 4 B), the FP64 staging
+    of q for upload and download (8 B x meqn), per padded cell, plus the
+    columns."""
+    cells = (g.mx + 2 * g.mbc) * (g.my + 2 * g.mbc) * (g.mz + 2 * g.mbc)
+    return cells * (8 * g.meqn + 20 + 8 * g.meqn) + 12 * (g.mz + 2 * g.mbc) * 8
+
+
+class _Writer(threading.Thread):
+    """Writes snapshots on the host while the GPU keeps stepping."""
+
+    def __init__(self, outdir: Optional[str]):
+        super().__init__(daemon=True)
+        self.outdir = outdir
+        self.q = queue.Queue()
+        self.error = None
+
+    def run(self):
+        from . import h5
+        while True:
+            item = self.q.get()
+            if item is None:
+                return
+            g, qh, t, frame, rid = item
+            try:
+                d = os.path.join(self.outdir, f"req{rid:05d}")
+                os.makedirs(d, exist_ok=True)
+                h5.out3(g, qh, t, frame, d)
+            except Exception as e:   # reported by Server.run
+                self.error = e
+
+
+class Server:
+    def __init__(self, budget_bytes: float = 250e9, check_every: int = 4, snapshots: str = "hdf5",
+                 outdir: Optional[str] = None, keep_final: bool = False):
+        """snapshots: "hdf5" (frames written under outdir), "download" (copied
+        to the host and dropped) or "none"."""
+        if snapshots not in ("hdf5", "download", "none"):
+            raise ValueError(snapshots)
+        if snapshots == "hdf5" and not outdir:
+            raise ValueError("hdf5 snapshots need outdir")
+        self.budget = budget_bytes
+        self.check_every = check_every
+        self.snapshots = snapshots
+        self.outdir = outdir
+        self.keep_final = keep_final
+
+    def run(self, requests: List[Request]) -> dict:
+        from .hip import MagicBatch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        if not reqs:
+            return {"outcomes": []}
+        g0 = reqs[0].grid
+        batch = MagicBatch([], [], [], meqn=g0.meqn, mbc=g0.mbc, maux=g0.maux)
+        writer = _Writer(self.outdir) if self.snapshots == "hdf5" else None
+        if writer:
+            writer.start()
+        out = {r.rid: Outcome(r.rid, r.arrival) for r in reqs}
+        pending = list(reqs)
+        waiting: List[Request] = []
+        active = {}                  # slot -> (request, frame, footprint)
+        used = 0.0
+        cells_steps = 0
+        max_active = 0
+        t0 = time.perf_counter()
+        try:
+            while pending or waiting or active:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    waiting.append(pending.pop(0))
+                while waiting:
+                    r = waiting[0]
+                    fp = footprint_bytes(r.grid)
+                    if fp > self.budget:
+                        raise ValueError(f"request {r.rid} needs {fp:.3g} B, more than the budget")
+                    if used + fp > self.budget:
+                        break
+                    waiting.pop(0)
+                    s = batch.add(r.grid, r.prob, r.column, r.q0)
+                    c = r.claw
+                    batch.start(s, 0.0, min(r.cadence, r.t_end), c.dtv[0], c.dtv, c.cflv, 1 << 30, c.mthbc)
+                    active[s] = (r, 0, fp)
+                    used += fp
+                    out[r.rid].admitted = time.perf_counter() - t0
+                max_active = max(max_active, len(active))
+                if not active:
+                    if pending:
+                        time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                    continue
+                batch.run(max_attempts=self.check_every, check_every=self.check_every)
+                for s in list(active):
+                    r, frame, fp = active[s]
+                    st = batch.state(s)
+                    if st["status"] != 0:
+                        out[r.rid].status = st["status"]
+                        out[r.rid].completed = time.perf_counter() - t0
+                        batch.remove(s)
+                        del active[s]
+                        used -= fp
+                        continue
+                    if st["active"]:
+                        continue
+                    # end of an interval: a snapshot
+                    g = r.grid
+                    cells_steps += g.mx * g.my * g.mz * st["nsteps"]
+                    out[r.rid].steps += st["nsteps"]
+                    out[r.rid].rejects += st["rejects"]
+                    frame += 1
+                    qh = None
+                    if self.snapshots != "none" or (self.keep_final and st["t"] >= r.t_end):
+                        qh = batch.get_q(s)
+                    path = None
+                    if writer:
+                        writer.q.put((g, qh, st["t"], frame, r.rid))
+                        from . import h5
+                        path = os.path.join(self.outdir, f"req{r.rid:05d}", h5.frame_name(frame))
+                    out[r.rid].snapshots.append((st["t"], path))
+                    if st["t"] >= r.t_end:
+                        if self.keep_final:
+                            out[r.rid].final_q = qh
+                        out[r.rid].completed = time.perf_counter() - t0
+                        batch.remove(s)
+                        del active[s]
+                        used -= fp
+                    else:
+                        c = r.claw
+                        batch.start(s, st["t"], min(st["t"] + r.cadence, r.t_end), st["dt"], c.dtv, c.cflv,
+                                    1 << 30, c.mthbc)
+                        active[s] = (r, frame, fp)
+            makespan = time.perf_counter() - t0
+        finally:
+            if writer:
+                writer.q.put(None)
+                writer.join()
+            batch.close()
+        if writer and writer.error:
+            raise writer.error
+        oc = [out[r.rid] for r in reqs]
+        ttc = np.array([o.time_to_completion for o in oc])
+        qw = np.array([o.queue_wait for o in oc])
+        return {
+            "outcomes": oc,
+            "requests": len(oc),
+            "makespan_s": makespan,
+            "mcell_updates_per_s": cells_steps / makespan / 1e6,
+            "ttc_p50_s": float(np.percentile(ttc, 50)), "ttc_p99_s": float(np.percentile(ttc, 99)),
+            "queue_p50_s": float(np.percentile(qw, 50)), "queue_p99_s": float(np.percentile(qw, 99)),
+            "max_concurrent": max_active,
+        }
+
+
+def poisson_requests(rate: float, window: float, seed: int = 7, t_end=(600.0, 6000.0),
+                     cadence: float = 60.0, nmin=(64, 64, 96), nmax=(200, 200, 300)) -> List[Request]:
+    """Requests arriving as a Poisson process of the given rate (per second)
+    over window seconds; each scenario is drawn like the ensemble workload
+    (problem.ensemble) with an end time uniform in t_end (model seconds)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    times = []
+    t = float(rng.exponential(1.0 / rate))
+    while t < window:
+        times.append(t)
+        t += float(rng.exponential(1.0 / rate))
+    scen = P.ensemble(len(times), seed=seed + 1, nmin=nmin, nmax=nmax)
+    return [Request(k, times[k], c, p, g, col, float(rng.uniform(*t_end)), cadence)
+            for k, (c, p, g, col) in enumerate(scen)]

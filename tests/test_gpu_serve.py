@@ -1,0 +1,50 @@
+"""The serving loop on the GPU (continuous batching of scenario requests).
+
+Small requests arrive while others run; every request completes, its
+snapshots are the reference's HDF5 frames at every cadence multiple up to its
+end time (the last one exactly at the end time), at most as many scenarios
+run at once as the memory budget admits, and each request's final state in
+the mix equals the same request served alone, bit for bit."""
+import numpy as np
+import pytest
+
+from magic_master_amd import h5, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    rq = serve.poisson_requests(40.0, 0.2, seed=11, t_end=(20.0, 45.0), cadence=10.0,
+                                nmin=(8, 8, 12), nmax=(16, 14, 20))
+    assert len(rq) >= 3
+    return rq
+
+
+def test_serving_loop_completes_and_snapshots(tmp_path):
+    rq = _requests()
+    print(f"[serve] {len(rq)} requests", flush=True)
+    fp = max(serve.footprint_bytes(r.grid) for r in rq)
+    res = serve.Server(budget_bytes=2.5 * fp, check_every=2, snapshots="hdf5", outdir=str(tmp_path),
+                       keep_final=True).run(rq)
+    print({k: v for k, v in res.items() if k != "outcomes"}, flush=True)
+    assert res["max_concurrent"] <= 2
+    for r, o in zip(sorted(rq, key=lambda r: r.arrival), res["outcomes"]):
+        assert o.status == 0 and o.completed >= o.admitted >= o.arrival
+        times = [t for t, _ in o.snapshots]
+        expect = list(np.arange(10.0, r.t_end, 10.0)) + [r.t_end]
+        assert np.allclose(times, expect, rtol=0, atol=1e-9), (times, expect)
+        for t, path in o.snapshots:
+            f = h5.in3(path)
+            assert f["params"][2] == t and f["dims"] == (r.grid.mz, r.grid.my, r.grid.mx, r.grid.meqn)
+        m = r.grid.mbc
+        np.testing.assert_array_equal(f["data"], o.final_q[m:m + r.grid.mz, m:m + r.grid.my, m:m + r.grid.mx])
+
+
+def test_each_result_alone_equals_in_the_mix():
+    rq = _requests()
+    mix = serve.Server(snapshots="none", keep_final=True).run(rq)["outcomes"]
+    for r, o in zip(sorted(rq, key=lambda r: r.arrival), mix):
+        r.arrival = 0.0
+        alone = serve.Server(snapshots="none", keep_final=True).run([r])["outcomes"][0]
+        assert alone.steps == o.steps
+        np.testing.assert_array_equal(alone.final_q, o.final_q)

@@ -1,0 +1,43 @@
+"""Host logic of the serving loop (no GPU): seeded Poisson arrivals, the
+scenario draws and the device-footprint estimate used for admission."""
+import numpy as np
+
+from magic_master_amd import problem as P
+from magic_master_amd import serve
+
+
+def test_poisson_requests_are_seeded_and_in_range():
+    a = serve.poisson_requests(2.0, 30.0, seed=3, t_end=(60.0, 300.0))
+    b = serve.poisson_requests(2.0, 30.0, seed=3, t_end=(60.0, 300.0))
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert [r.t_end for r in a] == [r.t_end for r in b]
+    assert 30 <= len(a) <= 90                       # 60 expected
+    arr = np.array([r.arrival for r in a])
+    assert np.all(np.diff(arr) > 0) and arr[-1] < 30.0
+    for r in a:
+        g = r.grid
+        assert 64 <= g.mx <= 200 and 64 <= g.my <= 200 and 96 <= g.mz <= 300
+        assert g.dx == 2e3 and g.dy == 2e3 and g.dz == 1e3
+        assert 60.0 <= r.t_end <= 300.0 and 2.0 <= r.prob.amplitude <= 60.0
+        assert 300.0 <= r.prob.twidth <= 900.0 and 3e3 <= r.prob.xwidth <= 10e3
+        assert 0.2 * 2e3 * g.mx <= r.prob.xpos <= 0.8 * 2e3 * g.mx
+        assert r.column.shape == (g.mz + 2 * g.mbc, g.maux)
+
+
+def test_footprint_counts_state_scratch_and_staging():
+    g = P.Grid(mx=100, my=100, mz=100, meqn=6, mbc=2, maux=32)
+    cells = 104 ** 3
+    fp = serve.footprint_bytes(g)
+    assert cells * (4 * 12 + 20 + 8 * 6) <= fp <= cells * (4 * 12 + 20 + 8 * 6) + 2e4
+
+
+def test_ensemble_column_is_hydrostatic():
+    (c, p, g, col), = P.ensemble(1, seed=5, nmin=(8, 8, 12), nmax=(12, 12, 16))
+    assert col.shape == (g.mz + 2 * g.mbc, g.maux)
+    rho, E, vg = col[:, P.AUX_RHO - 1], col[:, P.AUX_E - 1], col[:, P.AUX_VGRAV - 1]
+    assert np.all(rho > 0) and np.all(np.diff(rho) < 0)          # density falls with height
+    # aux(17) = (p_k - p_k-1) / ((rho_k + rho_k-1)/2) with p = (gamma-1) E equals -g dz
+    pbar = 0.4 * E
+    wb = (pbar[1:] - pbar[:-1]) / (0.5 * (rho[1:] + rho[:-1]))
+    np.testing.assert_allclose(vg[1:], wb, rtol=1e-12)
+    np.testing.assert_allclose(vg[1:], -9.81 * g.dz, rtol=1e-9)

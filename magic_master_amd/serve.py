@@ -129,6 +129,7 @@ class Server:
         used = 0.0
         cells_steps = 0
         max_active = 0
+        max_used = 0.0
         t0 = time.perf_counter()
         try:
             while pending or waiting or active:
@@ -150,6 +151,7 @@ class Server:
                     used += fp
                     out[r.rid].admitted = time.perf_counter() - t0
                 max_active = max(max_active, len(active))
+                max_used = max(max_used, used)
                 if not active:
                     if pending:
                         time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
@@ -213,6 +215,7 @@ class Server:
             "ttc_p50_s": float(np.percentile(ttc, 50)), "ttc_p99_s": float(np.percentile(ttc, 99)),
             "queue_p50_s": float(np.percentile(qw, 50)), "queue_p99_s": float(np.percentile(qw, 99)),
             "max_concurrent": max_active,
+            "max_footprint_bytes": max_used,
         }
 
 

@@ -46,6 +46,17 @@ SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
 }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of kernel from the committed rocprofv3 counter
+    summary (profiles/hbm_traffic_*.json), or None when there is none."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.json"))):
+        d = json.load(open(f))
+        if d.get("kernel") == kernel:
+            return d["hbm_bytes_per_launch"], os.path.relpath(f, ROOT)
+    return None, None
+
+
 def run_gpu(workload, steps, warmup, form="fp64"):
     from magic_master_amd import problem as P
     from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
@@ -216,6 +227,7 @@ def main():
                     else (HYP_BYTES_PER_CELL, SRC_BYTES_PER_CELL))
     bytes_per_launch = hyp_b * cells / launches_per_step3
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic, traffic_src = measured_traffic(kname) if a.workload == "full" else (None, None)
     st = np.array(r["step_ms"])
     out = {
         "metric": METRIC,
@@ -238,7 +250,8 @@ def main():
                     "p99": round(float(np.percentile(st, 99)), 4)},
         "kernel_ms": {k: [round(v[0], 3), int(v[1])] for k, v in r["times"].items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": kname, "avg_launch_ms": round(avg_launch_s * 1e3, 5),
                      "launches": k_n, "sweep_launches_per_step3": launches_per_step3,
                      "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -371,6 +371,7 @@ static int build_lists(magic_batch *b)
             a.cgam = c->col32 + C32_GAM * NZ;
             a.cvg = c->col32 + C32_VG * NZ;
             a.cres = c->col32 + C32_RES * NZ;
+            a.sink = (float *)c->sink;
             int a_is_g;
             bool pm;
             if (dir == 0) {          // step3.f:121-280 (g = y, h = z), tile axis y = g

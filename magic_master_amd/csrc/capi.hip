@@ -61,6 +61,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&c->stage, P * meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->dscal, 8 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipMalloc(&c->sink, 4096) == hipSuccess;
     ok = ok && hipHostMalloc(&c->hscal, 8 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
     if (ok && form == MAGIC_FORM_FP32P) {
         const size_t NZ = (size_t)c->d.NZ;
@@ -118,7 +119,7 @@ extern "C" int magic_destroy(magic_ctx *c)
     drain_timing(c);
     for (auto e : c->pool) hipEventDestroy(e);
     hipFree(c->q); hipFree(c->qold); hipFree(c->aux); hipFree(c->zratio); hipFree(c->stage);
-    hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal);
+    hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal); hipFree(c->sink);
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
     hipFree(c->col32); hipFree(c->bg64);
     if (c->hscal) hipHostFree(c->hscal);
@@ -579,6 +580,7 @@ static int step3_launch(magic_ctx *c, double dt)
         a.cgam = c->col32 + C32_GAM * NZ;
         a.cvg = c->col32 + C32_VG * NZ;
         a.cres = c->col32 + C32_RES * NZ;
+        a.sink = (float *)c->sink;
         TRY((step3_sweeps<float, true>(c, a, dt)));
     } else {
         SweepArgs a;
@@ -593,6 +595,7 @@ static int step3_launch(magic_ctx *c, double dt)
         a.cfl_bits = c->dscal;
         a.status = (int *)(c->dscal + 1);
         for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+        a.sink = (double *)c->sink;
         TRY((step3_sweeps<double, false>(c, a, dt)));
     }
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,

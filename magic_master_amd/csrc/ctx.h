@@ -57,6 +57,7 @@ struct magic_ctx {
     Zr32 zr32;
     bool own_stream = true;   // false when a batch lends its stream
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
+    void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     unsigned long long *hscal = nullptr;   // pinned mirror
     magic_params prm;
     bool have_params = false, have_aux = false;

@@ -69,7 +69,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     __shared__ T s_cfo[NT];
     __shared__ int s_flag[NT];
     __shared__ T s_cprev[TA];
-    __shared__ T LT[NC * 3 * TROW * NCELL];
+    __shared__ T LT[NC * 3 * TROW * NCELL + 1];   // + a slot that only ever holds 0
 
     const int t = threadIdx.x;
     const int pa = PM ? (t / CH) : (t % TA);    // pencil within the tile
@@ -92,7 +92,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     auto tix = [&](int m, int db, int ai, int ci) {
         return PM ? ((m * 3 + db) * TROW + ai) * NCELL + ci : ((m * 3 + db) * NCELL + ci) * TROW + ai;
     };
-    for (int e = t; e < NC * 3 * TROW * NCELL; e += NT) LT[e] = T(0.0);
+    for (int e = t; e <= NC * 3 * TROW * NCELL; e += NT) LT[e] = T(0.0);
     if (t < TA) s_cprev[t] = T(0.0);
     double cflmax = 0.0;            // Courant number, reduced in FP64
     const int nchunk = (n + 1 + (CH - 2) - 1) / (CH - 2);
@@ -435,23 +435,56 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         }
 
         // ---------------- flush the tile to q (one read-modify-write per row)
-        for (int e = t; e < NC * 3 * TROW * NCELL; e += NT) {
-            int ai, ci, db, m;
-            if (PM) {
-                ci = e % NCELL; int r = e / NCELL; ai = r % TROW; r /= TROW; db = r % 3; m = r / 3;
-            } else {
-                ai = e % TROW; int r = e / TROW; ci = r % NCELL; r /= NCELL; db = r % 3; m = r / 3;
+        // Each lane owns PPT (a offset, cell) positions of every (m, db) plane
+        // of the tile; m and db are compile-time so the plane bases come from
+        // scalar registers.  The flush is branch-free so that all loads are
+        // in flight together and no store waits for another: a position
+        // outside the grid reads and writes the lane's slot of A.sink, and
+        // its tile entry is the shared zero slot.
+        {
+            constexpr int NPOS = TROW * NCELL;
+            constexpr int PPT = (NPOS + NT - 1) / NT;
+            constexpr int ZERO = NC * 3 * NPOS;
+            long long poff[PPT];
+            bool pok[PPT];
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                const int e = t + k * NT;
+                const int ai = PM ? e / NCELL : e % TROW, ci = PM ? e % NCELL : e / TROW;
+                const int cell = S + ci;                // Fortran index along the pencil
+                const int aa = A0 - 1 + ai;             // tile-axis coordinate of the target row
+                pok[k] = e < NPOS && cell >= 1 && cell <= n && aa <= A.nA + 2;
+                poff[k] = (long long)(cell + A.mbc - 1) * A.sN + (long long)(aa + A.mbc - 1) * A.sA;
             }
-            const int cell = S + ci;                // Fortran index along the pencil
-            const int aa = A0 - 1 + ai;             // tile-axis coordinate of the target row
-            const T v = LT[e];
-            LT[e] = T(0.0);
-            if (cell >= 1 && cell <= n && aa <= A.nA + 2) {
-                const long long off = (long long)(cell + A.mbc - 1) * A.sN + (long long)(aa + A.mbc - 1) * A.sA +
-                                      (long long)(b + db - 1 + A.mbc - 1) * A.sB;
-                T *p = A.qnew + (long long)A.comp[m] * A.P + off;
-                *p = *p + v;
-            }
+            T *const sink = A.sink + t;
+            T val[NC][3][PPT];
+#pragma unroll
+            for (int m = 0; m < NC; m++)
+#pragma unroll
+                for (int db = 0; db < 3; db++) {
+                    T *base = A.qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
+#pragma unroll
+                    for (int k = 0; k < PPT; k++) val[m][db][k] = *(pok[k] ? base + poff[k] : sink);
+                }
+#pragma unroll
+            for (int m = 0; m < NC; m++)
+#pragma unroll
+                for (int db = 0; db < 3; db++)
+#pragma unroll
+                    for (int k = 0; k < PPT; k++) {
+                        const int e = t + k * NT;
+                        const int li = e < NPOS ? (m * 3 + db) * NPOS + e : ZERO;   // = tix(m, db, ai, ci)
+                        val[m][db][k] = val[m][db][k] + LT[li];
+                        LT[li] = T(0.0);
+                    }
+#pragma unroll
+            for (int m = 0; m < NC; m++)
+#pragma unroll
+                for (int db = 0; db < 3; db++) {
+                    T *base = A.qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
+#pragma unroll
+                    for (int k = 0; k < PPT; k++) *(pok[k] ? base + poff[k] : sink) = val[m][db][k];
+                }
         }
         // CFLup carried into the next chunk: last assignment among this chunk's interfaces
         if (t < TA) {
@@ -488,8 +521,12 @@ __global__ __launch_bounds__(256) void tile_sweep_list(const SweepArgsT<T> *args
         const int it = s_item;
         if (it >= nitems) break;
         const int4 w = items[it];
-        if (*run[w.x] == 0) continue;
-        tile_sweep_body<T, PERT, NTR, PM>(args[w.x], w.y, w.z, w.w);
+        // the item is the same for the whole workgroup: scalar, so that the
+        // argument block is read with scalar loads
+        const int ai = __builtin_amdgcn_readfirstlane(w.x);
+        if (*run[ai] == 0) continue;
+        tile_sweep_body<T, PERT, NTR, PM>(args[ai], __builtin_amdgcn_readfirstlane(w.y),
+                                          __builtin_amdgcn_readfirstlane(w.z), __builtin_amdgcn_readfirstlane(w.w));
     }
 }
 

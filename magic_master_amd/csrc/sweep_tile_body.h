@@ -97,6 +97,30 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     double cflmax = 0.0;            // Courant number, reduced in FP64
     const int nchunk = (n + 1 + (CH - 2) - 1) / (CH - 2);
 
+    // The states of a lane's interface in the next chunk are loaded while
+    // the current chunk's transverse terms are computed (registers), so the
+    // HBM latency of part 1 is hidden behind part 3.  Not in the FP64
+    // pencil-major kernel, where the registers would spill.
+    constexpr bool PREFETCH = !(PM && sizeof(T) == 8);
+    T pqa[NC], pqb[NC], pga = T(0.0), pgb = T(0.0), pvg = T(0.0);
+    auto fetch = [&](int Iq) {
+        if (pen && Iq <= n + 2) {
+            const long long ca = pbase + (long long)(Iq - 1 + A.mbc - 1) * A.sN;
+            const long long cb = ca + A.sN;
+#pragma unroll
+            for (int m = 0; m < NC; m++) {
+                pqa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
+                pqb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
+            }
+            if constexpr (!PERT) {
+                pga = A.gam[ca];
+                pgb = A.gam[cb];
+                if (A.gravity) pvg = A.vgrav[cb];
+            }
+        }
+    };
+    fetch(c);
+
     for (int chunk = 0; chunk < nchunk; chunk++) {
         const int S = chunk * (CH - 2);
         const int I = S + c;                    // Fortran interface index
@@ -111,14 +135,13 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         bool f1 = false, f2 = false, f5 = false;
 #pragma unroll
         for (int m = 0; m < NC; m++) { fw1[m] = fw2[m] = fw3[m] = amdq[m] = apdq[m] = T(0.0); }
+        if (!PREFETCH && chunk > 0) fetch(I);
         if (act1) {
-            const long long ca = pbase + (long long)(I - 1 + A.mbc - 1) * A.sN;
-            const long long cb = ca + A.sN;
             T qa[NC], qb[NC];
 #pragma unroll
             for (int m = 0; m < NC; m++) {
-                qa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
-                qb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
+                qa[m] = pqa[m];
+                qb[m] = pqb[m];
             }
             T ga, gb;
             // perturbation form: q holds q - background; the background is a
@@ -143,8 +166,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 ppa = (ga - T(1.0)) * (epa - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
                 ppb = (gb - T(1.0)) * (epb - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
             } else {
-                ga = A.gam[ca];
-                gb = A.gam[cb];
+                ga = pga;
+                gb = pgb;
             }
             if (!(qa[0] > T(0.0)) || !(qb[0] > T(0.0))) report_t(A, 1, I, pg, ph);
             const T rhsqrtl = sqrt(qa[0]), rhsqrtr = sqrt(qb[0]);
@@ -179,7 +202,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                     d2 = d2 - T(0.5) * (rpb + rpa) * vg + A.cres[kib];
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 } else {
-                    const T vg = A.vgrav[cb];
+                    const T vg = pvg;
                     d2 = d2 - T(0.5) * (qb[0] + qa[0]) * vg;
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 }
@@ -351,6 +374,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             else
                 cflmax = fmaxF(cflmax, fmaxF(fmaxF(c1, c2), c3));
         }
+        if (PREFETCH && chunk + 1 < nchunk) fetch(S + (CH - 2) + c);
+
         // ---------------- part 3: transverse terms, two phases
         // Each phase recomputes the two cqxx splits it needs, in an order that
         // keeps at most four split vectors alive next to the accumulator.

@@ -119,7 +119,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             }
         }
     };
-    fetch(c);
+    if constexpr (PREFETCH) fetch(c);
 
     for (int chunk = 0; chunk < nchunk; chunk++) {
         const int S = chunk * (CH - 2);
@@ -135,15 +135,32 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         bool f1 = false, f2 = false, f5 = false;
 #pragma unroll
         for (int m = 0; m < NC; m++) { fw1[m] = fw2[m] = fw3[m] = amdq[m] = apdq[m] = T(0.0); }
-        if (!PREFETCH && chunk > 0) fetch(I);
         if (act1) {
-            T qa[NC], qb[NC];
-#pragma unroll
-            for (int m = 0; m < NC; m++) {
-                qa[m] = pqa[m];
-                qb[m] = pqb[m];
-            }
+            T qa[NC], qb[NC], vgc = T(0.0);
             T ga, gb;
+            if constexpr (PREFETCH) {
+#pragma unroll
+                for (int m = 0; m < NC; m++) {
+                    qa[m] = pqa[m];
+                    qb[m] = pqb[m];
+                }
+                ga = pga;
+                gb = pgb;
+                vgc = pvg;
+            } else {
+                const long long ca = pbase + (long long)(I - 1 + A.mbc - 1) * A.sN;
+                const long long cb = ca + A.sN;
+#pragma unroll
+                for (int m = 0; m < NC; m++) {
+                    qa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
+                    qb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
+                }
+                if constexpr (!PERT) {
+                    ga = A.gam[ca];
+                    gb = A.gam[cb];
+                    if (A.gravity) vgc = A.vgrav[cb];
+                }
+            }
             // perturbation form: q holds q - background; the background is a
             // column at rest (rho_bar(k), E_bar(k), gamma(k)), so the full
             // states differ from q only in rho and E
@@ -165,9 +182,6 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 // p - p_bar = (gamma-1) (E' - kinetic energy), p_bar = (gamma-1) E_bar
                 ppa = (ga - T(1.0)) * (epa - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
                 ppb = (gb - T(1.0)) * (epb - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
-            } else {
-                ga = pga;
-                gb = pgb;
             }
             if (!(qa[0] > T(0.0)) || !(qb[0] > T(0.0))) report_t(A, 1, I, pg, ph);
             const T rhsqrtl = sqrt(qa[0]), rhsqrtr = sqrt(qb[0]);
@@ -202,7 +216,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                     d2 = d2 - T(0.5) * (rpb + rpa) * vg + A.cres[kib];
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 } else {
-                    const T vg = pvg;
+                    const T vg = vgc;
                     d2 = d2 - T(0.5) * (qb[0] + qa[0]) * vg;
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 }

@@ -39,11 +39,9 @@ SRC_BYTES_PER_CELL = 168.0
 # each pass, 2 x 4 B x meqn = 48 B at meqn = 6
 HYP_BYTES_PER_CELL_FP32 = 48.0
 SRC_BYTES_PER_CELL_FP32 = 48.0
-SWEEP_KERNELS = {   # timing class -> kernel (ntr = meqn - 5 tracers)
-    "sweep_x": "magic::tile_sweep<{t}, {pert}, {ntr}, true>",
-    "sweep_y": "magic::tile_sweep<{t}, {pert}, {ntr}, false>",
-    "sweep_z": "magic::tile_sweep<{t}, {pert}, {ntr}, false>",
-}
+SWEEP_CLASSES = ["sweep_x", "sweep_y", "sweep_z"]   # timing classes of the three directions
+# kernel of a direction (ntr = meqn - 5 tracers, sh = the direction's tile shape)
+SWEEP_KERNEL = "magic::tile_sweep<{t}, {pert}, {ntr}, {sh}>"
 
 
 def measured_traffic(kernel):
@@ -57,7 +55,7 @@ def measured_traffic(kernel):
     return None, None
 
 
-def run_gpu(workload, steps, warmup, form="fp64"):
+def run_gpu(workload, steps, warmup, form="fp64", tune=True):
     from magic_master_amd import problem as P
     from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
 
@@ -71,6 +69,11 @@ def run_gpu(workload, steps, warmup, form="fp64"):
     dev = MagicHIP(g, p, aux, q, form=FORM_FP32P if form == "fp32" else FORM_FP64)
     t0 = 0.0
     t, dtv, log = dev.run_steps(c, warmup, t0=t0)
+    tuned = None
+    if tune:   # tile shapes from the autotuner, on the warmed-up state
+        shapes, ms = dev.tune(dtv[0], reps=3)
+        tuned = {"shapes": shapes, "ms_per_sweep": [[round(v, 4) for v in row] for row in ms]}
+    shapes = dev.tile_shapes()
     dev.timing(True)
     dev.timing_reset()
     step_ms = []
@@ -86,16 +89,17 @@ def run_gpu(workload, steps, warmup, form="fp64"):
     wall = time.perf_counter() - tw0
     times = dev.timing_read()
     dev.close()
-    return dict(c=c, g=g, name=name, wall=wall, step_ms=step_ms, times=times, t=t,
+    return dict(c=c, g=g, name=name, wall=wall, step_ms=step_ms, times=times, t=t, shapes=shapes, tuned=tuned,
                 attempts=len(log.dt), rejected=sum(not a for a in log.accepted), timed_attempts=timed)
 
 
-def run_ensemble(steps, warmup, nscen=64, seed=2024):
+def run_ensemble(steps, warmup, nscen=64, seed=2024, tune=True):
     """The batched FP32 ensemble: nscen seeded scenarios (problem.ensemble),
     all advanced by the batched driver (device-resident step control, one
     graph replay per attempted step of every scenario).  warmup accepted
-    steps (graph capture included), then steps accepted steps of every
-    scenario timed on the host clock around magic_batch_run."""
+    steps, the batch autotuner's choice of tile shapes (graph capture
+    included), then steps accepted steps of every scenario timed on the host
+    clock around magic_batch_run."""
     from magic_master_amd import problem as P
     from magic_master_amd.hip import MagicBatch
 
@@ -108,6 +112,15 @@ def run_ensemble(steps, warmup, nscen=64, seed=2024):
         for s, (c, p, g, col) in enumerate(ens):
             st = b.state(s)
             b.start(s, st["t"], 1e300, st["dt"], c.dtv, c.cflv, steps, c.mthbc)
+        tuned = None
+        if tune:
+            shapes, ms = b.tune(reps=2)
+            tuned = {"shapes": shapes, "ms_per_sweep": [[round(v, 4) for v in row] for row in ms]}
+            b.run(max_attempts=1, check_every=1)   # graph capture with the chosen shapes, one attempt
+            for s, (c, p, g, col) in enumerate(ens):
+                st = b.state(s)
+                b.start(s, st["t"], 1e300, st["dt"], c.dtv, c.cflv, steps, c.mthbc)
+        shapes = b.tile_shapes()
         t0 = time.perf_counter()
         attempts = b.run(max_attempts=100 * steps, check_every=4)
         wall = time.perf_counter() - t0
@@ -117,20 +130,22 @@ def run_ensemble(steps, warmup, nscen=64, seed=2024):
     cells = sum(g.mx * g.my * g.mz for (c, p, g, col) in ens)
     if any(st["status"] != 0 or st["nsteps"] != steps for st in states):
         raise RuntimeError("ensemble: a scenario did not complete its steps")
-    return dict(value=cells * steps / wall / 1e6, wall=wall, cells=cells, attempts=attempts,
+    return dict(value=cells * steps / wall / 1e6, wall=wall, cells=cells, attempts=attempts, shapes=shapes,
+                tuned=tuned,
                 rejected=sum(st["rejects"] for st in states), nscen=nscen,
                 dims_range=[[min(e[2].mx for e in ens), max(e[2].mx for e in ens)],
                             [min(e[2].my for e in ens), max(e[2].my for e in ens)],
                             [min(e[2].mz for e in ens), max(e[2].mz for e in ens)]])
 
 
-def run_serving(rate, window, t_end=(60.0, 240.0), cadence=60.0, seed=7):
+def run_serving(rate, window, t_end=(60.0, 240.0), cadence=60.0, seed=7, shapes=None):
     """Continuous-batching serving loop (magic_master_amd.serve): Poisson
     arrivals of ensemble-drawn scenarios, admission against a 250 GB budget,
-    snapshots every cadence model seconds copied to the host."""
+    snapshots every cadence model seconds copied to the host; tile shapes
+    fixed for the whole run (the ensemble's autotuned ones)."""
     from magic_master_amd import serve
     rq = serve.poisson_requests(rate, window, seed=seed, t_end=t_end, cadence=cadence)
-    res = serve.Server(budget_bytes=250e9, check_every=4, snapshots="download").run(rq)
+    res = serve.Server(budget_bytes=250e9, check_every=4, snapshots="download", tile_shapes=shapes).run(rq)
     bad = [o.rid for o in res["outcomes"] if o.status != 0]
     if bad:
         raise RuntimeError(f"serving: requests {bad} failed")
@@ -186,6 +201,8 @@ def main():
     ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"],
                     help="state form: fp64 (the reference's arithmetic) or fp32 (perturbation form)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default tile shapes instead of the autotuner's choice")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the FP32 batched ensemble measurement")
@@ -204,7 +221,7 @@ def main():
         print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "
               "supported (the domain is not decomposed across GPUs)", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.form)
+    r = run_gpu(a.workload, a.steps, a.warmup, a.form, tune=not a.no_tune)
     g = r["g"]
     K = a.steps
     cells = g.ncell
@@ -212,15 +229,15 @@ def main():
     # dominant sweep kernel: the instantiation with the most time (y and z
     # share one); its average launch from HIP events on the library stream
     per_kernel = {}
-    for cls, kname in SWEEP_KERNELS.items():
+    for d, cls in enumerate(SWEEP_CLASSES):
         ms, n = r["times"][cls]
-        k = kname.format(t="float" if a.form == "fp32" else "double",
-                         pert="true" if a.form == "fp32" else "false", ntr=g.meqn - 5)
+        k = SWEEP_KERNEL.format(t="float" if a.form == "fp32" else "double",
+                                pert="true" if a.form == "fp32" else "false", ntr=g.meqn - 5, sh=r["shapes"][d])
         acc = per_kernel.setdefault(k, [0.0, 0])
         acc[0] += ms
         acc[1] += n
     kname, (k_ms, k_n) = max(per_kernel.items(), key=lambda kv: kv[1][0])
-    sw_n = sum(r["times"][k][1] for k in SWEEP_KERNELS)
+    sw_n = sum(r["times"][k][1] for k in SWEEP_CLASSES)
     launches_per_step3 = sw_n / max(r["timed_attempts"], 1)
     avg_launch_s = k_ms / max(k_n, 1) / 1e3
     hyp_b, src_b = ((HYP_BYTES_PER_CELL_FP32, SRC_BYTES_PER_CELL_FP32) if a.form == "fp32"
@@ -244,6 +261,7 @@ def main():
         "data": "synthetic (analytic background column, fluid at rest, plume forcing)",
         "config": {"workload": r["name"], "cells": [g.mx, g.my, g.mz], "meqn": g.meqn,
                    "precision": a.form, "parallelism": "single GPU",
+                   "tile_shapes": r["shapes"], "autotuner": r["tuned"],
                    "attempted_steps": r["attempts"], "rejected_steps": r["rejected"],
                    "model_time_end_s": round(r["t"], 3)},
         "latency": {"unit": "ms per accepted step", "p50": round(float(np.percentile(st, 50)), 4),
@@ -258,8 +276,10 @@ def main():
                      "bytes_per_cell_update": hyp_b + src_b,
                      "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5)},
     }
+    ens_shapes = None
     if not a.no_ensemble:
-        e = run_ensemble(a.ensemble_steps, 2)
+        e = run_ensemble(a.ensemble_steps, 2, tune=not a.no_tune)
+        ens_shapes = e["shapes"]
         out["fp32_batched"] = {
             "value": round(e["value"], 3), "unit": "Mcell-updates/s", "dtype": "f32",
             "vs_fp64": round(e["value"] / value, 3),
@@ -267,9 +287,10 @@ def main():
                          f"{e['cells'] / 1e6:.1f} M cells in all), FP32 perturbation form, batched "
                          "driver with device-resident step control and one graph replay per attempt"),
             "steps_per_scenario": a.ensemble_steps, "attempts": e["attempts"], "rejected": e["rejected"],
+            "tile_shapes": e["shapes"], "autotuner": e["tuned"],
             "wall_s": round(e["wall"], 4)}
     if not a.no_serving:
-        r = run_serving(a.serve_rate, a.serve_window)
+        r = run_serving(a.serve_rate, a.serve_window, shapes=ens_shapes)
         out["serving"] = {
             "workload": (f"Poisson arrivals at {a.serve_rate} req/s over {a.serve_window} s, scenarios drawn "
                          "like the ensemble, end time U[60, 240] s model time, snapshot every 60 s "
@@ -278,7 +299,7 @@ def main():
             "mcell_updates_per_s": round(r["mcell_updates_per_s"], 3),
             "ttc_p50_s": round(r["ttc_p50_s"], 3), "ttc_p99_s": round(r["ttc_p99_s"], 3),
             "queue_p50_s": round(r["queue_p50_s"], 3), "queue_p99_s": round(r["queue_p99_s"], 3),
-            "max_concurrent": r["max_concurrent"]}
+            "max_concurrent": r["max_concurrent"], "tile_shapes": ens_shapes}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_budget)
     print(json.dumps(out), flush=True)

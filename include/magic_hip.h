@@ -136,6 +136,26 @@ int magic_timing_enable(magic_ctx *ctx, int on);
 int magic_timing_read(magic_ctx *ctx, double ms[6], long long launches[6]);
 int magic_timing_reset(magic_ctx *ctx);
 
+/* ---- Tile shapes and the autotuner (no counterpart in the reference) ----
+ * The tiled sweep kernel comes in MAGIC_TILE_NSHAPE shapes: workgroups of
+ * 256 lanes covering TA neighbouring pencils x 256/TA interfaces, lanes
+ * tile-axis-major or pencil-major:
+ *   0: TA 16, tile-axis-major   1: TA 8, tile-axis-major
+ *   2: TA 8, pencil-major       3: TA 4, pencil-major
+ * shapes[0..2] are the shapes of the x-, y- and z-sweeps (default 0, 0, 0).
+ * Every shape computes the same increments; the grouping of their sums, and
+ * so the last bits of the result, depends on the shape.  magic_tune times
+ * every shape of every direction on the grid's current state (ghost cells
+ * set, dt > 0: one untimed and reps timed sweeps each), keeps the fastest
+ * per direction and returns them in shapes_out (and, if ms_out is not NULL,
+ * the mean ms per sweep in ms_out[3 * MAGIC_TILE_NSHAPE], direction-major).
+ * q is unchanged bit for bit; the backup copy then holds q.
+ * MAGIC_EUNSUPPORTED for the FP64 pencil kernel (MAGIC_SWEEP=pencil). */
+#define MAGIC_TILE_NSHAPE 4
+int magic_set_tile_shapes(magic_ctx *ctx, const int shapes[3]);
+int magic_get_tile_shapes(magic_ctx *ctx, int shapes[3]);
+int magic_tune(magic_ctx *ctx, double dt, int reps, int shapes_out[3], float *ms_out);
+
 /* ---- Batched driver (FP32 form): many grids advanced together ----------
  * Each scenario is a grid with its own state (a magic_ctx reachable through
  * magic_batch_ctx for parameters, aux and q); its step control is claw3's
@@ -168,6 +188,16 @@ int magic_batch_run(magic_batch *b, int max_attempts, int check_every, int *atte
 /* real_out: t, dt (next), cfl (last attempt), cflmax, tend;
  * int_out: accepted steps, attempts, rejected attempts, active, status, ndtr. */
 int magic_batch_state(magic_batch *b, int s, double real_out[5], int int_out[6]);
+/* Tile shapes of the batch's x-, y- and z-sweeps (one per direction for all
+ * scenarios; default 0, 0, 0); setting them captures the graph again.
+ * magic_batch_tune is the autotuner of the batch: every shape of every
+ * direction is timed as one persistent launch per tile colour over the
+ * tiles of all scenarios (their current states and next dt; one untimed
+ * and reps timed sweeps), and the fastest per direction is kept.  The
+ * scenarios' states are unchanged bit for bit. */
+int magic_batch_set_tile_shapes(magic_batch *b, const int shapes[3]);
+int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3]);
+int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], float *ms_out);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,21 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "sweep_tile32.hip", "bc_src.hip", "bc_src32.hip", "capi.hip",
                "batch.hip"]
+# the tiled sweeps are compiled once per tile shape (TILE_NSHAPE in magic_dev.h)
+PER_SHAPE = ("sweep_tile.hip", "sweep_tile32.hip")
+TILE_NSHAPE = 4
+
+
+def hip_units():
+    """(source, object name, extra flags) of every compile of the library."""
+    units = []
+    for src in HIP_SOURCES:
+        stem = os.path.splitext(src)[0]
+        if src in PER_SHAPE:
+            units += [(src, f"{stem}_s{k}.o", [f"-DMAGIC_TILE_SHAPE={k}"]) for k in range(TILE_NSHAPE)]
+        else:
+            units.append((src, stem + ".o", []))
+    return units
 HIP_LIB = os.path.join(HERE, "libmagic_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -42,11 +57,12 @@ def build_hip() -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
-    objs = [os.path.join(objdir, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
-    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    units = hip_units()
+    objs = [os.path.join(objdir, obj) for _, obj, _ in units]
+    jobs = max(1, min(len(units), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(jobs) as ex:
-        for f in [ex.submit(_run, [HIPCC] + compile_flags + ["-c", src, "-o", obj], CSRC)
-                  for src, obj in zip(HIP_SOURCES, objs)]:
+        for f in [ex.submit(_run, [HIPCC] + compile_flags + extra + ["-c", src, "-o", obj], CSRC)
+                  for (src, _, extra), obj in zip(units, objs)]:
             f.result()
     tmp = HIP_LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)

@@ -35,7 +35,9 @@ struct magic_batch {
     int4 *items_d = nullptr;
     int item_off[19] = {0};              // work list of direction d, colour k at [item_off[6d+k], ..)
     int *counters_d = nullptr;           // 18 work counters
-    int grid_pm = 0, grid_tm = 0;        // persistent grid sizes
+    int shape[3] = {0, 0, 0};            // tile shape of the x-, y- and z-sweeps
+    int grid[3] = {0, 0, 0};             // persistent grid size of each direction's kernel
+    int *one_d = nullptr;                // a device 1: the run flag of every tile while tuning
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;                   // control blocks changed on the host
@@ -168,33 +170,39 @@ static int src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
 }
 
 template <int NTR>
-static void launch_lists(magic_batch *b, int dir, int k)
+static int launch_lists(magic_batch *b, int dir, int k)
 {
-    const int lo = b->item_off[6 * dir + k], hi = b->item_off[6 * dir + k + 1];
-    if (hi <= lo) return;
+    int lo = b->item_off[6 * dir + k], n = b->item_off[6 * dir + k + 1] - lo;
+    if (n <= 0) return MAGIC_OK;
+    const void *fn = tile_list_fn<NTR>(b->shape[dir]);
+    if (!fn || b->grid[dir] < 1) return MAGIC_EARG;
+    const SweepArgsT<float> *args = b->args_d;
+    const int4 *items = b->items_d + lo;
+    const int *const *run = b->run_d;
     int *cnt = b->counters_d + 6 * dir + k;
-    if (dir == 0)
-        tile_sweep_list<float, true, NTR, true><<<b->grid_pm, 256, 0, b->stream>>>(b->args_d, b->items_d + lo, hi - lo,
-                                                                                b->run_d, cnt);
-    else
-        tile_sweep_list<float, true, NTR, false><<<b->grid_tm, 256, 0, b->stream>>>(b->args_d, b->items_d + lo, hi - lo,
-                                                                                 b->run_d, cnt);
+    void *kargs[] = {&args, &items, &n, &run, &cnt};
+    HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, b->stream));
+    return MAGIC_OK;
+}
+
+static int launch_dir(magic_batch *b, int dir)
+{
+    for (int k = 0; k < 6; k++) {
+        switch (b->meqn - 5) {
+        case 0: TRY(launch_lists<0>(b, dir, k)); break;
+        case 1: TRY(launch_lists<1>(b, dir, k)); break;
+        case 2: TRY(launch_lists<2>(b, dir, k)); break;
+        case 5: TRY(launch_lists<5>(b, dir, k)); break;
+        default: return MAGIC_EUNSUPPORTED;
+        }
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
 }
 
 static int launch_sweeps(magic_batch *b)
 {
-    const int ntr = b->meqn - 5;
-    for (int dir = 0; dir < 3; dir++)
-        for (int k = 0; k < 6; k++) {
-            switch (ntr) {
-            case 0: launch_lists<0>(b, dir, k); break;
-            case 1: launch_lists<1>(b, dir, k); break;
-            case 2: launch_lists<2>(b, dir, k); break;
-            case 5: launch_lists<5>(b, dir, k); break;
-            default: return MAGIC_EUNSUPPORTED;
-            }
-            TRY(launch_check());
-        }
+    for (int dir = 0; dir < 3; dir++) TRY(launch_dir(b, dir));
     return MAGIC_OK;
 }
 
@@ -338,8 +346,10 @@ static int record_attempt(magic_batch *b)
     return launch_check();
 }
 
-// sweep argument blocks and tile work lists (geometry only)
-static int build_lists(magic_batch *b)
+// Sweep argument blocks and tile work lists (geometry only; the step's dt
+// factors are written by k_batch_prepare).  For the autotuner (tuning) every
+// tile runs and the dt factors come from each scenario's next dt.
+static int build_lists(magic_batch *b, bool tuning = false)
 {
     std::vector<SweepArgsT<float>> args((size_t)b->n * 18);
     std::vector<const int *> run((size_t)b->n * 18);
@@ -373,27 +383,30 @@ static int build_lists(magic_batch *b)
             a.cres = c->col32 + C32_RES * NZ;
             a.sink = (float *)c->sink;
             int a_is_g;
-            bool pm;
+            const double dt = b->ctl[s].dt > 0.0 ? b->ctl[s].dt : 1.0;
             if (dir == 0) {          // step3.f:121-280 (g = y, h = z), tile axis y = g
                 a.comp[0] = 0; a.comp[1] = 1; a.comp[2] = 2; a.comp[3] = 3; a.comp[4] = 4;
                 a.n = c->mx; a.nA = c->my; a.nB = c->mz;
                 a.sN = 1; a.sA = d.NX; a.sB = NXY;
                 a.gravity = 0; a.sweep = 1;
-                a_is_g = 1; pm = true;
+                a_is_g = 1;
+                if (tuning) { a.dtdn = dt / c->dx; a.fg = dt / c->dy; a.fh = dt / c->dz; }
             } else if (dir == 1) {   // step3.f:289-450 (g = z, h = x), tile axis x = h
                 a.comp[0] = 0; a.comp[1] = 2; a.comp[2] = 3; a.comp[3] = 1; a.comp[4] = 4;
                 a.n = c->my; a.nA = c->mx; a.nB = c->mz;
                 a.sN = d.NX; a.sA = 1; a.sB = NXY;
                 a.gravity = 0; a.sweep = 2;
-                a_is_g = 0; pm = false;
+                a_is_g = 0;
+                if (tuning) { a.dtdn = dt / c->dy; a.fg = dt / c->dz; a.fh = dt / c->dx; }
             } else {                 // step3.f:459-621 (g = x, h = y), tile axis x = g
                 a.comp[0] = 0; a.comp[1] = 3; a.comp[2] = 1; a.comp[3] = 2; a.comp[4] = 4;
                 a.n = c->mz; a.nA = c->mx; a.nB = c->my;
                 a.sN = NXY; a.sA = 1; a.sB = d.NX;
                 a.gravity = 1; a.sweep = 3;
-                a_is_g = 1; pm = false;
+                a_is_g = 1;
+                if (tuning) { a.dtdn = dt / c->dz; a.fg = dt / c->dx; a.fh = dt / c->dy; }
             }
-            const int TA = pm ? 4 : 16;
+            const int TA = tile_ta(b->shape[dir]);
             const int ntiles = (a.nA + 2 + TA - 1) / TA;
             for (int ca = 0; ca < 2; ca++)
                 for (int cb = 0; cb < 3; cb++) {
@@ -402,7 +415,7 @@ static int build_lists(magic_batch *b)
                     t.cA = ca;
                     t.cB = cb;
                     args[ai] = t;
-                    run[ai] = &b->ctl_d[s].run;
+                    run[ai] = tuning ? b->one_d : &b->ctl_d[s].run;
                     const int gx = (ntiles - ca + 1) / 2, gy = (a.nB + 1 - cb) / 3 + 1;
                     for (int by = 0; by < gy; by++)
                         for (int bx = 0; bx < gx; bx++) lists[dir * 6 + k].push_back(make_int4(ai, a_is_g, bx, by));
@@ -428,13 +441,25 @@ static int build_lists(magic_batch *b)
     return MAGIC_OK;
 }
 
-template <int NTR>
-static int occupancy(int *pm, int *tm)
+// persistent grid of each direction: as many workgroups as fit on the chip
+static int list_grids(magic_batch *b)
 {
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        pm, reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, true>), 256, 0));
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        tm, reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, false>), 256, 0));
+    int dev = 0, ncu = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int dir = 0; dir < 3; dir++) {
+        const void *fn = nullptr;
+        switch (b->meqn - 5) {
+        case 0: fn = tile_list_fn<0>(b->shape[dir]); break;
+        case 1: fn = tile_list_fn<1>(b->shape[dir]); break;
+        case 2: fn = tile_list_fn<2>(b->shape[dir]); break;
+        case 5: fn = tile_list_fn<5>(b->shape[dir]); break;
+        default: return MAGIC_EUNSUPPORTED;
+        }
+        int occ = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
+        b->grid[dir] = ncu * (occ > 0 ? occ : 1);
+    }
     return MAGIC_OK;
 }
 
@@ -445,18 +470,7 @@ static int build_graph(magic_batch *b)
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_ok = false;
-    int dev = 0, ncu = 0, opm = 1, otm = 1;
-    HIPCHECK(hipGetDevice(&dev));
-    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    switch (b->meqn - 5) {
-    case 0: TRY(occupancy<0>(&opm, &otm)); break;
-    case 1: TRY(occupancy<1>(&opm, &otm)); break;
-    case 2: TRY(occupancy<2>(&opm, &otm)); break;
-    case 5: TRY(occupancy<5>(&opm, &otm)); break;
-    default: return MAGIC_EUNSUPPORTED;
-    }
-    b->grid_pm = ncu * (opm > 0 ? opm : 1);
-    b->grid_tm = ncu * (otm > 0 ? otm : 1);
+    TRY(list_grids(b));
     TRY(build_lists(b));
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = record_attempt(b);
@@ -557,6 +571,11 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
     if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
     if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
+    if (st == MAGIC_OK) st = hipMalloc(&b->one_d, sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
+    if (st == MAGIC_OK) {
+        const int one = 1;
+        st = hipMemcpy(b->one_d, &one, sizeof(int), hipMemcpyHostToDevice) == hipSuccess ? MAGIC_OK : MAGIC_EHIP;
+    }
     for (int s = 0; s < nscen && st == MAGIC_OK; s++) {
         int slot;
         st = magic_batch_add(b, dims + 3 * s, geom + 6 * s, &slot);
@@ -578,6 +597,7 @@ extern "C" int magic_batch_destroy(magic_batch *b)
     for (auto c : b->sc)
         if (c) magic_destroy(c);
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
+    hipFree(b->one_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -694,6 +714,90 @@ extern "C" int magic_batch_state(magic_batch *b, int s, double real_out[5], int 
         int_out[3] = h.active;
         int_out[4] = h.status;
         int_out[5] = h.ndtr;
+    }
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_set_tile_shapes(magic_batch *b, const int shapes[3])
+{
+    if (!b || !shapes || !shapes_valid(shapes)) return MAGIC_EARG;
+    for (int k = 0; k < 3; k++) b->shape[k] = shapes[k];
+    b->graph_ok = false;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3])
+{
+    if (!b || !shapes) return MAGIC_EARG;
+    for (int k = 0; k < 3; k++) shapes[k] = b->shape[k];
+    return MAGIC_OK;
+}
+
+// Every candidate reads the backup planes and adds into q (one list build
+// per shape, all directions); q is restored from the backup afterwards.
+// The candidates' Courant numbers and reports land in the control blocks'
+// reduction fields, which the next attempt's prepare clears.
+extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], float *ms_out)
+{
+    if (!b || reps < 1) return MAGIC_EARG;
+    bool any = false;
+    for (int s = 0; s < b->n; s++) any = any || b->sc[s] != nullptr;
+    if (!any) return MAGIC_EARG;
+    const int saved[3] = {b->shape[0], b->shape[1], b->shape[2]};
+    b->graph_ok = false;   // the lists and argument blocks are rebuilt before the next attempt
+    for (int s = 0; s < b->n; s++) {
+        magic_ctx *c = b->sc[s];
+        if (c)
+            HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, (size_t)c->P * c->meqn * sizeof(float),
+                                    hipMemcpyDeviceToDevice, b->stream));
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return MAGIC_EHIP;
+    }
+    float tms[3][TILE_NSHAPE] = {};
+    int st = MAGIC_OK;
+    for (int sh = 0; sh < TILE_NSHAPE && st == MAGIC_OK; sh++) {
+        for (int k = 0; k < 3; k++) b->shape[k] = sh;
+        st = list_grids(b);
+        if (st == MAGIC_OK) st = build_lists(b, true);
+        for (int dir = 0; dir < 3 && st == MAGIC_OK; dir++) {
+            for (int r = 0; r <= reps && st == MAGIC_OK; r++) {
+                if (r == 1 && hipEventRecord(e0, b->stream) != hipSuccess) st = MAGIC_EHIP;
+                if (st == MAGIC_OK && hipMemsetAsync(b->counters_d, 0, 18 * sizeof(int), b->stream) != hipSuccess)
+                    st = MAGIC_EHIP;
+                if (st == MAGIC_OK) st = launch_dir(b, dir);
+            }
+            float ms = 0.f;
+            if (st == MAGIC_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                st = MAGIC_EHIP;
+            tms[dir][sh] = ms / (float)reps;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (int s = 0; s < b->n && st == MAGIC_OK; s++) {
+        magic_ctx *c = b->sc[s];
+        if (c && hipMemcpyAsync(c->q32, c->qold32, (size_t)c->P * c->meqn * sizeof(float), hipMemcpyDeviceToDevice,
+                                b->stream) != hipSuccess)
+            st = MAGIC_EHIP;
+    }
+    if (st == MAGIC_OK && hipStreamSynchronize(b->stream) != hipSuccess) st = MAGIC_EHIP;
+    if (st != MAGIC_OK) {
+        for (int k = 0; k < 3; k++) b->shape[k] = saved[k];
+        return st;
+    }
+    for (int dir = 0; dir < 3; dir++) {
+        int best = 0;
+        for (int sh = 1; sh < TILE_NSHAPE; sh++)
+            if (tms[dir][sh] < tms[dir][best]) best = sh;
+        b->shape[dir] = best;
+        if (shapes_out) shapes_out[dir] = best;
+        if (ms_out)
+            for (int sh = 0; sh < TILE_NSHAPE; sh++) ms_out[dir * TILE_NSHAPE + sh] = tms[dir][sh];
     }
     return MAGIC_OK;
 }

@@ -463,9 +463,11 @@ static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
 }
 
 template <typename T, bool PERT, int NTR>
-static int launch_tile(magic_ctx *c, SweepArgsT<T> a, bool pm, int a_is_g, int *nl)
+static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl)
 {
-    const int TA = pm ? 4 : 16;
+    const void *fn = tile_fn<T, PERT, NTR>(sh);
+    if (!fn) return MAGIC_EARG;
+    const int TA = tile_ta(sh);
     const int ntiles = (a.nA + 2 + TA - 1) / TA;
     for (int ca = 0; ca < 2; ca++)
         for (int cb = 0; cb < 3; cb++) {
@@ -474,22 +476,21 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, bool pm, int a_is_g, int *
             const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
             const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
             if (gx == 0) continue;
-            if (pm) tile_sweep<T, PERT, NTR, true><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            else tile_sweep<T, PERT, NTR, false><<<dim3(gx, gy), 256, 0, c->stream>>>(a, a_is_g);
-            TRY(launch_check());
+            void *kargs[] = {&a, &a_is_g};
+            HIPCHECK(hipLaunchKernel(fn, dim3(gx, gy), dim3(256), kargs, 0, c->stream));
             ++*nl;
         }
     return MAGIC_OK;
 }
 
 template <typename T, bool PERT>
-static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, bool pm, int a_is_g, int *nl)
+static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<T, PERT, 0>(c, a, pm, a_is_g, nl);
-    case 1: return launch_tile<T, PERT, 1>(c, a, pm, a_is_g, nl);
-    case 2: return launch_tile<T, PERT, 2>(c, a, pm, a_is_g, nl);
-    case 5: return launch_tile<T, PERT, 5>(c, a, pm, a_is_g, nl);
+    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl);
+    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl);
+    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl);
+    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -511,93 +512,116 @@ static int sweep_dispatch(magic_ctx *c, const SweepArgs &a, int *nl)
     }
 }
 
-// The three sweeps of step3 (lib/step3.f:121-621) on argument block a,
-// whose state pointers and form-specific fields are already set.
-template <typename T, bool PERT>
-static int step3_sweeps(magic_ctx *c, SweepArgsT<T> a, double dt)
+// Directional fields of the argument block: dir 0 = x-sweep
+// (lib/step3.f:121-280, g = y, h = z), 1 = y-sweep (step3.f:289-450, g = z,
+// h = x), 2 = z-sweep (step3.f:459-621, g = x, h = y, gravity on).
+template <typename T>
+static void dir_args(const magic_ctx *c, SweepArgsT<T> &a, int dir, double dt)
 {
     const Dims &d = c->d;
     const long long NXY = (long long)d.NX * d.NY;
-    const bool tile = PERT || use_tile();
     const double dtdx = dt / c->dx, dtdy = dt / c->dy, dtdz = dt / c->dz;
-    {   // x-sweeps: lib/step3.f:121-280 (g = y, h = z)
-        Span sp(c, TC_SWX, 0);
+    if (dir == 0) {
         a.comp[0] = 0; a.comp[1] = 1; a.comp[2] = 2; a.comp[3] = 3; a.comp[4] = 4;
         a.n = c->mx; a.nA = c->my; a.nB = c->mz;
         a.sN = 1; a.sA = d.NX; a.sB = NXY;
         a.gravity = 0; a.sweep = 1;
         a.dtdn = dtdx; a.fg = dtdy; a.fh = dtdz;
-        if (tile) TRY((tile_dispatch<T, PERT>(c, a, true, 1, &sp.n)));   // tile axis y = g
-        else if constexpr (!PERT) TRY(sweep_dispatch(c, a, &sp.n));
-    }
-    {   // y-sweeps: lib/step3.f:289-450 (g = z, h = x)
-        Span sp(c, TC_SWY, 0);
+    } else if (dir == 1) {
         a.comp[0] = 0; a.comp[1] = 2; a.comp[2] = 3; a.comp[3] = 1; a.comp[4] = 4;
         a.n = c->my; a.nA = c->mz; a.nB = c->mx;
         a.sN = d.NX; a.sA = NXY; a.sB = 1;
         a.gravity = 0; a.sweep = 2;
         a.dtdn = dtdy; a.fg = dtdz; a.fh = dtdx;
-        if (tile) {                                           // tile axis x = h
-            SweepArgsT<T> t = a;
-            t.nA = c->mx; t.nB = c->mz; t.sA = 1; t.sB = NXY;
-            TRY((tile_dispatch<T, PERT>(c, t, false, 0, &sp.n)));
-        } else if constexpr (!PERT) {
-            TRY(sweep_dispatch(c, a, &sp.n));
-        }
-    }
-    {   // z-sweeps: lib/step3.f:459-621 (g = x, h = y), gravity on
-        Span sp(c, TC_SWZ, 0);
+    } else {
         a.comp[0] = 0; a.comp[1] = 3; a.comp[2] = 1; a.comp[3] = 2; a.comp[4] = 4;
         a.n = c->mz; a.nA = c->mx; a.nB = c->my;
         a.sN = NXY; a.sA = 1; a.sB = d.NX;
         a.gravity = 1; a.sweep = 3;
         a.dtdn = dtdz; a.fg = dtdx; a.fh = dtdy;
-        if (tile) TRY((tile_dispatch<T, PERT>(c, a, false, 1, &sp.n)));  // tile axis x = g
-        else if constexpr (!PERT) TRY(sweep_dispatch(c, a, &sp.n));
+    }
+}
+
+// One directional sweep with the tiled kernel of shape sh (or the pencil
+// kernel of the FP64 form).  The tile axis is g for the x- and z-sweeps and
+// x = h for the y-sweep, so that the y- and z-sweeps tile along rows of x.
+template <typename T, bool PERT>
+static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, bool tile, int *nl)
+{
+    dir_args(c, a, dir, dt);
+    if (tile) {
+        int a_is_g = 1;
+        if (dir == 1) {
+            a.nA = c->mx; a.nB = c->mz;
+            a.sA = 1; a.sB = (long long)c->d.NX * c->d.NY;
+            a_is_g = 0;
+        }
+        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl);
+    }
+    if constexpr (!PERT) return sweep_dispatch(c, a, nl);
+    return MAGIC_EUNSUPPORTED;
+}
+
+// The three sweeps of step3 (lib/step3.f:121-621) on argument block a,
+// whose state pointers and form-specific fields are already set.
+template <typename T, bool PERT>
+static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
+{
+    const bool tile = PERT || use_tile();
+    for (int dir = 0; dir < 3; dir++) {
+        Span sp(c, TC_SWX + dir, 0);
+        TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], tile, &sp.n)));
     }
     return MAGIC_OK;
+}
+
+// form-specific part of the argument block: state pointers, columns, scratch
+static SweepArgsT<float> args32(magic_ctx *c)
+{
+    SweepArgsT<float> a;
+    memset(&a, 0, sizeof(a));
+    a.qold = c->qold32;
+    a.qnew = c->q32;
+    a.P = c->P;
+    a.mbc = c->mbc;
+    a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
+    a.cfl_bits = c->dscal;
+    a.status = (int *)(c->dscal + 1);
+    for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+    const int NZ = c->d.NZ;
+    a.crho = c->col32 + C32_RHO * NZ;
+    a.cE = c->col32 + C32_E * NZ;
+    a.cgam = c->col32 + C32_GAM * NZ;
+    a.cvg = c->col32 + C32_VG * NZ;
+    a.cres = c->col32 + C32_RES * NZ;
+    a.sink = (float *)c->sink;
+    return a;
+}
+
+static SweepArgs args64(magic_ctx *c)
+{
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.qold = c->qold;
+    a.qnew = c->q;
+    a.gam = c->aux + AX_GAM * c->P;
+    a.vgrav = c->aux + AX_VG * c->P;
+    a.P = c->P;
+    a.mbc = c->mbc;
+    a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
+    a.cfl_bits = c->dscal;
+    a.status = (int *)(c->dscal + 1);
+    for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
+    a.sink = (double *)c->sink;
+    return a;
 }
 
 static int step3_launch(magic_ctx *c, double dt)
 {
     k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);
     TRY(launch_check());
-    const int nspec = c->have_params ? c->prm.nspecadv : 0;
-    if (c->form == MAGIC_FORM_FP32P) {
-        SweepArgsT<float> a;
-        memset(&a, 0, sizeof(a));
-        a.qold = c->qold32;
-        a.qnew = c->q32;
-        a.P = c->P;
-        a.mbc = c->mbc;
-        a.nspecadv = nspec;
-        a.cfl_bits = c->dscal;
-        a.status = (int *)(c->dscal + 1);
-        for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
-        const int NZ = c->d.NZ;
-        a.crho = c->col32 + C32_RHO * NZ;
-        a.cE = c->col32 + C32_E * NZ;
-        a.cgam = c->col32 + C32_GAM * NZ;
-        a.cvg = c->col32 + C32_VG * NZ;
-        a.cres = c->col32 + C32_RES * NZ;
-        a.sink = (float *)c->sink;
-        TRY((step3_sweeps<float, true>(c, a, dt)));
-    } else {
-        SweepArgs a;
-        memset(&a, 0, sizeof(a));
-        a.qold = c->qold;
-        a.qnew = c->q;
-        a.gam = c->aux + AX_GAM * c->P;
-        a.vgrav = c->aux + AX_VG * c->P;
-        a.P = c->P;
-        a.mbc = c->mbc;
-        a.nspecadv = nspec;
-        a.cfl_bits = c->dscal;
-        a.status = (int *)(c->dscal + 1);
-        for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
-        a.sink = (double *)c->sink;
-        TRY((step3_sweeps<double, false>(c, a, dt)));
-    }
+    if (c->form == MAGIC_FORM_FP32P) TRY((step3_sweeps<float, true>(c, args32(c), dt)));
+    else TRY((step3_sweeps<double, false>(c, args64(c), dt)));
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));
     return MAGIC_OK;
@@ -624,6 +648,74 @@ extern "C" int magic_step3(magic_ctx *c, double dt, double *cfl)
     int s = step3_finish(c, cfl);
     if (c->timing) drain_timing(c);
     return s;
+}
+
+// ------------------------------------------------------- tile shapes -----
+extern "C" int magic_set_tile_shapes(magic_ctx *c, const int shapes[3])
+{
+    if (!c || !shapes || !shapes_valid(shapes)) return MAGIC_EARG;
+    for (int k = 0; k < 3; k++) c->shape[k] = shapes[k];
+    return MAGIC_OK;
+}
+
+extern "C" int magic_get_tile_shapes(magic_ctx *c, int shapes[3])
+{
+    if (!c || !shapes) return MAGIC_EARG;
+    for (int k = 0; k < 3; k++) shapes[k] = c->shape[k];
+    return MAGIC_OK;
+}
+
+// Autotuner: every tile shape of every sweep direction is timed on this
+// grid's current state (one untimed sweep, then reps timed ones, HIP
+// events on the grid's stream) and the fastest shape of each direction is
+// kept.  The candidates read the backup copy and add into q; q is restored
+// from the backup afterwards, so the state is unchanged bit for bit (the
+// backup then holds the current state).
+extern "C" int magic_tune(magic_ctx *c, double dt, int reps, int shapes_out[3], float *ms_out)
+{
+    if (!c || !c->have_aux || !(dt > 0.0) || reps < 1) return MAGIC_EARG;
+    if (!(c->form == MAGIC_FORM_FP32P || use_tile()) || !sweep_supported(ntracer(c))) return MAGIC_EUNSUPPORTED;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return MAGIC_EHIP;
+    }
+    int st = copy_state(c, true);
+    int best[3] = {0, 0, 0};
+    for (int dir = 0; dir < 3 && st == MAGIC_OK; dir++) {
+        float tbest = 0.f;
+        for (int sh = 0; sh < TILE_NSHAPE && st == MAGIC_OK; sh++) {
+            int nl = 0;
+            float ms = 0.f;
+            for (int r = 0; r <= reps && st == MAGIC_OK; r++) {
+                if (r == 1 && hipEventRecord(e0, c->stream) != hipSuccess) st = MAGIC_EHIP;
+                if (st == MAGIC_OK)
+                    st = c->form == MAGIC_FORM_FP32P ? sweep_dir<float, true>(c, args32(c), dir, dt, sh, true, &nl)
+                                                     : sweep_dir<double, false>(c, args64(c), dir, dt, sh, true, &nl);
+            }
+            if (st == MAGIC_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                st = MAGIC_EHIP;
+            ms /= (float)reps;
+            if (ms_out) ms_out[dir * TILE_NSHAPE + sh] = ms;
+            if (st == MAGIC_OK && (sh == 0 || ms < tbest)) {
+                tbest = ms;
+                best[dir] = sh;
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    TRY(st);
+    TRY(copy_state(c, false));
+    k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);   // the candidates' Courant numbers and reports
+    TRY(launch_check());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; k++) c->shape[k] = best[k];
+    if (shapes_out)
+        for (int k = 0; k < 3; k++) shapes_out[k] = best[k];
+    return MAGIC_OK;
 }
 
 // ------------------------------------------------------------- src3 ------

@@ -58,6 +58,7 @@ struct magic_ctx {
     bool own_stream = true;   // false when a batch lends its stream
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
+    int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps
     unsigned long long *hscal = nullptr;   // pinned mirror
     magic_params prm;
     bool have_params = false, have_aux = false;
@@ -133,5 +134,40 @@ static inline int launch_check() {
 static inline int ntracer(const magic_ctx *c) { return c->meqn - 5; }
 
 static inline bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
+
+// ------------------------------------------------------- tile shapes ----
+// The tiled sweep kernel of shape sh (0 .. TILE_NSHAPE-1); one grid and the
+// persistent work-list form.
+template <typename T, bool PERT, int NTR>
+static inline const void *tile_fn(int sh)
+{
+    switch (sh) {
+    case 0: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 0>);
+    case 1: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 1>);
+    case 2: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 2>);
+    case 3: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 3>);
+    default: return nullptr;
+    }
+}
+
+template <int NTR>
+static inline const void *tile_list_fn(int sh)
+{
+    switch (sh) {
+    case 0: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 0>);
+    case 1: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 1>);
+    case 2: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 2>);
+    case 3: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 3>);
+    default: return nullptr;
+    }
+}
+static_assert(TILE_NSHAPE == 4, "tile_fn and tile_list_fn list every shape");
+
+static inline bool shapes_valid(const int *sh)
+{
+    for (int k = 0; k < 3; k++)
+        if (sh[k] < 0 || sh[k] >= TILE_NSHAPE) return false;
+    return true;
+}
 
 enum { C32_RHO = 0, C32_E, C32_GAM, C32_R, C32_VG, C32_RES, C32_NU, C32_KAP, NC32 };

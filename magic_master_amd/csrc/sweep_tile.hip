@@ -1,16 +1,17 @@
 // FP64 instantiations of the tiled sweep (built without contraction of
-// products and sums into fused multiply-adds, as the reference rounds).
+// products and sums into fused multiply-adds, as the reference rounds), for
+// the tile shape MAGIC_TILE_SHAPE: this file is compiled once per shape.
 #include "sweep_tile_body.h"
+
+#ifndef MAGIC_TILE_SHAPE
+#error "compile once per tile shape with -DMAGIC_TILE_SHAPE=0..TILE_NSHAPE-1"
+#endif
 
 namespace magic {
 
-template __global__ void tile_sweep<double, false, 0, true>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 0, false>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 1, true>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 1, false>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 2, true>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 2, false>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 5, true>(SweepArgsT<double>, int);
-template __global__ void tile_sweep<double, false, 5, false>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 0, MAGIC_TILE_SHAPE>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 1, MAGIC_TILE_SHAPE>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 2, MAGIC_TILE_SHAPE>(SweepArgsT<double>, int);
+template __global__ void tile_sweep<double, false, 5, MAGIC_TILE_SHAPE>(SweepArgsT<double>, int);
 
 }  // namespace magic

@@ -11,9 +11,9 @@
 // Same per-interface algebra as sweep.hip (rpn3euX.f, rpt3euX.f, rptt3euX.f,
 // flux3fw.f, step3.f); different data movement:
 //   * a workgroup owns TA pencils that are neighbours along the tile axis "a"
-//     (x-sweep: a = y, 4 pencils, lanes along the pencil; y- and z-sweeps:
-//     a = x, 16 pencils, lanes along x so every access is a 128-byte row) and
-//     marches along the pencils in chunks of CH interfaces (overlap 2);
+//     (x-sweep: a = y; y- and z-sweeps: a = x, so that tile-axis-major lanes
+//     read rows of x) and marches along the pencils in chunks of CH
+//     interfaces (overlap 2); TA, CH and the lane order form the tile shape;
 //   * the 3x3 increments of each interface (its cells i-1 and i into the
 //     pencil and its 8 transverse neighbours, step3.f:189-226) are summed in
 //     an LDS tile LT[b offset][a][cell] in barrier-separated sub-steps (one per
@@ -28,16 +28,31 @@
 
 namespace magic {
 
-template <bool PM>
+// Tile shapes: TA pencils x CH interfaces per chunk (TA*CH = 256 lanes),
+// lanes either tile-axis-major (consecutive lanes on neighbouring pencils)
+// or pencil-major (consecutive lanes along one pencil).  The host picks one
+// per sweep direction (a fixed table or the autotuner); every shape computes
+// the same increments, only the grouping of their sums differs.
+template <int SH>
 struct TileShape;
 template <>
-struct TileShape<true> {   // pencil-major lanes (x-sweep)
-    static constexpr int TA = 4, CH = 64;
-};
+struct TileShape<0> { static constexpr int TA = 16; static constexpr bool PM = false; };
 template <>
-struct TileShape<false> {  // tile-axis-major lanes (y- and z-sweeps)
-    static constexpr int TA = 16, CH = 16;
-};
+struct TileShape<1> { static constexpr int TA = 8; static constexpr bool PM = false; };
+template <>
+struct TileShape<2> { static constexpr int TA = 8; static constexpr bool PM = true; };
+template <>
+struct TileShape<3> { static constexpr int TA = 4; static constexpr bool PM = true; };
+static_assert(TileShape<TILE_NSHAPE - 1>::TA > 0, "one TileShape per shape index");
+
+// Pointer into global memory.  The work-list kernel reads its state pointers
+// from an argument block in memory, where the compiler cannot tell their
+// address space and would use flat loads and stores; flat accesses count on
+// the LDS counter too, so every LDS wait would also wait for them.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) { return (gptr<T>)p; }
 
 template <typename AT>
 __device__ __forceinline__ void report_t(const AT &A, int code, int i, int pg, int ph)
@@ -53,15 +68,16 @@ __device__ __forceinline__ void report_t(const AT &A, int code, int i, int pg, i
 
 // One tile (bx along the tile axis, by along the coloured axis) of one
 // colour launch; called by every lane of a 256-lane workgroup.
-template <typename T, bool PERT, int NTR, bool PM>
+template <typename T, bool PERT, int NTR, int SH>
 __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is_g, int bx, int by)
 {
     constexpr int NC = 5 + NTR;
-    constexpr int TA = TileShape<PM>::TA, CH = TileShape<PM>::CH, NT = TA * CH;
+    constexpr int NT = 256, TA = TileShape<SH>::TA, CH = NT / TA;
+    constexpr bool PM = TileShape<SH>::PM;
     constexpr int NCELL = CH - 1;               // cells S .. S+CH-2 of a chunk
     constexpr int TROW = TA + 2;                // a offsets A0-1 .. A0+TA
     constexpr T beta = T(0.666666666666666);
-    static_assert(NT == 256, "tile is 256 lanes");
+    static_assert(TA * CH == NT && TA == tile_ta(SH), "tile is 256 lanes");
 
     __shared__ T s_del[5][NT];
     __shared__ T s_s2[NT];
@@ -85,6 +101,11 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     const T dtdn = T(A.dtdn);
     const T cg = T(A.fg), ch = T(A.fh);
     const T d = dtdn;
+    const gptr<const T> qold = to_global(A.qold);
+    const gptr<T> qnew = to_global(A.qnew);
+    const gptr<const double> gam = to_global(A.gam), vgrav = to_global(A.vgrav);
+    const gptr<const float> crho = to_global(A.crho), cE = to_global(A.cE), cgam = to_global(A.cgam),
+                            cvg = to_global(A.cvg), cres = to_global(A.cres);
     const T k6g = (T(1.0) / T(6.0)) * d * ch;
     const T k6h = (T(1.0) / T(6.0)) * d * cg;
 
@@ -109,13 +130,13 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             const long long cb = ca + A.sN;
 #pragma unroll
             for (int m = 0; m < NC; m++) {
-                pqa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
-                pqb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
+                pqa[m] = qold[(long long)A.comp[m] * A.P + ca];
+                pqb[m] = qold[(long long)A.comp[m] * A.P + cb];
             }
             if constexpr (!PERT) {
-                pga = A.gam[ca];
-                pgb = A.gam[cb];
-                if (A.gravity) pvg = A.vgrav[cb];
+                pga = gam[ca];
+                pgb = gam[cb];
+                if (A.gravity) pvg = vgrav[cb];
             }
         }
     };
@@ -152,13 +173,13 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 const long long cb = ca + A.sN;
 #pragma unroll
                 for (int m = 0; m < NC; m++) {
-                    qa[m] = A.qold[(long long)A.comp[m] * A.P + ca];
-                    qb[m] = A.qold[(long long)A.comp[m] * A.P + cb];
+                    qa[m] = qold[(long long)A.comp[m] * A.P + ca];
+                    qb[m] = qold[(long long)A.comp[m] * A.P + cb];
                 }
                 if constexpr (!PERT) {
-                    ga = A.gam[ca];
-                    gb = A.gam[cb];
-                    if (A.gravity) vgc = A.vgrav[cb];
+                    ga = gam[ca];
+                    gb = gam[cb];
+                    if (A.gravity) vgc = vgrav[cb];
                 }
             }
             // perturbation form: q holds q - background; the background is a
@@ -170,15 +191,15 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 const int ka = A.sweep == 3 ? I - 1 : b, kb = A.sweep == 3 ? I : b;
                 const int kia = ka + A.mbc - 1;
                 kib = kb + A.mbc - 1;
-                ga = A.cgam[kia];
-                gb = A.cgam[kib];
+                ga = cgam[kia];
+                gb = cgam[kib];
                 rpa = qa[0];
                 rpb = qb[0];
                 const T epa = qa[4], epb = qb[4];
-                qa[0] = A.crho[kia] + rpa;
-                qb[0] = A.crho[kib] + rpb;
-                qa[4] = A.cE[kia] + epa;
-                qb[4] = A.cE[kib] + epb;
+                qa[0] = crho[kia] + rpa;
+                qb[0] = crho[kib] + rpb;
+                qa[4] = cE[kia] + epa;
+                qb[4] = cE[kib] + epb;
                 // p - p_bar = (gamma-1) (E' - kinetic energy), p_bar = (gamma-1) E_bar
                 ppa = (ga - T(1.0)) * (epa - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
                 ppb = (gb - T(1.0)) * (epb - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
@@ -212,8 +233,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 if constexpr (PERT) {
                     // the background's own jump p_bar_b - p_bar_a - .5 (rho_bar_a +
                     // rho_bar_b) aux17 is the column residual (zero to FP64 rounding)
-                    const T vg = A.cvg[kib];
-                    d2 = d2 - T(0.5) * (rpb + rpa) * vg + A.cres[kib];
+                    const T vg = cvg[kib];
+                    d2 = d2 - T(0.5) * (rpb + rpa) * vg + cres[kib];
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 } else {
                     const T vg = vgc;
@@ -495,13 +516,13 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 pok[k] = e < NPOS && cell >= 1 && cell <= n && aa <= A.nA + 2;
                 poff[k] = (long long)(cell + A.mbc - 1) * A.sN + (long long)(aa + A.mbc - 1) * A.sA;
             }
-            T *const sink = A.sink + t;
+            const gptr<T> sink = to_global(A.sink) + t;
             T val[NC][3][PPT];
 #pragma unroll
             for (int m = 0; m < NC; m++)
 #pragma unroll
                 for (int db = 0; db < 3; db++) {
-                    T *base = A.qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
+                    const gptr<T> base = qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
 #pragma unroll
                     for (int k = 0; k < PPT; k++) val[m][db][k] = *(pok[k] ? base + poff[k] : sink);
                 }
@@ -520,7 +541,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             for (int m = 0; m < NC; m++)
 #pragma unroll
                 for (int db = 0; db < 3; db++) {
-                    T *base = A.qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
+                    const gptr<T> base = qnew + (long long)A.comp[m] * A.P + (long long)(b + db - 1 + A.mbc - 1) * A.sB;
 #pragma unroll
                     for (int k = 0; k < PPT; k++) *(pok[k] ? base + poff[k] : sink) = val[m][db][k];
                 }
@@ -538,18 +559,32 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 }
 
 
-template <typename T, bool PERT, int NTR, bool PM>
-__global__ __launch_bounds__(256) void tile_sweep(SweepArgsT<T> A, int a_is_g)
+// Waves per SIMD the register allocation aims at: two for the FP32 form
+// with tile-axis-major lanes and no tracer, and for the work-list form with
+// one tracer at TA = 8 (it misses 256 registers by one unless asked), one
+// otherwise.  The other FP32 tile-axis-major kernels with one tracer reach
+// two waves without being asked, and asking changes their schedule for the
+// worse (5 % on the batched ensemble, a spill at TA = 8 in the one-grid form).
+template <typename T, int NTR, int SH, bool LIST>
+constexpr int tile_waves()
 {
-    tile_sweep_body<T, PERT, NTR, PM>(A, a_is_g, (int)blockIdx.x, (int)blockIdx.y);
+    return sizeof(T) == 4 && !TileShape<SH>::PM && (NTR == 0 || (LIST && NTR == 1 && TileShape<SH>::TA == 8)) ? 2 : 1;
+}
+
+template <typename T, bool PERT, int NTR, int SH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH, false>())))
+void tile_sweep(SweepArgsT<T> A, int a_is_g)
+{
+    tile_sweep_body<T, PERT, NTR, SH>(A, a_is_g, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // Persistent form for a batch of grids: the workgroups take tiles from a
 // work list (item = {argument block, a_is_g, bx, by}) through an atomic
 // counter until it runs out; tiles of grids whose control flag (*run) is 0
 // in this attempt are skipped.
-template <typename T, bool PERT, int NTR, bool PM>
-__global__ __launch_bounds__(256) void tile_sweep_list(const SweepArgsT<T> *args, const int4 *items, int nitems,
+template <typename T, bool PERT, int NTR, int SH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH, true>())))
+void tile_sweep_list(const SweepArgsT<T> *__restrict__ args, const int4 *items, int nitems,
                                                         const int *const *run, int *next)
 {
     __shared__ int s_item;
@@ -564,7 +599,7 @@ __global__ __launch_bounds__(256) void tile_sweep_list(const SweepArgsT<T> *args
         // argument block is read with scalar loads
         const int ai = __builtin_amdgcn_readfirstlane(w.x);
         if (*run[ai] == 0) continue;
-        tile_sweep_body<T, PERT, NTR, PM>(args[ai], __builtin_amdgcn_readfirstlane(w.y),
+        tile_sweep_body<T, PERT, NTR, SH>(args[ai], __builtin_amdgcn_readfirstlane(w.y),
                                           __builtin_amdgcn_readfirstlane(w.z), __builtin_amdgcn_readfirstlane(w.w));
     }
 }

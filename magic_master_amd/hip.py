@@ -39,7 +39,13 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_error_location", "magic_timing_enable", "magic_timing_read",
            "magic_timing_reset", "magic_set_aux_column", "magic_qinit_rest", "magic_batch_create", "magic_batch_destroy",
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
-           "magic_batch_add", "magic_batch_remove"]
+           "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
+           "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune"]
+
+TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
+# (pencils per tile, lane order) of each tile shape
+TILE_SHAPES = {0: (16, "tile-axis-major"), 1: (8, "tile-axis-major"), 2: (8, "pencil-major"),
+               3: (4, "pencil-major")}
 
 
 class MagicError(RuntimeError):
@@ -111,6 +117,13 @@ def load_library(path: str = LIB_PATH):
     L.magic_batch_state.argtypes = [vp, C.c_int, _dp, _ip]
     L.magic_batch_add.argtypes = [vp, _ip, _dp, _ip]
     L.magic_batch_remove.argtypes = [vp, C.c_int]
+    _fp = C.POINTER(C.c_float)
+    L.magic_set_tile_shapes.argtypes = [vp, _ip]
+    L.magic_get_tile_shapes.argtypes = [vp, _ip]
+    L.magic_tune.argtypes = [vp, C.c_double, C.c_int, _ip, _fp]
+    L.magic_batch_set_tile_shapes.argtypes = [vp, _ip]
+    L.magic_batch_get_tile_shapes.argtypes = [vp, _ip]
+    L.magic_batch_tune.argtypes = [vp, C.c_int, _ip, _fp]
     _LIB = L
     return L
 
@@ -258,6 +271,23 @@ class MagicHIP:
         names = ["sweep_x", "sweep_y", "sweep_z", "bc3", "src3", "other"]
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
 
+    # -- tile shapes --------------------------------------------------------
+    def set_tile_shapes(self, shapes: Sequence[int]):
+        self._check(self.L.magic_set_tile_shapes(self.h, (C.c_int * 3)(*shapes)), "magic_set_tile_shapes")
+
+    def tile_shapes(self):
+        sh = (C.c_int * 3)()
+        self._check(self.L.magic_get_tile_shapes(self.h, sh), "magic_get_tile_shapes")
+        return list(sh)
+
+    def tune(self, dt: float, reps: int = 3):
+        """Autotuner: times every tile shape of every sweep direction on the
+        current state and keeps the fastest; returns (shapes, ms[3][NSHAPE])."""
+        sh = (C.c_int * 3)()
+        ms = (C.c_float * (3 * TILE_NSHAPE))()
+        self._check(self.L.magic_tune(self.h, dt, reps, sh, ms), "magic_tune")
+        return list(sh), [list(ms[d * TILE_NSHAPE:(d + 1) * TILE_NSHAPE]) for d in range(3)]
+
 
 class MagicBatch:
     """Many grids of the FP32 perturbation form advanced together with the
@@ -353,3 +383,20 @@ class MagicBatch:
         self._check(self.L.magic_batch_state(self.h, s, r, i), "magic_batch_state")
         return {"t": r[0], "dt": r[1], "cfl": r[2], "cflmax": r[3], "tend": r[4], "nsteps": i[0],
                 "attempts": i[1], "rejects": i[2], "active": i[3], "status": i[4], "ndtr": i[5]}
+
+    def set_tile_shapes(self, shapes):
+        self._check(self.L.magic_batch_set_tile_shapes(self.h, (C.c_int * 3)(*shapes)),
+                    "magic_batch_set_tile_shapes")
+
+    def tile_shapes(self):
+        sh = (C.c_int * 3)()
+        self._check(self.L.magic_batch_get_tile_shapes(self.h, sh), "magic_batch_get_tile_shapes")
+        return list(sh)
+
+    def tune(self, reps: int = 3):
+        """Autotuner of the batch (all scenarios' tiles, current states and
+        next dt); returns (shapes, ms[3][NSHAPE])."""
+        sh = (C.c_int * 3)()
+        ms = (C.c_float * (3 * TILE_NSHAPE))()
+        self._check(self.L.magic_batch_tune(self.h, reps, sh, ms), "magic_batch_tune")
+        return list(sh), [list(ms[d * TILE_NSHAPE:(d + 1) * TILE_NSHAPE]) for d in range(3)]

@@ -99,9 +99,11 @@ class _Writer(threading.Thread):
 
 class Server:
     def __init__(self, budget_bytes: float = 250e9, check_every: int = 4, snapshots: str = "hdf5",
-                 outdir: Optional[str] = None, keep_final: bool = False):
+                 outdir: Optional[str] = None, keep_final: bool = False, tile_shapes=None):
         """snapshots: "hdf5" (frames written under outdir), "download" (copied
-        to the host and dropped) or "none"."""
+        to the host and dropped) or "none".  tile_shapes: the sweeps' tile
+        shapes (x, y, z) for the whole run (None: the library's default),
+        fixed so that a request's result does not depend on when it ran."""
         if snapshots not in ("hdf5", "download", "none"):
             raise ValueError(snapshots)
         if snapshots == "hdf5" and not outdir:
@@ -111,6 +113,7 @@ class Server:
         self.snapshots = snapshots
         self.outdir = outdir
         self.keep_final = keep_final
+        self.tile_shapes = tile_shapes
 
     def run(self, requests: List[Request]) -> dict:
         from .hip import MagicBatch
@@ -119,6 +122,8 @@ class Server:
             return {"outcomes": []}
         g0 = reqs[0].grid
         batch = MagicBatch([], [], [], meqn=g0.meqn, mbc=g0.mbc, maux=g0.maux)
+        if self.tile_shapes is not None:
+            batch.set_tile_shapes(self.tile_shapes)
         writer = _Writer(self.outdir) if self.snapshots == "hdf5" else None
         if writer:
             writer.start()

@@ -55,7 +55,7 @@ def measured_traffic(kernel):
     return None, None
 
 
-def run_gpu(workload, steps, warmup, form="fp64", tune=True):
+def run_gpu(workload, steps, warmup, form="fp64", tune=True, shapes=None):
     from magic_master_amd import problem as P
     from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
 
@@ -70,7 +70,9 @@ def run_gpu(workload, steps, warmup, form="fp64", tune=True):
     t0 = 0.0
     t, dtv, log = dev.run_steps(c, warmup, t0=t0)
     tuned = None
-    if tune:   # tile shapes from the autotuner, on the warmed-up state
+    if shapes is not None:
+        dev.set_tile_shapes(shapes)
+    elif tune:   # tile shapes from the autotuner, on the warmed-up state
         shapes, ms = dev.tune(dtv[0], reps=3)
         tuned = {"shapes": shapes, "ms_per_sweep": [[round(v, 4) for v in row] for row in ms]}
     shapes = dev.tile_shapes()
@@ -203,6 +205,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="default tile shapes instead of the autotuner's choice")
+    ap.add_argument("--shapes", default=None,
+                    help="tile shapes of the x, y, z sweeps of the single-grid run, e.g. 2,1,1 "
+                         "(instead of the autotuner's choice)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the FP32 batched ensemble measurement")
@@ -221,7 +226,8 @@ def main():
         print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "
               "supported (the domain is not decomposed across GPUs)", file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(a.workload, a.steps, a.warmup, a.form, tune=not a.no_tune)
+    shapes = [int(v) for v in a.shapes.split(",")] if a.shapes else None
+    r = run_gpu(a.workload, a.steps, a.warmup, a.form, tune=not a.no_tune, shapes=shapes)
     g = r["g"]
     K = a.steps
     cells = g.ncell

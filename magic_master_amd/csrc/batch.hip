@@ -250,7 +250,7 @@ static int launch_src(magic_batch *b, int s)
     const Dims &d = c->d;
     const long long P = c->P;
     const int NZ = d.NZ;
-    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gint = interior_grid(d);
     const dim3 gface(blocks(d.NX, 64), d.NY);
     const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
     const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
@@ -259,7 +259,7 @@ static int launch_src(magic_batch *b, int s)
     memcpy(mthbc, &b->mthbc[6 * s], sizeof(mthbc));
     mthbc[4] = -1;
     mthbc[5] = -1;
-    k32_src_convert<<<dim3(blocks(d.NX, 64), d.NY, d.NZ), 64, 0, b->stream>>>(d, c->q32, col,
+    k32_src_convert<<<padded_cells(d), cell_block(), 0, b->stream>>>(d, c->q32, col,
                                                                              gate(b, s, GATE_ACCEPTED));
     TRY(launch_check());
     for (int nn = 1; nn <= h.ndtr_cap; nn++) {
@@ -267,17 +267,17 @@ static int launch_src(magic_batch *b, int s)
         TRY(launch_bc(b, s, mthbc, gd));
         k_zghost_uvw<<<gface, 64, 0, b->stream>>>(d, c->q32, gd);
         TRY(launch_copy(b, c->u32, c->q32 + P, 3 * P, gd));
-        k_diff_cross<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->q32, c->u32, nu, 0.f, 0.f, 0.f,
+        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->q32, c->u32, nu, 0.f, 0.f, 0.f,
                                                                        gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
             const int f0 = FAC_DIFF + 3 * (cc - 1);
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->qt32, qc, uc, nu, 0.f, 0,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->qt32, qc, uc, nu, 0.f, 0,
                                                                      gate(b, s, GATE_DIFF, nn, 0, f0 + 0));
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, uc, c->qt32, uc, nu, 0.f, 1,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, uc, c->qt32, uc, nu, 0.f, 1,
                                                                      gate(b, s, GATE_DIFF, nn, 0, f0 + 1));
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, qc, uc, uc, nu, 0.f, 2,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, qc, uc, uc, nu, 0.f, 2,
                                                                      gate(b, s, GATE_DIFF, nn, 0, f0 + 2));
             TRY(launch_check());
         }
@@ -285,19 +285,19 @@ static int launch_src(magic_batch *b, int s)
         TRY(launch_copy(b, c->ut32, q5, P, gd));
         for (int kk = 1; kk <= h.ndtrc_cap; kk++) {
             const StepGate gk = gate(b, s, GATE_COND, nn, kk);
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
                                                                      gate(b, s, GATE_COND, nn, kk, FAC_COND + 0));
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
                                                                      gate(b, s, GATE_COND, nn, kk, FAC_COND + 1));
             k_conduct_zghost<<<gface, 64, 0, b->stream>>>(d, c->ut32, gk);
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->stream>>>(d, c->ut32, S, gk);
             }
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(
                 d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
-            k_copy_interior<<<gint, 64, 0, b->stream>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
-            k_pass<float, true, StepGate><<<gint, 64, 0, b->stream>>>(
+            k_copy_interior<<<gint, cell_block(), 0, b->stream>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(
                 d, q5, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_LAST, nn, kk, FAC_COND + 2));
             TRY(launch_check());
         }
@@ -312,7 +312,7 @@ static int launch_src(magic_batch *b, int s)
     f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
     f.tcenter = p.tcenter; f.twidth = p.twidth;
     f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
-    k32_src_finish<<<gint, 64, 0, b->stream>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
+    k32_src_finish<<<gint, cell_block(), 0, b->stream>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
     return launch_check();
 }
 
@@ -734,15 +734,36 @@ extern "C" int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3])
 }
 
 // Every candidate reads the backup planes and adds into q (one list build
-// per shape, all directions); q is restored from the backup afterwards.
-// The candidates' Courant numbers and reports land in the control blocks'
-// reduction fields, which the next attempt's prepare clears.
+// per shape and round, all directions); q is restored from the backup
+// afterwards.  The candidates' Courant numbers and reports land in the
+// control blocks' reduction fields, which the next attempt's prepare clears.
+static int tune_rounds(magic_batch *b, int reps, TuneClock &clk)
+{
+    for (int r = 0; r <= reps; r++) {   // round 0 is not timed
+        for (int sh = 0; sh < TILE_NSHAPE; sh++) {
+            for (int k = 0; k < 3; k++) b->shape[k] = sh;
+            TRY(list_grids(b));
+            TRY(build_lists(b, true));
+            for (int dir = 0; dir < 3; dir++) {
+                TRY(clk.mark(dir, sh, 0, b->stream));
+                HIPCHECK(hipMemsetAsync(b->counters_d, 0, 18 * sizeof(int), b->stream));
+                TRY(launch_dir(b, dir));
+                TRY(clk.mark(dir, sh, 1, b->stream));
+            }
+        }
+        if (r > 0) TRY(clk.add_round());
+    }
+    return MAGIC_OK;
+}
+
 extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], float *ms_out)
 {
     if (!b || reps < 1) return MAGIC_EARG;
     bool any = false;
     for (int s = 0; s < b->n; s++) any = any || b->sc[s] != nullptr;
     if (!any) return MAGIC_EARG;
+    TuneClock clk;
+    if (!clk.ok) return MAGIC_EHIP;
     const int saved[3] = {b->shape[0], b->shape[1], b->shape[2]};
     b->graph_ok = false;   // the lists and argument blocks are rebuilt before the next attempt
     for (int s = 0; s < b->n; s++) {
@@ -751,35 +772,8 @@ extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], flo
             HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, (size_t)c->P * c->meqn * sizeof(float),
                                     hipMemcpyDeviceToDevice, b->stream));
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHECK(hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
-        return MAGIC_EHIP;
-    }
-    float tms[3][TILE_NSHAPE] = {};
-    int st = MAGIC_OK;
-    for (int sh = 0; sh < TILE_NSHAPE && st == MAGIC_OK; sh++) {
-        for (int k = 0; k < 3; k++) b->shape[k] = sh;
-        st = list_grids(b);
-        if (st == MAGIC_OK) st = build_lists(b, true);
-        for (int dir = 0; dir < 3 && st == MAGIC_OK; dir++) {
-            for (int r = 0; r <= reps && st == MAGIC_OK; r++) {
-                if (r == 1 && hipEventRecord(e0, b->stream) != hipSuccess) st = MAGIC_EHIP;
-                if (st == MAGIC_OK && hipMemsetAsync(b->counters_d, 0, 18 * sizeof(int), b->stream) != hipSuccess)
-                    st = MAGIC_EHIP;
-                if (st == MAGIC_OK) st = launch_dir(b, dir);
-            }
-            float ms = 0.f;
-            if (st == MAGIC_OK && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                st = MAGIC_EHIP;
-            tms[dir][sh] = ms / (float)reps;
-        }
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (int s = 0; s < b->n && st == MAGIC_OK; s++) {
+    int st = tune_rounds(b, reps, clk);
+    for (int s = 0; s < b->n; s++) {   // also after a failed round
         magic_ctx *c = b->sc[s];
         if (c && hipMemcpyAsync(c->q32, c->qold32, (size_t)c->P * c->meqn * sizeof(float), hipMemcpyDeviceToDevice,
                                 b->stream) != hipSuccess)
@@ -790,14 +784,11 @@ extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], flo
         for (int k = 0; k < 3; k++) b->shape[k] = saved[k];
         return st;
     }
-    for (int dir = 0; dir < 3; dir++) {
-        int best = 0;
-        for (int sh = 1; sh < TILE_NSHAPE; sh++)
-            if (tms[dir][sh] < tms[dir][best]) best = sh;
-        b->shape[dir] = best;
-        if (shapes_out) shapes_out[dir] = best;
-        if (ms_out)
-            for (int sh = 0; sh < TILE_NSHAPE; sh++) ms_out[dir * TILE_NSHAPE + sh] = tms[dir][sh];
+    int best[3];
+    clk.result(reps, best, ms_out);
+    for (int k = 0; k < 3; k++) {
+        b->shape[k] = best[k];
+        if (shapes_out) shapes_out[k] = best[k];
     }
     return MAGIC_OK;
 }

@@ -266,11 +266,6 @@ extern "C" int magic_set_aux_column(magic_ctx *c, const double *col)
     return set_aux64(c, get);
 }
 
-static dim3 padded_grid(const magic_ctx *c)
-{
-    return dim3(blocks(c->d.NX, 64), c->d.NY, c->d.NZ);
-}
-
 extern "C" int magic_set_q(magic_ctx *c, const double *q)
 {
     if (!c || !q) return MAGIC_EARG;
@@ -279,7 +274,7 @@ extern "C" int magic_set_q(magic_ctx *c, const double *q)
         if (!c->have_aux) return MAGIC_EARG;   // the perturbation needs the background
         const int NZ = c->d.NZ;
         HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        k32_aos_to_pert<<<padded_grid(c), 64, 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn, c->bg64,
+        k32_aos_to_pert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn, c->bg64,
                                                                c->bg64 + NZ);
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
@@ -318,7 +313,7 @@ extern "C" int magic_get_q(magic_ctx *c, double *q)
     const size_t n = (size_t)c->P * c->meqn;
     if (c->form == MAGIC_FORM_FP32P) {
         const int NZ = c->d.NZ;
-        k32_pert_to_aos<<<padded_grid(c), 64, 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn, c->bg64,
+        k32_pert_to_aos<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn, c->bg64,
                                                                c->bg64 + NZ);
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -671,47 +666,37 @@ extern "C" int magic_get_tile_shapes(magic_ctx *c, int shapes[3])
 // kept.  The candidates read the backup copy and add into q; q is restored
 // from the backup afterwards, so the state is unchanged bit for bit (the
 // backup then holds the current state).
+static int tune_rounds(magic_ctx *c, double dt, int reps, TuneClock &clk)
+{
+    for (int r = 0; r <= reps; r++) {   // round 0 is not timed
+        for (int dir = 0; dir < 3; dir++)
+            for (int sh = 0; sh < TILE_NSHAPE; sh++) {
+                int nl = 0;
+                TRY(clk.mark(dir, sh, 0, c->stream));
+                TRY((c->form == MAGIC_FORM_FP32P ? sweep_dir<float, true>(c, args32(c), dir, dt, sh, true, &nl)
+                                                 : sweep_dir<double, false>(c, args64(c), dir, dt, sh, true, &nl)));
+                TRY(clk.mark(dir, sh, 1, c->stream));
+            }
+        if (r > 0) TRY(clk.add_round());
+    }
+    return MAGIC_OK;
+}
+
 extern "C" int magic_tune(magic_ctx *c, double dt, int reps, int shapes_out[3], float *ms_out)
 {
     if (!c || !c->have_aux || !(dt > 0.0) || reps < 1) return MAGIC_EARG;
     if (!(c->form == MAGIC_FORM_FP32P || use_tile()) || !sweep_supported(ntracer(c))) return MAGIC_EUNSUPPORTED;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHECK(hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
-        return MAGIC_EHIP;
-    }
-    int st = copy_state(c, true);
-    int best[3] = {0, 0, 0};
-    for (int dir = 0; dir < 3 && st == MAGIC_OK; dir++) {
-        float tbest = 0.f;
-        for (int sh = 0; sh < TILE_NSHAPE && st == MAGIC_OK; sh++) {
-            int nl = 0;
-            float ms = 0.f;
-            for (int r = 0; r <= reps && st == MAGIC_OK; r++) {
-                if (r == 1 && hipEventRecord(e0, c->stream) != hipSuccess) st = MAGIC_EHIP;
-                if (st == MAGIC_OK)
-                    st = c->form == MAGIC_FORM_FP32P ? sweep_dir<float, true>(c, args32(c), dir, dt, sh, true, &nl)
-                                                     : sweep_dir<double, false>(c, args64(c), dir, dt, sh, true, &nl);
-            }
-            if (st == MAGIC_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                   hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                st = MAGIC_EHIP;
-            ms /= (float)reps;
-            if (ms_out) ms_out[dir * TILE_NSHAPE + sh] = ms;
-            if (st == MAGIC_OK && (sh == 0 || ms < tbest)) {
-                tbest = ms;
-                best[dir] = sh;
-            }
-        }
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    TuneClock clk;
+    if (!clk.ok) return MAGIC_EHIP;
+    TRY(copy_state(c, true));
+    const int st = tune_rounds(c, dt, reps, clk);
+    TRY(copy_state(c, false));   // also after a failed round
     TRY(st);
-    TRY(copy_state(c, false));
     k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);   // the candidates' Courant numbers and reports
     TRY(launch_check());
     HIPCHECK(hipStreamSynchronize(c->stream));
+    int best[3];
+    clk.result(reps, best, ms_out);
     for (int k = 0; k < 3; k++) c->shape[k] = best[k];
     if (shapes_out)
         for (int k = 0; k < 3; k++) shapes_out[k] = best[k];
@@ -727,14 +712,14 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const Dims &d = c->d;
     const long long P = c->P;
     const int NZ = d.NZ;
-    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gint = interior_grid(d);
     const dim3 gface(blocks(d.NX, 64), d.NY);
     const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
     const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
                        c->col32 + C32_R * NZ};
     {
         Span sp(c, TC_SRC, 1);
-        k32_src_convert<<<padded_grid(c), 64, 0, c->stream>>>(d, c->q32, col, NoGate{});
+        k32_src_convert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(d, c->q32, col, NoGate{});
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -746,7 +731,7 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->u32, c->q32 + P, 3 * P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->q32, c->u32, nu,
+        k_diff_cross<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->q32, c->u32, nu,
                                                              (float)(dtr32 / (c->dx * c->dy)),
                                                              (float)(dtr32 / (c->dx * c->dz)),
                                                              (float)(dtr32 / (c->dy * c->dz)), NoGate{});
@@ -762,9 +747,9 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
                 dtdy2 = dtdy2 / fourthirds;
             }
             float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
-            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0, NoGate{});
-            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1, NoGate{});
-            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2, NoGate{});
+            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0, NoGate{});
+            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1, NoGate{});
+            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2, NoGate{});
             TRY(launch_check());
         }
         const float cx = (float)((dtrc) / (c->dx * c->dx)), cy = (float)((dtrc) / (c->dy * c->dy)),
@@ -772,25 +757,25 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         float *q5 = c->q32 + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut32, q5, P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cx, 0, NoGate{});
-            k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, cy, 1, NoGate{});
+            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cx, 0, NoGate{});
+            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, cy, 1, NoGate{});
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut32, NoGate{});
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut32, S, NoGate{});
             }
             if (k < ndtrc) {
-                k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cz, 2, NoGate{});
-                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut32, c->qt32, 1, NoGate{});
+                k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, cz, 2, NoGate{});
+                k_copy_interior<<<gint, cell_block(), 0, c->stream>>>(d, c->ut32, c->qt32, 1, NoGate{});
             } else {
-                k_pass<float, true, NoGate><<<gint, 64, 0, c->stream>>>(d, q5, c->ut32, c->ut32, kap, cz, 2, NoGate{});
+                k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, q5, c->ut32, c->ut32, kap, cz, 2, NoGate{});
             }
             TRY(launch_check());
         }
     }
     {
         Span sp(c, TC_SRC, 1);
-        k32_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q32, c->meqn, col, f, NoGate{});
+        k32_src_finish<<<gint, cell_block(), 0, c->stream>>>(d, c->q32, c->meqn, col, f, NoGate{});
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -824,7 +809,7 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     mthbc[4] = -1;
     mthbc[5] = -1;
     const dim3 gcell(blocks(P, 256));
-    const dim3 gint(blocks(c->mx, 64), c->my, c->mz);
+    const dim3 gint = interior_grid(d);
     const dim3 gface(blocks(d.NX, 64), d.NY);
     Forcing f;
     f.forcemth = fm;
@@ -856,7 +841,7 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->u, c->q + P, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
+        k_diff_cross<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
                                                  dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
@@ -870,9 +855,9 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdy2 = dtdy2 / fourthirds;
             }
             double *uc = c->u + (cc - 1) * P, *qc = c->q + cc * P;
-            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0, NoGate{});
-            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1, NoGate{});
-            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2, NoGate{});
+            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0, NoGate{});
+            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1, NoGate{});
+            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2, NoGate{});
             TRY(launch_check());
         }
         // conduct: conduct.f:245-332.  ut = q(5) on every cell once; the
@@ -883,25 +868,25 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         double *q5 = c->q + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut, q5, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0, NoGate{});
-            k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1, NoGate{});
+            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0, NoGate{});
+            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1, NoGate{});
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut, NoGate{});
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S, NoGate{});
             }
             if (k < ndtrc) {
-                k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2, NoGate{});
-                k_copy_interior<<<gint, 64, 0, c->stream>>>(d, c->ut, c->qt, 1, NoGate{});
+                k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2, NoGate{});
+                k_copy_interior<<<gint, cell_block(), 0, c->stream>>>(d, c->ut, c->qt, 1, NoGate{});
             } else {
-                k_pass<double, false, NoGate><<<gint, 64, 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2, NoGate{});
+                k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2, NoGate{});
             }
             TRY(launch_check());
         }
     }
     {
         Span sp(c, TC_SRC, 1);
-        k_src_finish<<<gint, 64, 0, c->stream>>>(d, c->q, c->meqn, ax, f);
+        k_src_finish<<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->meqn, ax, f);
         TRY(launch_check());
     }
     return MAGIC_OK;

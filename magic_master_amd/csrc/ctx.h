@@ -133,6 +133,12 @@ static inline int launch_check() {
 
 static inline int ntracer(const magic_ctx *c) { return c->meqn - 5; }
 
+// Launch shape of the per-cell kernels (INTERIOR_IJK, PADDED_IJK): 64 cells
+// along x by 4 rows along y per workgroup, one layer per grid z index.
+static inline dim3 cell_block() { return dim3(64, 4, 1); }
+static inline dim3 interior_grid(const Dims &d) { return dim3(blocks(d.mx, 64), blocks(d.my, 4), d.mz); }
+static inline dim3 padded_cells(const Dims &d) { return dim3(blocks(d.NX, 64), blocks(d.NY, 4), d.NZ); }
+
 static inline bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
 
 // ------------------------------------------------------- tile shapes ----
@@ -162,6 +168,56 @@ static inline const void *tile_list_fn(int sh)
     }
 }
 static_assert(TILE_NSHAPE == 4, "tile_fn and tile_list_fn list every shape");
+
+// Event pairs of the autotuners, one per (direction, shape).  The shapes
+// are timed round-robin (every round times each candidate once), so that a
+// drift of the clocks over the tuning run favours none of them.
+struct TuneClock {
+    hipEvent_t ev[3][TILE_NSHAPE][2] = {};
+    float ms[3][TILE_NSHAPE] = {};
+    bool ok = true;
+    TuneClock()
+    {
+        for (auto &d : ev)
+            for (auto &s : d)
+                for (auto &e : s) ok = ok && hipEventCreate(&e) == hipSuccess;
+    }
+    ~TuneClock()
+    {
+        for (auto &d : ev)
+            for (auto &s : d)
+                for (auto &e : s)
+                    if (e) hipEventDestroy(e);
+    }
+    int mark(int dir, int sh, int end, hipStream_t st)
+    {
+        return hipEventRecord(ev[dir][sh][end], st) == hipSuccess ? MAGIC_OK : MAGIC_EHIP;
+    }
+    // adds every pair of the round just recorded
+    int add_round()
+    {
+        for (int d = 0; d < 3; d++)
+            for (int s = 0; s < TILE_NSHAPE; s++) {
+                float t = 0.f;
+                if (hipEventSynchronize(ev[d][s][1]) != hipSuccess ||
+                    hipEventElapsedTime(&t, ev[d][s][0], ev[d][s][1]) != hipSuccess)
+                    return MAGIC_EHIP;
+                ms[d][s] += t;
+            }
+        return MAGIC_OK;
+    }
+    // fastest shape per direction; mean ms per sweep over reps rounds
+    void result(int reps, int best[3], float *ms_out) const
+    {
+        for (int d = 0; d < 3; d++) {
+            best[d] = 0;
+            for (int s = 1; s < TILE_NSHAPE; s++)
+                if (ms[d][s] < ms[d][best[d]]) best[d] = s;
+            if (ms_out)
+                for (int s = 0; s < TILE_NSHAPE; s++) ms_out[d * TILE_NSHAPE + s] = ms[d][s] / (float)reps;
+        }
+    }
+};
 
 static inline bool shapes_valid(const int *sh)
 {

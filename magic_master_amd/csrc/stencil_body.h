@@ -100,9 +100,9 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
 // Interior thread mapping: x fastest.
 #define INTERIOR_IJK                                                          \
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;           \
-    const int j = (int)blockIdx.y + 1;                                        \
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;           \
     const int k = (int)blockIdx.z + 1;                                        \
-    if (i > d.mx) return;                                                     \
+    if (i > d.mx || j > d.my) return;                                         \
     const long long c = cid(d, i, j, k);
 
 // diffuse.f:70-118: cross-derivative terms from u (copy of q(2:4) taken

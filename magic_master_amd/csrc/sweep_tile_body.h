@@ -133,8 +133,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     // The states of a lane's interface in the next chunk are loaded while
     // the current chunk's transverse terms are computed (registers), so the
     // HBM latency of part 1 is hidden behind part 3.  Not in the FP64
-    // pencil-major kernel, where the registers would spill.
-    constexpr bool PREFETCH = !(PM && sizeof(T) == 8);
+    // pencil-major kernel at TA = 4, where the registers would spill.
+    constexpr bool PREFETCH = !(PM && sizeof(T) == 8 && TA == 4);
     T pqa[NC], pqb[NC], pga = T(0.0), pgb = T(0.0), pvg = T(0.0);
     auto fetch = [&](int Iq) {
         if (pen && Iq <= n + 2) {

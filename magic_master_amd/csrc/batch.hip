@@ -266,19 +266,13 @@ static int launch_src(magic_batch *b, int s)
         const StepGate gd = gate(b, s, GATE_DIFF, nn);
         TRY(launch_bc(b, s, mthbc, gd));
         k_zghost_uvw<<<gface, 64, 0, b->stream>>>(d, c->q32, gd);
-        TRY(launch_copy(b, c->u32, c->q32 + P, 3 * P, gd));
-        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->q32, c->u32, nu, 0.f, 0.f, 0.f,
+        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->u32, c->q32, nu, 0.f, 0.f, 0.f,
                                                                        gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
-            float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
-            const int f0 = FAC_DIFF + 3 * (cc - 1);
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->qt32, qc, uc, nu, 0.f, 0,
-                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 0));
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, uc, c->qt32, uc, nu, 0.f, 1,
-                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 1));
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, qc, uc, uc, nu, 0.f, 2,
-                                                                     gate(b, s, GATE_DIFF, nn, 0, f0 + 2));
+            k_diff_xyz<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->stream>>>(
+                d, c->q32 + cc * P, c->u32 + (cc - 1) * P, nu, 0.f, 0.f, 0.f, diff_kb(d),
+                gate(b, s, GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)));
             TRY(launch_check());
         }
         float *q5 = c->q32 + 4 * P;

@@ -144,6 +144,8 @@ template __global__ void k_bc_x<double, NoGate>(Dims, double *, int, int, int, N
 template __global__ void k_bc_y<double, NoGate>(Dims, double *, int, int, int, NoGate);
 template __global__ void k_clamp<double, NoGate>(double *, long long, int, NoGate);
 template __global__ void k_zghost_uvw<double, NoGate>(Dims, double *, NoGate);
+template __global__ void k_diff_xyz<double, false, NoGate>(Dims, double *, const double *, const double *, double, double,
+                                                          double, int, NoGate);
 template __global__ void k_diff_cross<double, false, NoGate>(Dims, double *, const double *, const double *, double, double, double,
                                                         NoGate);
 template __global__ void k_copy_interior<double, NoGate>(Dims, double *, const double *, int, NoGate);

@@ -20,6 +20,8 @@ template __global__ void k_bc_x<float, NoGate>(Dims, float *, int, int, int, NoG
 template __global__ void k_bc_y<float, NoGate>(Dims, float *, int, int, int, NoGate);
 template __global__ void k_clamp<float, NoGate>(float *, long long, int, NoGate);
 template __global__ void k_zghost_uvw<float, NoGate>(Dims, float *, NoGate);
+template __global__ void k_diff_xyz<float, true, NoGate>(Dims, float *, const float *, const float *, float, float, float,
+                                                        int, NoGate);
 template __global__ void k_diff_cross<float, true, NoGate>(Dims, float *, const float *, const float *, float, float, float,
                                                         NoGate);
 template __global__ void k_copy_interior<float, NoGate>(Dims, float *, const float *, int, NoGate);
@@ -162,6 +164,8 @@ template __global__ void k_bc_x<float, StepGate>(Dims, float *, int, int, int, S
 template __global__ void k_bc_y<float, StepGate>(Dims, float *, int, int, int, StepGate);
 template __global__ void k_clamp<float, StepGate>(float *, long long, int, StepGate);
 template __global__ void k_zghost_uvw<float, StepGate>(Dims, float *, StepGate);
+template __global__ void k_diff_xyz<float, true, StepGate>(Dims, float *, const float *, const float *, float, float,
+                                                          float, int, StepGate);
 template __global__ void k_diff_cross<float, true, StepGate>(Dims, float *, const float *, const float *, float,
                                                             float, float, StepGate);
 template __global__ void k_copy_interior<float, StepGate>(Dims, float *, const float *, int, StepGate);

@@ -724,14 +724,13 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     }
     for (int nn = 1; nn <= ndtr; nn++) {
         TRY(bc3_launch(c, mthbc));
-        Span sp(c, TC_SRC, 13 + 6 * (ndtrc > 0 ? ndtrc : 0));
+        Span sp(c, TC_SRC, 6 + 6 * (ndtrc > 0 ? ndtrc : 0));
         const double fourthirds = 4.0 / 3.0;
         double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q32, NoGate{});
         TRY(launch_check());
-        HIPCHECK(hipMemcpyAsync(c->u32, c->q32 + P, 3 * P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->q32, c->u32, nu,
+        k_diff_cross<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u32, c->q32, nu,
                                                              (float)(dtr32 / (c->dx * c->dy)),
                                                              (float)(dtr32 / (c->dx * c->dz)),
                                                              (float)(dtr32 / (c->dy * c->dz)), NoGate{});
@@ -746,10 +745,9 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
                 dtdz2 = fourthirds * dtdz2;
                 dtdy2 = dtdy2 / fourthirds;
             }
-            float *uc = c->u32 + (cc - 1) * P, *qc = c->q32 + cc * P;
-            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt32, qc, uc, nu, (float)dtdx2, 0, NoGate{});
-            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, uc, c->qt32, uc, nu, (float)dtdy2, 1, NoGate{});
-            k_pass<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, qc, uc, uc, nu, (float)dtdz2, 2, NoGate{});
+            k_diff_xyz<float, true, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(
+                d, c->q32 + cc * P, c->u32 + (cc - 1) * P, nu, (float)dtdx2, (float)dtdy2, (float)dtdz2, diff_kb(d),
+                NoGate{});
             TRY(launch_check());
         }
         const float cx = (float)((dtrc) / (c->dx * c->dx)), cy = (float)((dtrc) / (c->dy * c->dy)),
@@ -830,18 +828,18 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     }
     for (int nn = 1; nn <= ndtr; nn++) {
         TRY(bc3_launch(c, mthbc));
-        Span sp(c, TC_SRC, 13 + 6 * (ndtrc > 0 ? ndtrc : 0));
-        // diffuse: diffuse.f:37-210.  u = q(2:4) after the z-ghost copy; the
-        // cross terms write q's interior; each component then takes three
-        // directional passes whose interior input is the previous pass's
-        // output and whose ghost input is u (stale, as in the reference).
+        Span sp(c, TC_SRC, 6 + 6 * (ndtrc > 0 ? ndtrc : 0));
+        // diffuse: diffuse.f:37-210.  After the z-ghost fill the cross terms
+        // of q(2:4) go to u's interior; each component then takes its three
+        // directional passes in one kernel, whose ghost input is q's own
+        // ghost layers (the reference's copy, stale as there), and whose
+        // result replaces q's interior.
         const double fourthirds = 4.0 / 3.0;
         double dtdx2 = (dtr) / (c->dx * c->dx), dtdy2 = (dtr) / (c->dy * c->dy), dtdz2 = (dtr) / (c->dz * c->dz);
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q, NoGate{});
         TRY(launch_check());
-        HIPCHECK(hipMemcpyAsync(c->u, c->q + P, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->u, nu, dtr32 / (c->dx * c->dy),
+        k_diff_cross<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u, c->q, nu, dtr32 / (c->dx * c->dy),
                                                  dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
@@ -854,10 +852,8 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdz2 = fourthirds * dtdz2;
                 dtdy2 = dtdy2 / fourthirds;
             }
-            double *uc = c->u + (cc - 1) * P, *qc = c->q + cc * P;
-            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, qc, uc, nu, dtdx2, 0, NoGate{});
-            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, uc, c->qt, uc, nu, dtdy2, 1, NoGate{});
-            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, qc, uc, uc, nu, dtdz2, 2, NoGate{});
+            k_diff_xyz<double, false, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(
+                d, c->q + cc * P, c->u + (cc - 1) * P, nu, dtdx2, dtdy2, dtdz2, diff_kb(d), NoGate{});
             TRY(launch_check());
         }
         // conduct: conduct.f:245-332.  ut = q(5) on every cell once; the

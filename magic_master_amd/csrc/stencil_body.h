@@ -105,10 +105,12 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
     if (i > d.mx || j > d.my) return;                                         \
     const long long c = cid(d, i, j, k);
 
-// diffuse.f:70-118: cross-derivative terms from u (copy of q(2:4) taken
-// after the z-ghost fill); writes q(2:4) on the interior.
+// diffuse.f:70-118: cross-derivative terms of q(2:4) (after the z-ghost
+// fill), written to the interior of the three planes of u; q is not
+// changed (its planes keep the values the reference's copy holds, whose
+// ghost layers the directional passes read).
 template <typename T, bool COLC, class G>
-__global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T dxz, T dyz, G gate)
+__global__ void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz, G gate)
 {
     if (!gate.on()) return;
     dxy = gate.f(dxy, 0);
@@ -116,7 +118,7 @@ __global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T d
     dyz = gate.f(dyz, 2);
     INTERIOR_IJK
     const long long sx = 1, sy = d.NX, sz = (long long)d.NX * d.NY;
-    const T *u1 = u, *u2 = u + d.P, *u3 = u + 2 * d.P;
+    const T *u1 = q + d.P, *u2 = q + 2 * d.P, *u3 = q + 3 * d.P;
     auto X = [&](const T *f, long long a, long long b) {
         return f[c + a + b] - f[c - a + b] + f[c - a - b] - f[c + a - b];
     };
@@ -124,9 +126,82 @@ __global__ void k_diff_cross(Dims d, T *q, const T *u, const T *coef, T dxy, T d
     const T ctyxy = X(u1, sx, sy), ctyyz = X(u3, sy, sz);
     const T ctzxz = X(u1, sx, sz), ctzyz = X(u2, sy, sz);
     const T co = coef[COLC ? (long long)(k + d.mbc - 1) : c];
-    q[d.P + c] = u1[c] + co * (dxy * ctxxy + dxz * ctxxz);
-    q[2 * d.P + c] = u2[c] + co * (dxy * ctyxy + dyz * ctyyz);
-    q[3 * d.P + c] = u3[c] + co * (dxz * ctzxz + dyz * ctzyz);
+    u[c] = u1[c] + co * (dxy * ctxxy + dxz * ctxxz);
+    u[d.P + c] = u2[c] + co * (dxy * ctyxy + dyz * ctyyz);
+    u[2 * d.P + c] = u3[c] + co * (dxz * ctzxz + dyz * ctzyz);
+}
+
+// diffuse.f:153-208 for one velocity component: the x, y and z passes of
+// one cycle of the component loop, fused.  u holds the cross-term result on
+// the interior; the plane q holds the values before the cross terms, whose
+// ghost layers are the reference's stale ghosts of every pass; the result
+// replaces q's interior.  Each pass is the reference's (k_pass below, same
+// operand order and roundings):
+//   x: X = U + fx*co*((U[i+1]-U)-U+U[i-1])   on the interior rows
+//   y: Y = X + fy*co*((X[j+1]-X)-X+X[j-1])   (q's ghosts at j = 0, my+1)
+//   z: q = Y + fz*co*((Y[k-1]-Y)-Y+Y[k+1])   (q's ghosts at k = 0, mz+1)
+// A workgroup covers 64 x-cells by 4 y-rows and kb layers: per layer it
+// forms X on 6 rows (its rows and one on either side) in LDS, Y on its 4
+// rows, and keeps Y of three consecutive layers in registers for the z pass
+// (the layer below its first is recomputed).  Launch: block (64, 4), grid
+// (ceil(mx/64), ceil(my/4), ceil(mz/kb)).
+template <typename T, bool COLC, class G>
+__global__ __launch_bounds__(256) void k_diff_xyz(Dims d, T *q, const T *u, const T *coef, T fx, T fy, T fz, int kb,
+                                                  G gate)
+{
+    if (!gate.on()) return;
+    fx = gate.f(fx, 0);
+    fy = gate.f(fy, 1);
+    fz = gate.f(fz, 2);
+    __shared__ T sx[6][64];
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int i = (int)blockIdx.x * 64 + tx + 1;
+    const int j0 = (int)blockIdx.y * 4 + 1;
+    const int j = j0 + ty;
+    const int k0 = (int)blockIdx.z * kb + 1;
+    const int k1 = k0 + kb - 1 < d.mz ? k0 + kb - 1 : d.mz;
+    const bool own = i <= d.mx && j <= d.my;
+    const long long sy = d.NX, sz = (long long)d.NX * d.NY;
+    auto co_at = [&](long long c, int k) { return coef[COLC ? (long long)(k + d.mbc - 1) : c]; };
+    // Y at (i, j, kk) for an interior layer kk; every lane of the block calls it
+    auto yres = [&](int kk) {
+        __syncthreads();   // the previous layer's readers of sx are done
+        for (int r = ty; r < 6; r += 4) {
+            const int jr = j0 - 1 + r;
+            if (i <= d.mx && jr >= 1 && jr <= d.my) {
+                const long long c = cid(d, i, jr, kk);
+                const T aq = u[c];
+                const T am = i > 1 ? u[c - 1] : q[c - 1];
+                const T ap = i < d.mx ? u[c + 1] : q[c + 1];
+                const T qt = ap - aq - aq + am;
+                sx[r][tx] = aq + fx * co_at(c, kk) * qt;
+            }
+        }
+        __syncthreads();
+        T y = T(0.0);
+        if (own) {
+            const long long c = cid(d, i, j, kk);
+            const T aq = sx[ty + 1][tx];
+            const T am = j > 1 ? sx[ty][tx] : q[c - sy];
+            const T ap = j < d.my ? sx[ty + 2][tx] : q[c + sy];
+            const T qt = ap - aq - aq + am;
+            y = aq + fy * co_at(c, kk) * qt;
+        }
+        return y;
+    };
+    const long long cb = own ? cid(d, i, j, k0) : 0;
+    T ym = k0 > 1 ? yres(k0 - 1) : (own ? q[cb - sz] : T(0.0));
+    T y0 = yres(k0);
+    for (int k = k0; k <= k1; k++) {
+        const long long c = own ? cid(d, i, j, k) : 0;
+        const T yp = k < d.mz ? yres(k + 1) : (own ? q[c + sz] : T(0.0));
+        if (own) {
+            const T qt = ym - y0 - y0 + yp;
+            q[c] = y0 + fz * co_at(c, k) * qt;
+        }
+        ym = y0;
+        y0 = yp;
+    }
 }
 
 // interior copy of planes src -> dst (n planes with stride P)

@@ -11,6 +11,7 @@ namespace magic {
 template <typename T>
 struct RoeT {
     T u, v, w, enth, a, g1a2, euv, u2v2w2;
+    T i2a;   // 1 / (2 a): the wave-strength divisor of rpn3 and of every transverse split
 };
 using Roe = RoeT<double>;
 
@@ -73,7 +74,7 @@ __device__ __forceinline__ void tsplit(const RoeT<T> &R, const T (&in)[NC], T (&
     T w1[5], w2[5], w3[5], sb1, sb2, sb3;
     if (ICOOR == 2) {
         const T a2 = qu - R.u * q1, a3 = qw - R.w * q1;
-        const T a5 = (qv + (R.a - R.v) * q1 - R.a * a4) / (T(2.0) * R.a);
+        const T a5 = (qv + (R.a - R.v) * q1 - R.a * a4) * R.i2a;
         const T a1 = q1 - a4 - a5;
         w1[0] = a1; w1[1] = a1 * R.u; w1[2] = a1 * (R.v - R.a); w1[3] = a1 * R.w;
         w1[4] = a1 * (R.enth - R.v * R.a); sb1 = R.v - R.a;
@@ -83,7 +84,7 @@ __device__ __forceinline__ void tsplit(const RoeT<T> &R, const T (&in)[NC], T (&
         w3[4] = a5 * (R.enth + R.v * R.a); sb3 = R.v + R.a;
     } else {
         const T a2 = (QUIRK ? qw : qu) - R.u * q1, a3 = qv - R.v * q1;
-        const T a5 = (qw + (R.a - R.w) * q1 - R.a * a4) / (T(2.0) * R.a);
+        const T a5 = (qw + (R.a - R.w) * q1 - R.a * a4) * R.i2a;
         const T a1 = q1 - a4 - a5;
         w1[0] = a1; w1[1] = a1 * R.u; w1[2] = a1 * R.v; w1[3] = a1 * (R.w - R.a);
         w1[4] = a1 * (R.enth - R.w * R.a); sb1 = R.w - R.a;

@@ -85,6 +85,7 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs A)
         const double a2 = gamma1 * (R.enth - 0.5 * R.u2v2w2);
         if (!(a2 > 0.0)) report(A, 2, i, pa, pb);
         R.a = sqrt(a2);
+        R.i2a = 1.0 / (2.0 * R.a);
         R.g1a2 = gamma1 / a2;
         R.euv = R.enth - R.u2v2w2;
         // jumps and eigen-coefficients: rpn3euX.f:194-216

@@ -133,8 +133,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     // The states of a lane's interface in the next chunk are loaded while
     // the current chunk's transverse terms are computed (registers), so the
     // HBM latency of part 1 is hidden behind part 3.  Not in the FP64
-    // pencil-major kernel at TA = 4, where the registers would spill.
-    constexpr bool PREFETCH = !(PM && sizeof(T) == 8 && TA == 4);
+    // pencil-major kernels, where the registers would spill.
+    constexpr bool PREFETCH = !(PM && sizeof(T) == 8);
     T pqa[NC], pqb[NC], pga = T(0.0), pgb = T(0.0), pvg = T(0.0);
     auto fetch = [&](int Iq) {
         if (pen && Iq <= n + 2) {
@@ -162,7 +162,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         __syncthreads();   // previous chunk's readers of s_* and T are done
 
         // ---------------- part 1: Roe solve of interface I (rpn3euX.f:138-421)
-        RoeT<T> R = {0, 0, 0, 0, 1, 0, 0, 0};
+        RoeT<T> R = {0, 0, 0, 0, 1, 0, 0, 0, 0};
         T fw1[NC], fw2[NC], fw3[NC], amdq[NC], apdq[NC];
         T s1 = 0, s2 = 0, s3 = 0, c1 = 0, c2 = 0, c3 = 0;
         bool f1 = false, f2 = false, f5 = false;
@@ -219,21 +219,27 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             if (!(qa[0] > T(0.0)) || !(qb[0] > T(0.0))) report_t(A, 1, I, pg, ph);
             const T rhsqrtl = sqrt(qa[0]), rhsqrtr = sqrt(qb[0]);
             const T rhsq2 = rhsqrtl + rhsqrtr;
-            const T gamma = (rhsqrtr * gb + rhsqrtl * ga) / rhsq2;
+            // reciprocals taken once where rpn3euX.f:159-216 divides by the
+            // same value several times (the products differ from the
+            // quotients by at most one rounding)
+            const T il = T(1.0) / rhsqrtl, ir = T(1.0) / rhsqrtr, i2 = T(1.0) / rhsq2;
+            const T iqa = T(1.0) / qa[0], iqb = T(1.0) / qb[0];
+            const T gamma = (rhsqrtr * gb + rhsqrtl * ga) * i2;
             const T gamma1 = gamma - T(1.0);
-            const T pl = (ga - T(1.0)) * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
-            const T pr = (gb - T(1.0)) * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
-            R.u = (qa[1] / rhsqrtl + qb[1] / rhsqrtr) / rhsq2;
-            R.v = (qa[2] / rhsqrtl + qb[2] / rhsqrtr) / rhsq2;
-            R.w = (qa[3] / rhsqrtl + qb[3] / rhsqrtr) / rhsq2;
-            R.enth = ((qa[4] + pl) / rhsqrtl + (qb[4] + pr) / rhsqrtr) / rhsq2;
+            const T pl = (ga - T(1.0)) * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
+            const T pr = (gb - T(1.0)) * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
+            R.u = (qa[1] * il + qb[1] * ir) * i2;
+            R.v = (qa[2] * il + qb[2] * ir) * i2;
+            R.w = (qa[3] * il + qb[3] * ir) * i2;
+            R.enth = ((qa[4] + pl) * il + (qb[4] + pr) * ir) * i2;
             R.u2v2w2 = R.u * R.u + R.v * R.v + R.w * R.w;
             const T a2 = gamma1 * (R.enth - T(0.5) * R.u2v2w2);
             if (!(a2 > T(0.0))) report_t(A, 2, I, pg, ph);
             R.a = sqrt(a2);
+            R.i2a = T(1.0) / (T(2.0) * R.a);
             R.g1a2 = gamma1 / a2;
             R.euv = R.enth - R.u2v2w2;
-            const T ur = qb[1] / qb[0], ul = qa[1] / qa[0];
+            const T ur = qb[1] * iqb, ul = qa[1] * iqa;
             const T d1 = qb[1] - qa[1];
             T d2;
             if constexpr (PERT) d2 = (qb[1] * ur + ppb) - (qa[1] * ul + ppa);   // background part below
@@ -257,7 +263,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             const T b4 = R.g1a2 * (R.euv * d1 + R.u * d2 + R.v * d3 + R.w * d4 - d5);
             const T b2 = d3 - R.v * d1;
             const T b3 = d4 - R.w * d1;
-            const T b5 = (d2 + (R.a - R.u) * d1 - R.a * b4) / (T(2.0) * R.a);
+            const T b5 = (d2 + (R.a - R.u) * d1 - R.a * b4) * R.i2a;
             const T b1 = d1 - b4 - b5;
             fw1[0] = b1; fw1[1] = b1 * (R.u - R.a); fw1[2] = b1 * R.v; fw1[3] = b1 * R.w;
             fw1[4] = b1 * (R.enth - R.u * R.a);
@@ -274,48 +280,50 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             // entropy fix and amdq: rpn3euX.f:316-398
             {
                 const T gmm = ga, gm1 = ga - T(1.0);
-                const T rhoim1 = qa[0];
-                const T pim1 = gm1 * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / rhoim1);
-                const T cim1 = sqrt(gmm * pim1 / rhoim1);
-                const T s0 = qa[1] / rhoim1 - cim1;
+                const T pim1 = gm1 * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
+                const T cim1 = sqrt(gmm * pim1 * iqa);
+                const T s0 = qa[1] * iqa - cim1;
                 if (!(s0 >= T(0.0) && s1 > T(0.0))) {
-                    const T rho1 = qa[0] + fw1[0] / s1;
-                    const T rhou1 = qa[1] + fw1[1] / s1;
-                    const T rhov1 = qa[2] + fw1[2] / s1;
-                    const T rhow1 = qa[3] + fw1[3] / s1;
-                    const T en1 = qa[4] + fw1[4] / s1;
-                    const T p1 = gm1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
-                    const T cc1 = sqrt(gmm * p1 / rho1);
-                    const T sa = rhou1 / rho1 - cc1;
+                    const T is1 = T(1.0) / s1;
+                    const T rho1 = qa[0] + fw1[0] * is1;
+                    const T rhou1 = qa[1] + fw1[1] * is1;
+                    const T rhov1 = qa[2] + fw1[2] * is1;
+                    const T rhow1 = qa[3] + fw1[3] * is1;
+                    const T en1 = qa[4] + fw1[4] * is1;
+                    const T ir1 = T(1.0) / rho1;
+                    const T p1 = gm1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) * ir1);
+                    const T cc1 = sqrt(gmm * p1 * ir1);
+                    const T sa = rhou1 * ir1 - cc1;
                     T sfract;
                     if (s0 < T(0.0) && sa > T(0.0)) sfract = s0 * (sa - s1) / (sa - s0);
                     else if (s1 < T(0.0)) sfract = s1;
                     else sfract = T(0.0);
 #pragma unroll
-                    for (int m = 0; m < NC; m++) amdq[m] = sfract * fw1[m] / s1;
+                    for (int m = 0; m < NC; m++) amdq[m] = sfract * fw1[m] * is1;
                     if (s2 < T(0.0)) {
 #pragma unroll
                         for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + fw2[m];
                         const T gm = gb, g1 = gb - T(1.0);
-                        const T rhoi = qb[0];
-                        const T pi_ = g1 * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / rhoi);
-                        const T ci = sqrt(gm * pi_ / rhoi);
-                        const T s3r = qb[1] / rhoi + ci;
-                        const T rho2 = qb[0] - fw3[0] / s3;
-                        const T rhou2 = qb[1] - fw3[1] / s3;
-                        const T rhov2 = qb[2] - fw3[2] / s3;
-                        const T rhow2 = qb[3] - fw3[3] / s3;
-                        const T en2 = qb[4] - fw3[4] / s3;
-                        const T p2 = g1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
-                        const T cc2 = sqrt(gm * p2 / rho2);
-                        const T sb = rhou2 / rho2 + cc2;
+                        const T pi_ = g1 * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
+                        const T ci = sqrt(gm * pi_ * iqb);
+                        const T s3r = qb[1] * iqb + ci;
+                        const T is3 = T(1.0) / s3;
+                        const T rho2 = qb[0] - fw3[0] * is3;
+                        const T rhou2 = qb[1] - fw3[1] * is3;
+                        const T rhov2 = qb[2] - fw3[2] * is3;
+                        const T rhow2 = qb[3] - fw3[3] * is3;
+                        const T en2 = qb[4] - fw3[4] * is3;
+                        const T ir2 = T(1.0) / rho2;
+                        const T p2 = g1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) * ir2);
+                        const T cc2 = sqrt(gm * p2 * ir2);
+                        const T sb = rhou2 * ir2 + cc2;
                         bool add = true;
                         if (sb < T(0.0) && s3r > T(0.0)) sfract = sb * (s3r - s3) / (s3r - sb);
                         else if (s3 < T(0.0)) sfract = s3;
                         else add = false;
                         if (add) {
 #pragma unroll
-                            for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + sfract * fw3[m] / s3;
+                            for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + sfract * fw3[m] * is3;
                         }
                     }
                 }
@@ -364,7 +372,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 const int up = (int)(-copysign(T(1.0), s1));
                 const T b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
                                               R.w * del(3, up) - del(4, up));
-                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) / (T(2.0) * R.a);
+                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) * R.i2a;
                 const T b1Id = del(0, up) - b4Id - b5Id;
                 wl1 = suppow(b1Id / b1, c1, c1);
             }
@@ -397,7 +405,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 const int up = (int)(-copysign(T(1.0), s3));
                 const T b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
                                               R.w * del(3, up) - del(4, up));
-                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) / (T(2.0) * R.a);
+                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) * R.i2a;
                 wl5 = suppow(b5Id / b5, c3, c3);
             }
 #pragma unroll
@@ -591,14 +599,14 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 
 // Waves per SIMD the register allocation aims at: two for the FP32 form
 // with tile-axis-major lanes and no tracer, and for the work-list form with
-// one tracer at TA = 8 (it misses 256 registers by one unless asked), one
-// otherwise.  The other FP32 tile-axis-major kernels with one tracer reach
-// two waves without being asked, and asking changes their schedule for the
-// worse (5 % on the batched ensemble, a spill at TA = 8 in the one-grid form).
+// one tracer (left to itself the allocator can miss 256 registers by one and
+// halve the occupancy), one otherwise.  The one-grid kernels with one tracer
+// reach two waves without being asked, and asking changes their schedule
+// for the worse (6 % on the full-domain FP32 step).
 template <typename T, int NTR, int SH, bool LIST>
 constexpr int tile_waves()
 {
-    return sizeof(T) == 4 && !TileShape<SH>::PM && (NTR == 0 || (LIST && NTR == 1 && TileShape<SH>::TA == 8)) ? 2 : 1;
+    return sizeof(T) == 4 && !TileShape<SH>::PM && (NTR == 0 || (LIST && NTR == 1)) ? 2 : 1;
 }
 
 template <typename T, bool PERT, int NTR, int SH>

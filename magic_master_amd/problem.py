@@ -404,15 +404,16 @@ def qinit(g: Grid, aux: np.ndarray, noise: float = 0.0, seed: int = 0,
 # ---------------------------------------------------------------------------
 # Named workloads
 
-def config_plain(mx=64, my=64, mz=48, noise=1e-6, seed=1):
+def config_plain(mx=64, my=64, mz=48, noise=1e-6, seed=1, meqn=6):
     """Plain MAGIC step on an isothermal 250 K atmosphere: 64x64x48 cells,
     dx = dy = 4 km, dz = 2 km, gamma 1.4, R 287, g 9.81, surface rho 1.2,
     mu = 1e-5 constant (Prandtl 0.7), Gaussian plume forcing (forcemth 3,
     6 km / 600 s, centred at the domain middle and t = 600 s, 20 mm/10 min),
-    periodic sides, wall bottom, extrapolation top, CFL 0.9, tracer advected."""
+    periodic sides, wall bottom, extrapolation top, CFL 0.9, tracer advected
+    (meqn - 5 tracers, 6 by default)."""
     c = ClawData(mx=mx, my=my, mz=mz, outstyle=3, nstepout=200, nstop=200, nout=200,
                  xupper=4e3 * mx, yupper=4e3 * my, zupper=2e3 * mz,
-                 mthbc=[2, 2, 2, 2, 3, 1], meqn=6)
+                 mthbc=[2, 2, 2, 2, 3, 1], meqn=meqn)
     c.method[3] = 0
     p = ProbData()
     p.pres = p.dens * 287.0 * 250.0

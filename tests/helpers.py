@@ -17,11 +17,14 @@ CASES = [
 ]
 
 
-def perturbed(mx, my, mz, mthbc, seed=0, vel=0.5, visc=1e6, ntr_values=True):
+def perturbed(mx, my, mz, mthbc, seed=0, vel=0.5, visc=1e6, ntr_values=True, meqn=6, nspecadv=1):
     """Plain isothermal workload on a small grid with random velocities,
-    temperature noise, a tracer field in [0, 1] and amplified diffusivities
-    (so the source pass does measurable work), plume at the domain centre."""
-    c, p, g, aux, q = P.config_plain(mx=mx, my=my, mz=mz, noise=1e-3, seed=seed + 3)
+    temperature noise, tracer fields in [0, 1] (meqn - 5 of them, the first
+    nspecadv advected, the others in continuity form) and amplified
+    diffusivities (so the source pass does measurable work), plume at the
+    domain centre."""
+    c, p, g, aux, q = P.config_plain(mx=mx, my=my, mz=mz, noise=1e-3, seed=seed + 3, meqn=meqn)
+    p.nspecadv = nspecadv
     c.mthbc = list(mthbc)
     aux[..., 10] *= visc
     aux[..., 11] *= visc
@@ -33,8 +36,9 @@ def perturbed(mx, my, mz, mthbc, seed=0, vel=0.5, visc=1e6, ntr_values=True):
     q[..., 1:4] += vel * rng.standard_normal(q.shape[:3] + (3,)) * q[..., 0:1]
     q[..., 4] *= 1 + 1e-3 * rng.standard_normal(q.shape[:3])
     q[..., 4] += 0.5 * (q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 3] ** 2) / q[..., 0]
-    if ntr_values and g.meqn > 5:
-        q[..., 5] = np.clip(rng.random(q.shape[:3]) * 1.2 - 0.1, 0, 1)
+    if ntr_values:
+        for m in range(5, g.meqn):
+            q[..., m] = np.clip(rng.random(q.shape[:3]) * 1.2 - 0.1, 0, 1)
     return c, p, g, aux, q
 
 

@@ -57,3 +57,31 @@ def test_claw3_bitwise(ci, both):
         dto[0] = dto[4]
         dtr[0] = dtr[4]
     assert np.array_equal(qo, qr)
+
+
+@pytest.mark.parametrize("meqn,nspec", [(5, 0), (7, 1), (7, 2), (10, 3)])
+def test_tracer_counts_bitwise(meqn, nspec, both):
+    """No tracer, and 2 and 5 tracers with some advected (tracer limiter,
+    rpn3euX.f:449-456) and the rest in continuity form (minmod, :462)."""
+    O, R = both
+    (mx, my, mz), mthbc = CASES[0]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=meqn + 3 * nspec, vel=0.1, visc=1e8, meqn=meqn,
+                                nspecadv=nspec)
+    R.setup(p, mthbc)
+    qo, co, st = O.step3(g, q, aux, 1.5, nspec)
+    qr, cr = R.step3(g, q, aux, 1.5)
+    assert st == 0 and co == cr and np.array_equal(qo, qr)
+    qo, qr = q.copy(), q.copy()
+    dto = [30.0, 10.0, 0, 0, 0]
+    dtr = list(dto)
+    t_o = t_r = 0.0
+    for n in range(3):
+        st, t_o, dto, co, _, _ = O.claw3(g, p, qo, aux, t_o, t_o - 1.0, dto, c.cflv, [100, 0],
+                                         c.method, mthbc)
+        info, t_r, dtr, cr, _ = R.claw3(g, qr, aux, t_r, t_r - 1.0, dtr, c.cflv, [100, 0],
+                                        c.method, mthbc)
+        assert st == 0 and info == 0
+        assert t_o == t_r and dto == dtr and co == cr
+        dto[0] = dto[4]
+        dtr[0] = dtr[4]
+    assert np.array_equal(qo, qr)

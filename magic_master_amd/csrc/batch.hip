@@ -25,6 +25,14 @@ struct magic_batch {
     int cap = 0;                         // slots the device arrays hold
     int meqn = 0, mbc = 0, maux = 0, form = 0;
     hipStream_t stream = nullptr;
+    // The per-scenario kernel chains of an attempt (boundaries and backup,
+    // then restore and source pass) are independent across scenarios: the
+    // capture spreads them over NSIDE side streams (graph branches), so small
+    // scenarios' kernels overlap instead of running one after the other.
+    static constexpr int NSIDE = 4;
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t fork_ev[2] = {}, join_ev[2][NSIDE] = {};
+    hipStream_t cs = nullptr;            // stream of the per-scenario launches being recorded
     std::vector<magic_ctx *> sc;         // nullptr: empty slot
     std::vector<ScenCtl> ctl;            // host copy of the control blocks
     std::vector<int> mthbc;              // 6 per scenario
@@ -223,13 +231,13 @@ static int launch_bc(magic_batch *b, int s, const int *mthbc, StepGate g)
     const Dims &d = c->d;
     auto lohi = [](int m) { return m < 0 ? 0 : m; };
     if (mthbc[0] >= 0 || mthbc[1] >= 0)
-        k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
+        k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
                                                                  lohi(mthbc[1]), g);
     if (mthbc[2] >= 0 || mthbc[3] >= 0)
-        k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
+        k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
                                                                  lohi(mthbc[3]), g);
     if (mthbc[4] >= 0 || mthbc[5] >= 0)
-        k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, b->stream>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
+        k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
                                                                    lohi(mthbc[5]), c->zr32, g);
     return launch_check();
 }
@@ -238,7 +246,7 @@ static int launch_copy(magic_batch *b, float *dst, const float *src, long long n
 {
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
-    k_copy<float, StepGate><<<(unsigned)nb, 256, 0, b->stream>>>(dst, src, n, g);
+    k_copy<float, StepGate><<<(unsigned)nb, 256, 0, b->cs>>>(dst, src, n, g);
     return launch_check();
 }
 
@@ -259,18 +267,18 @@ static int launch_src(magic_batch *b, int s)
     memcpy(mthbc, &b->mthbc[6 * s], sizeof(mthbc));
     mthbc[4] = -1;
     mthbc[5] = -1;
-    k32_src_convert<<<padded_cells(d), cell_block(), 0, b->stream>>>(d, c->q32, col,
+    k32_src_convert<<<padded_cells(d), cell_block(), 0, b->cs>>>(d, c->q32, col,
                                                                              gate(b, s, GATE_ACCEPTED));
     TRY(launch_check());
     for (int nn = 1; nn <= h.ndtr_cap; nn++) {
         const StepGate gd = gate(b, s, GATE_DIFF, nn);
         TRY(launch_bc(b, s, mthbc, gd));
-        k_zghost_uvw<<<gface, 64, 0, b->stream>>>(d, c->q32, gd);
-        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->u32, c->q32, nu, 0.f, 0.f, 0.f,
+        k_zghost_uvw<<<gface, 64, 0, b->cs>>>(d, c->q32, gd);
+        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->u32, c->q32, nu, 0.f, 0.f, 0.f,
                                                                        gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
-            k_diff_xyz<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->stream>>>(
+            k_diff_xyz<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->cs>>>(
                 d, c->q32 + cc * P, c->u32 + (cc - 1) * P, nu, 0.f, 0.f, 0.f, diff_kb(d),
                 gate(b, s, GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)));
             TRY(launch_check());
@@ -279,19 +287,19 @@ static int launch_src(magic_batch *b, int s)
         TRY(launch_copy(b, c->ut32, q5, P, gd));
         for (int kk = 1; kk <= h.ndtrc_cap; kk++) {
             const StepGate gk = gate(b, s, GATE_COND, nn, kk);
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
                                                                      gate(b, s, GATE_COND, nn, kk, FAC_COND + 0));
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
                                                                      gate(b, s, GATE_COND, nn, kk, FAC_COND + 1));
-            k_conduct_zghost<<<gface, 64, 0, b->stream>>>(d, c->ut32, gk);
+            k_conduct_zghost<<<gface, 64, 0, b->cs>>>(d, c->ut32, gk);
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
-                k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->stream>>>(d, c->ut32, S, gk);
+                k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->cs>>>(d, c->ut32, S, gk);
             }
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
                 d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
-            k_copy_interior<<<gint, cell_block(), 0, b->stream>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->stream>>>(
+            k_copy_interior<<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
+            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
                 d, q5, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_LAST, nn, kk, FAC_COND + 2));
             TRY(launch_check());
         }
@@ -306,36 +314,62 @@ static int launch_src(magic_batch *b, int s)
     f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
     f.tcenter = p.tcenter; f.twidth = p.twidth;
     f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
-    k32_src_finish<<<gint, cell_block(), 0, b->stream>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
+    k32_src_finish<<<gint, cell_block(), 0, b->cs>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
     return launch_check();
+}
+
+// the side streams wait for the main stream (fork) / the main stream waits
+// for them (join), at recording point k
+static int fork_side(magic_batch *b, int k)
+{
+    HIPCHECK(hipEventRecord(b->fork_ev[k], b->stream));
+    for (int i = 0; i < magic_batch::NSIDE; i++) HIPCHECK(hipStreamWaitEvent(b->side[i], b->fork_ev[k], 0));
+    return MAGIC_OK;
+}
+
+static int join_side(magic_batch *b, int k)
+{
+    for (int i = 0; i < magic_batch::NSIDE; i++) {
+        HIPCHECK(hipEventRecord(b->join_ev[k][i], b->side[i]));
+        HIPCHECK(hipStreamWaitEvent(b->stream, b->join_ev[k][i], 0));
+    }
+    b->cs = b->stream;
+    return MAGIC_OK;
 }
 
 // one attempt of every scenario, as launched into the capture
 static int record_attempt(magic_batch *b)
 {
+    b->cs = b->stream;
     const unsigned nb = blocks(b->n > 18 ? b->n : 18, 256);
     k_batch_prepare<<<nb, 256, 0, b->stream>>>(b->ctl_d, b->n, b->args_d, b->counters_d);
     TRY(launch_check());
-    for (int s = 0; s < b->n; s++) {
+    TRY(fork_side(b, 0));
+    for (int s = 0, k = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (!c) continue;
+        b->cs = b->side[k++ % magic_batch::NSIDE];
         const StepGate g = gate(b, s, GATE_RUN);
         TRY(launch_bc(b, s, &b->mthbc[6 * s], g));
         if (c->meqn > 5) {
-            k_clamp<<<blocks(c->P, 256), 256, 0, b->stream>>>(c->q32, c->P, c->meqn, g);
+            k_clamp<<<blocks(c->P, 256), 256, 0, b->cs>>>(c->q32, c->P, c->meqn, g);
             TRY(launch_check());
         }
         TRY(launch_copy(b, c->qold32, c->q32, c->P * c->meqn, g));
     }
+    TRY(join_side(b, 0));
     TRY(launch_sweeps(b));
     k_batch_decide<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     TRY(launch_check());
-    for (int s = 0; s < b->n; s++) {
+    TRY(fork_side(b, 1));
+    for (int s = 0, k = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (!c) continue;
+        b->cs = b->side[k++ % magic_batch::NSIDE];
         TRY(launch_copy(b, c->q32, c->qold32, c->P * c->meqn, gate(b, s, GATE_REJECTED)));
         TRY(launch_src(b, s));
     }
+    TRY(join_side(b, 1));
     k_batch_finalize<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     return launch_check();
 }
@@ -468,6 +502,7 @@ static int build_graph(magic_batch *b)
     TRY(build_lists(b));
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = record_attempt(b);
+    b->cs = b->stream;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (st != MAGIC_OK) {
@@ -563,6 +598,14 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
     b->form = form;
     int st = MAGIC_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
+    b->cs = b->stream;
+    for (int i = 0; i < magic_batch::NSIDE && st == MAGIC_OK; i++)
+        if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
+    for (int k = 0; k < 2 && st == MAGIC_OK; k++) {
+        if (hipEventCreateWithFlags(&b->fork_ev[k], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
+        for (int i = 0; i < magic_batch::NSIDE && st == MAGIC_OK; i++)
+            if (hipEventCreateWithFlags(&b->join_ev[k][i], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
+    }
     if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
     if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
     if (st == MAGIC_OK) st = hipMalloc(&b->one_d, sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
@@ -593,6 +636,16 @@ extern "C" int magic_batch_destroy(magic_batch *b)
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
     hipFree(b->one_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
+    for (int k = 0; k < 2; k++) {
+        if (b->fork_ev[k]) hipEventDestroy(b->fork_ev[k]);
+        for (int i = 0; i < magic_batch::NSIDE; i++)
+            if (b->join_ev[k][i]) hipEventDestroy(b->join_ev[k][i]);
+    }
+    for (int i = 0; i < magic_batch::NSIDE; i++)
+        if (b->side[i]) {
+            hipStreamSynchronize(b->side[i]);
+            hipStreamDestroy(b->side[i]);
+        }
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return MAGIC_OK;

@@ -66,7 +66,8 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
                        const double *zratio);
 template <typename T, class G>
 __global__ void k_clamp(T *q, long long P, int nc, G gate);
-__global__ void k_src_convert(long long P, double *q, const Aux a);
+template <bool COL>
+__global__ void k_src_convert(Dims d, double *q, const Aux a);
 template <typename T, class G>
 __global__ void k_zghost_uvw(Dims d, T *q, G gate);
 template <typename T, bool COLC, class G>
@@ -81,6 +82,7 @@ template <typename T, class G>
 __global__ void k_conduct_zspill(Dims d, T *u, int S, G gate);
 template <typename T, bool COLC, class G>
 __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate);
+template <bool COL>
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
 

@@ -59,18 +59,21 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
 }
 
 // ---------------------------------------------------------------- src3 ---
-// conserved -> (rho, u', v', w', T') on every cell: src3.f90:242-304
-__global__ void k_src_convert(long long P, double *q, const Aux a)
+// conserved -> (rho, u', v', w', T') on every cell: src3.f90:242-304.
+// COL: the background comes as columns (index kk), else as planes.
+template <bool COL>
+__global__ void k_src_convert(Dims d, double *q, const Aux a)
 {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= P) return;
-    const double gamma1 = a.gam[c] - 1.0;
+    PADDED_IJK
+    const long long P = d.P;
+    const long long x = COL ? (long long)kk : c;
+    const double gamma1 = a.gam[x] - 1.0;
     const double q1 = q[c], q2 = q[P + c], q3 = q[2 * P + c], q4 = q[3 * P + c];
-    const double a1 = a.rho[c], a2 = a.mx[c], a3 = a.my[c], a4 = a.mz[c];
+    const double a1 = a.rho[x], a2 = a.mx[x], a3 = a.my[x], a4 = a.mz[x];
     const double qtemp = gamma1 * (q[4 * P + c] - 0.5 * (1.0 / q1) * (q2 * q2 + q3 * q3 + q4 * q4)) /
-                         (q1 * a.R[c]);
-    const double auxtemp = gamma1 * (a.E[c] - 0.5 * (1.0 / a1) * (a2 * a2 + a3 * a3 + a4 * a4)) /
-                           (a1 * a.R[c]);
+                         (q1 * a.R[x]);
+    const double auxtemp = gamma1 * (a.E[x] - 0.5 * (1.0 / a1) * (a2 * a2 + a3 * a3 + a4 * a4)) /
+                           (a1 * a.R[x]);
     q[4 * P + c] = qtemp - auxtemp;
     q[P + c] = q2 / q1 - a2 / a1;
     q[2 * P + c] = q3 / q1 - a3 / a1;
@@ -79,17 +82,19 @@ __global__ void k_src_convert(long long P, double *q, const Aux a)
 
 // src3.f90:380-418 (reassembly), 430-445 / 514-607 (forcing), 616-626
 // (tracer clamp), on the interior.
+template <bool COL>
 __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f)
 {
     INTERIOR_IJK
     const long long P = d.P;
-    const double gamma1 = a.gam[c] - 1.0;
-    const double a1 = a.rho[c], a2 = a.mx[c], a3 = a.my[c], a4 = a.mz[c], R = a.R[c];
+    const long long x = COL ? (long long)(k + d.mbc - 1) : c;
+    const double gamma1 = a.gam[x] - 1.0;
+    const double a1 = a.rho[x], a2 = a.mx[x], a3 = a.my[x], a4 = a.mz[x], R = a.R[x];
     const double q1 = q[c];
     const double q2 = q[P + c] * q1 + (a2 / a1) * q1;
     const double q3 = q[2 * P + c] * q1 + (a3 / a1) * q1;
     double q4 = q[3 * P + c] * q1 + (a4 / a1) * q1;
-    const double auxtemp = gamma1 * (a.E[c] - 0.5 * (1.0 / a1) * (a2 * a2 + a3 * a3 + a4 * a4)) / (a1 * R);
+    const double auxtemp = gamma1 * (a.E[x] - 0.5 * (1.0 / a1) * (a2 * a2 + a3 * a3 + a4 * a4)) / (a1 * R);
     double q5 = (auxtemp + q[4 * P + c]) * q1 * R / gamma1 + 0.5 * (1.0 / q1) * (q2 * q2 + q3 * q3 + q4 * q4);
     const double xcell = f.xlower + (i - 0.5) * f.dx - f.t * f.vsrcx;
     const double ycell = f.ylower + (j - 0.5) * f.dy;
@@ -144,6 +149,16 @@ template __global__ void k_bc_x<double, NoGate>(Dims, double *, int, int, int, N
 template __global__ void k_bc_y<double, NoGate>(Dims, double *, int, int, int, NoGate);
 template __global__ void k_clamp<double, NoGate>(double *, long long, int, NoGate);
 template __global__ void k_zghost_uvw<double, NoGate>(Dims, double *, NoGate);
+template __global__ void k_src_convert<false>(Dims, double *, const Aux);
+template __global__ void k_src_convert<true>(Dims, double *, const Aux);
+template __global__ void k_src_finish<false>(Dims, double *, int, const Aux, Forcing);
+template __global__ void k_src_finish<true>(Dims, double *, int, const Aux, Forcing);
+template __global__ void k_diff_xyz<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,
+                                                         double, int, NoGate);
+template __global__ void k_diff_cross<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,
+                                                           double, NoGate);
+template __global__ void k_pass<double, true, NoGate>(Dims, double *, const double *, const double *, const double *, double,
+                                                     int, NoGate);
 template __global__ void k_diff_xyz<double, false, NoGate>(Dims, double *, const double *, const double *, double, double,
                                                           double, int, NoGate);
 template __global__ void k_diff_cross<double, false, NoGate>(Dims, double *, const double *, const double *, double, double, double,

@@ -71,14 +71,6 @@ __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G ga
     }
 }
 
-// All-cells thread mapping over the padded grid: x fastest.
-#define PADDED_IJK                                                            \
-    const int ip = (int)(blockIdx.x * blockDim.x + threadIdx.x);              \
-    const int jp = (int)(blockIdx.y * blockDim.y + threadIdx.y);              \
-    if (ip >= d.NX || jp >= d.NY) return;                                     \
-    const int kk = (int)blockIdx.z;                                           \
-    const long long c = ((long long)kk * d.NY + jp) * d.NX + ip;
-
 // conserved perturbation -> (rho', u, v, w, T') on every cell
 // (src3.f90:242-304 with u_bar = 0):
 //   T' = T - T_bar = (p' - rho' (gamma-1) E_bar / rho_bar) / (rho R),

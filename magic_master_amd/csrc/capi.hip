@@ -94,6 +94,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
     ok = ok && hipMalloc(&c->u, 3 * P * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->qt, P * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->ut, P * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&c->col64, (size_t)NAUXP * c->d.NZ * sizeof(double)) == hipSuccess;
     if (ok) {
         ok = hipMemsetAsync(c->q, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
              hipMemsetAsync(c->qold, 0, P * meqn * sizeof(double), c->stream) == hipSuccess &&
@@ -121,7 +122,7 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->q); hipFree(c->qold); hipFree(c->aux); hipFree(c->zratio); hipFree(c->stage);
     hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal); hipFree(c->sink);
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
-    hipFree(c->col32); hipFree(c->bg64);
+    hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64);
     if (c->hscal) hipHostFree(c->hscal);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
@@ -221,6 +222,29 @@ static int set_aux64(magic_ctx *c, F get)
     }
     HIPCHECK(hipMemcpyAsync(c->aux, h.data(), (size_t)P * NAUXP * sizeof(double), hipMemcpyHostToDevice,
                             c->stream));
+    {   // layered background: every slot constant on each layer, bit for bit
+        const long long NXY = (long long)c->d.NX * c->d.NY;
+        const int NZ = c->d.NZ;
+        bool layered = true;
+        for (int s = 0; s < NAUXP && layered; s++)
+            for (int k = 0; k < NZ && layered; k++) {
+                const double *row = h.data() + (size_t)s * P + (size_t)k * NXY;
+                for (long long e = 1; e < NXY; e++)
+                    if (memcmp(row + e, row, sizeof(double)) != 0) {
+                        layered = false;
+                        break;
+                    }
+            }
+        c->aux_layered = layered;
+        if (layered) {
+            std::vector<double> cols((size_t)NAUXP * NZ);
+            for (int s = 0; s < NAUXP; s++)
+                for (int k = 0; k < NZ; k++) cols[(size_t)s * NZ + k] = h[(size_t)s * P + (size_t)k * NXY];
+            HIPCHECK(hipMemcpyAsync(c->col64, cols.data(), cols.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+        }
+    }
     // bc3 z-face scale factors (aux(1,ghost)/aux(1,boundary))**0.5 in FP64
     // (bc3_mpi.f:560-573, 676-689), evaluated once on the host.
     const int NX = c->d.NX, NY = c->d.NY, mbc = c->mbc, mz = c->mz;
@@ -779,51 +803,26 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     return MAGIC_OK;
 }
 
-static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, int *ndtr_out,
-                       int *ndtrc_out)
+// The source pass of the FP64 form (src3.f90:203-634 with diffuse.f and
+// conduct.f).  COL: the background (aux 1-5, 11, 12, 18, 19) is read from
+// its columns, which hold the same values when it is constant on each layer.
+template <bool COL>
+static int src3_passes64(magic_ctx *c, const int *mthbc, int ndtr, double dtr, int ndtrc, double dtrc,
+                         const Forcing &f)
 {
-    const magic_params &p = c->prm;
-    if (p.omega1 != 0.0) return MAGIC_EUNSUPPORTED;   // time-dependent wind (src3.f90:389-399)
-    const int fm = (int)p.forcemth;
-    if (!(p.forcemth == 0.0 || p.forcemth == 1.0 || p.forcemth == 3.0)) return MAGIC_EUNSUPPORTED;
     const Dims &d = c->d;
-    const long long P = c->P, NXY = (long long)d.NX * d.NY;
-    const double *A = c->aux;
-    Aux ax = {A + AX_RHO * P, A + AX_MX * P, A + AX_MY * P, A + AX_MZ * P, A + AX_E * P, A + AX_R * P,
-              A + AX_GAM * P};
-    // substep counts: src3.f90:314-322
-    double dmin = c->dx < c->dy ? c->dx : c->dy;
-    dmin = dmin < c->dz ? dmin : c->dz;
-    const double dxyzmin2 = dmin * dmin;
-    const double difmax = (4.0 / 3.0) * c->aux11_top;
-    const int ndtr = (int)std::ceil((dt * difmax / (dxyzmin2)) / 0.45);
-    const double dtr = dt / ndtr;
-    const int ndtrc = (int)std::ceil((dtr * c->aux12_top / (dxyzmin2)) / 0.45);
-    const double dtrc = dtr / ndtrc;
-    *ndtr_out = ndtr;
-    *ndtrc_out = ndtrc;
-    int mthbc[6];
-    memcpy(mthbc, mthbc_in, sizeof(mthbc));
-    mthbc[4] = -1;
-    mthbc[5] = -1;
-    const dim3 gcell(blocks(P, 256));
+    const long long P = c->P;
+    const int NZ = d.NZ;
+    const double *A = COL ? c->col64 : c->aux;
+    const long long S = COL ? NZ : P;   // stride between slots
+    const Aux ax = {A + AX_RHO * S, A + AX_MX * S, A + AX_MY * S, A + AX_MZ * S, A + AX_E * S, A + AX_R * S,
+                    A + AX_GAM * S};
+    const double *nu = A + AX_NU * S, *kap = A + AX_KAP * S;
     const dim3 gint = interior_grid(d);
     const dim3 gface(blocks(d.NX, 64), d.NY);
-    Forcing f;
-    f.forcemth = fm;
-    f.t = t; f.dt = dt;
-    f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
-    f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
-    f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
-    f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
-    f.tcenter = p.tcenter; f.twidth = p.twidth;
-    f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
-    f.tgauss = std::exp(-.5 * ((t - p.tcenter) * (t - p.tcenter) / (p.twidth * p.twidth)));
-    if (c->form == MAGIC_FORM_FP32P) return src3_passes32(c, mthbc, ndtr, dtr, ndtrc, dtrc, f);
-    const double *nu = A + AX_NU * P, *kap = A + AX_KAP * P;
     {
         Span sp(c, TC_SRC, 1);
-        k_src_convert<<<gcell, 256, 0, c->stream>>>(P, c->q, ax);
+        k_src_convert<COL><<<padded_cells(d), cell_block(), 0, c->stream>>>(d, c->q, ax);
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -839,7 +838,7 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q, NoGate{});
         TRY(launch_check());
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u, c->q, nu, dtr32 / (c->dx * c->dy),
+        k_diff_cross<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u, c->q, nu, dtr32 / (c->dx * c->dy),
                                                  dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
@@ -852,7 +851,7 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
                 dtdz2 = fourthirds * dtdz2;
                 dtdy2 = dtdy2 / fourthirds;
             }
-            k_diff_xyz<double, false, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(
+            k_diff_xyz<double, COL, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(
                 d, c->q + cc * P, c->u + (cc - 1) * P, nu, dtdx2, dtdy2, dtdz2, diff_kb(d), NoGate{});
             TRY(launch_check());
         }
@@ -864,28 +863,65 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
         double *q5 = c->q + 4 * P;
         HIPCHECK(hipMemcpyAsync(c->ut, q5, P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int k = 1; k <= ndtrc; k++) {
-            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0, NoGate{});
-            k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1, NoGate{});
+            k_pass<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cx, 0, NoGate{});
+            k_pass<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->ut, c->qt, c->ut, kap, cy, 1, NoGate{});
             k_conduct_zghost<<<gface, 64, 0, c->stream>>>(d, c->ut, NoGate{});
             if (c->mz > c->mx) {
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, c->stream>>>(d, c->ut, S, NoGate{});
             }
             if (k < ndtrc) {
-                k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2, NoGate{});
+                k_pass<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->qt, c->ut, c->ut, kap, cz, 2, NoGate{});
                 k_copy_interior<<<gint, cell_block(), 0, c->stream>>>(d, c->ut, c->qt, 1, NoGate{});
             } else {
-                k_pass<double, false, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2, NoGate{});
+                k_pass<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, q5, c->ut, c->ut, kap, cz, 2, NoGate{});
             }
             TRY(launch_check());
         }
     }
     {
         Span sp(c, TC_SRC, 1);
-        k_src_finish<<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->meqn, ax, f);
+        k_src_finish<COL><<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->meqn, ax, f);
         TRY(launch_check());
     }
     return MAGIC_OK;
+}
+
+static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, int *ndtr_out,
+                       int *ndtrc_out)
+{
+    const magic_params &p = c->prm;
+    if (p.omega1 != 0.0) return MAGIC_EUNSUPPORTED;   // time-dependent wind (src3.f90:389-399)
+    const int fm = (int)p.forcemth;
+    if (!(p.forcemth == 0.0 || p.forcemth == 1.0 || p.forcemth == 3.0)) return MAGIC_EUNSUPPORTED;
+    // substep counts: src3.f90:314-322
+    double dmin = c->dx < c->dy ? c->dx : c->dy;
+    dmin = dmin < c->dz ? dmin : c->dz;
+    const double dxyzmin2 = dmin * dmin;
+    const double difmax = (4.0 / 3.0) * c->aux11_top;
+    const int ndtr = (int)std::ceil((dt * difmax / (dxyzmin2)) / 0.45);
+    const double dtr = dt / ndtr;
+    const int ndtrc = (int)std::ceil((dtr * c->aux12_top / (dxyzmin2)) / 0.45);
+    const double dtrc = dtr / ndtrc;
+    *ndtr_out = ndtr;
+    *ndtrc_out = ndtrc;
+    int mthbc[6];
+    memcpy(mthbc, mthbc_in, sizeof(mthbc));
+    mthbc[4] = -1;
+    mthbc[5] = -1;
+    Forcing f;
+    f.forcemth = fm;
+    f.t = t; f.dt = dt;
+    f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
+    f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+    f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
+    f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
+    f.tcenter = p.tcenter; f.twidth = p.twidth;
+    f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
+    f.tgauss = std::exp(-.5 * ((t - p.tcenter) * (t - p.tcenter) / (p.twidth * p.twidth)));
+    if (c->form == MAGIC_FORM_FP32P) return src3_passes32(c, mthbc, ndtr, dtr, ndtrc, dtrc, f);
+    return c->aux_layered ? src3_passes64<true>(c, mthbc, ndtr, dtr, ndtrc, dtrc, f)
+                          : src3_passes64<false>(c, mthbc, ndtr, dtr, ndtrc, dtrc, f);
 }
 
 extern "C" int magic_src3(magic_ctx *c, double t, double dt, const int mthbc[6], int *ndtr, int *ndtrc)

@@ -55,6 +55,11 @@ struct magic_ctx {
     float *col32 = nullptr;
     double *bg64 = nullptr;
     Zr32 zr32;
+    // FP64 form: when every aux slot the source pass reads is constant on
+    // each layer (bitwise), those slots also live as columns (index k+mbc-1,
+    // AX_ order) and the source pass reads the columns instead of the planes
+    double *col64 = nullptr;
+    bool aux_layered = false;
     bool own_stream = true;   // false when a batch lends its stream
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)

@@ -105,6 +105,15 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
     if (i > d.mx || j > d.my) return;                                         \
     const long long c = cid(d, i, j, k);
 
+// All-cells thread mapping over the padded grid: x fastest; kk is the
+// padded layer index (k + mbc - 1).
+#define PADDED_IJK                                                            \
+    const int ip = (int)(blockIdx.x * blockDim.x + threadIdx.x);              \
+    const int jp = (int)(blockIdx.y * blockDim.y + threadIdx.y);              \
+    if (ip >= d.NX || jp >= d.NY) return;                                     \
+    const int kk = (int)blockIdx.z;                                           \
+    const long long c = ((long long)kk * d.NY + jp) * d.NX + ip;
+
 // diffuse.f:70-118: cross-derivative terms of q(2:4) (after the z-ghost
 // fill), written to the interior of the three planes of u; q is not
 // changed (its planes keep the values the reference's copy holds, whose

@@ -97,3 +97,45 @@ def test_claw3_steps(case, oracle):
     d.close()
     assert abs(t_g - t_o) <= 1e-12 * t_o
     assert rel_err(qg, qo, interior(g)) <= 1e-9
+
+
+def test_src3_background_varying_within_layers(oracle):
+    """The FP64 source pass reads the background from planes when it varies
+    within a layer (here nu, kappa and the background x momentum vary from
+    cell to cell) and from columns when it is constant on each layer."""
+    (mx, my, mz), mthbc = CASES[0]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=5, visc=3e10)
+    p.tcenter = 0.0
+    rng = np.random.default_rng(9)
+    aux = aux.copy()
+    aux[..., 10] *= 1 + 0.2 * rng.random(aux.shape[:3])     # aux(11): nu
+    aux[..., 11] *= 1 + 0.2 * rng.random(aux.shape[:3])     # aux(12): kappa
+    aux[..., 1] = 0.01 * aux[..., 0] * rng.standard_normal(aux.shape[:3])   # aux(2)
+    print("[src3 planes]", flush=True)
+    qo = q.copy()
+    st, ndtr_o, ndtrc_o = oracle.src3(g, p, qo, aux, 0.0, 5.0, mthbc)
+    assert st == 0
+    d = _dev(g, p, aux, q)
+    ndtr, ndtrc = d.src3(0.0, 5.0, mthbc)
+    qg = d.get_q()
+    d.close()
+    assert (ndtr, ndtrc) == (ndtr_o, ndtrc_o)
+    assert rel_err(qg, qo, interior(g)) <= 1e-12
+
+
+def test_src3_columns_equal_planes():
+    """A layered background takes the column path; changing aux(17) (which
+    the source pass does not read) in one ghost cell sends the same problem
+    down the plane path: the results must agree bit for bit."""
+    (mx, my, mz), mthbc = CASES[1]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=6, visc=3e10)
+    p.tcenter = 0.0
+    aux2 = aux.copy()
+    aux2[0, 0, 0, 16] *= 1.5
+    out = []
+    for a in (aux, aux2):
+        d = _dev(g, p, a, q)
+        d.src3(0.0, 5.0, mthbc)
+        out.append(d.get_q())
+        d.close()
+    np.testing.assert_array_equal(out[0], out[1])

@@ -15,6 +15,31 @@ struct RoeT {
 };
 using Roe = RoeT<double>;
 
+// The well-balanced gravity jump (rpn3euX.f:194-208): across a hydrostatic
+// column p_b - p_a - .5 (rho_a + rho_b) aux17 cancels to rounding, so a
+// single fused multiply-add in it, or in the pressures it is made of, moves
+// the jump by a unit of the pressure.  These pieces are rounded as the
+// reference rounds them (no contraction) whatever the surrounding code's
+// contraction setting.
+template <typename T>
+__device__ __forceinline__ T pressure_nc(T gam, T E, T m1, T m2, T m3, T irho)
+{
+#pragma clang fp contract(off)
+    return (gam - T(1.0)) * (E - T(0.5) * (m1 * m1 + m2 * m2 + m3 * m3) * irho);
+}
+template <typename T>
+__device__ __forceinline__ T flux_jump_nc(T mb, T ub, T pb, T ma, T ua, T pa)
+{
+#pragma clang fp contract(off)
+    return (mb * ub + pb) - (ma * ua + pa);
+}
+template <typename T>
+__device__ __forceinline__ T gravity_jump_nc(T d, T ra, T rb, T vg)
+{
+#pragma clang fp contract(off)
+    return d - T(0.5) * (rb + ra) * vg;
+}
+
 // flang's MIN/MAX: a < b ? a : b, a > b ? a : b
 template <typename T>
 __device__ __forceinline__ T fminF(T a, T b) { return a < b ? a : b; }

@@ -1,6 +1,9 @@
-// FP64 instantiations of the tiled sweep (built without contraction of
-// products and sums into fused multiply-adds, as the reference rounds), for
-// the tile shape MAGIC_TILE_SHAPE: this file is compiled once per shape.
+// FP64 instantiations of the tiled sweep, for the tile shape
+// MAGIC_TILE_SHAPE: this file is compiled once per shape.  Products and sums
+// may contract into fused multiply-adds here, except in the pressures and
+// the gravity jump of the Riemann problem (pressure_nc, flux_jump_nc,
+// gravity_jump_nc), which are rounded as the reference rounds them.
+#pragma clang fp contract(fast)
 #include "sweep_tile_body.h"
 
 #ifndef MAGIC_TILE_SHAPE

@@ -226,8 +226,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             const T iqa = T(1.0) / qa[0], iqb = T(1.0) / qb[0];
             const T gamma = (rhsqrtr * gb + rhsqrtl * ga) * i2;
             const T gamma1 = gamma - T(1.0);
-            const T pl = (ga - T(1.0)) * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
-            const T pr = (gb - T(1.0)) * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
+            const T pl = pressure_nc(ga, qa[4], qa[1], qa[2], qa[3], iqa);
+            const T pr = pressure_nc(gb, qb[4], qb[1], qb[2], qb[3], iqb);
             R.u = (qa[1] * il + qb[1] * ir) * i2;
             R.v = (qa[2] * il + qb[2] * ir) * i2;
             R.w = (qa[3] * il + qb[3] * ir) * i2;
@@ -243,7 +243,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             const T d1 = qb[1] - qa[1];
             T d2;
             if constexpr (PERT) d2 = (qb[1] * ur + ppb) - (qa[1] * ul + ppa);   // background part below
-            else d2 = (qb[1] * ur + pr) - (qa[1] * ul + pl);
+            else d2 = flux_jump_nc(qb[1], ur, pr, qa[1], ul, pl);
             const T d3 = ur * qb[2] - ul * qa[2];
             const T d4 = ur * qb[3] - ul * qa[3];
             T d5 = (qb[4] + pr) * ur - (qa[4] + pl) * ul;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 } else {
                     const T vg = vgc;
-                    d2 = d2 - T(0.5) * (qb[0] + qa[0]) * vg;
+                    d2 = gravity_jump_nc(d2, qa[0], qb[0], vg);
                     d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
                 }
             }

@@ -40,6 +40,20 @@ __device__ __forceinline__ T gravity_jump_nc(T d, T ra, T rb, T vg)
     return d - T(0.5) * (rb + ra) * vg;
 }
 
+// Reciprocal and square root of the Riemann solver's densities, sound
+// speeds and wave speeds: correctly rounded in FP64 (as the reference); in
+// the FP32 form, whose error bar is a fraction of the field peak, the
+// hardware's single-instruction approximations (about one unit in the last
+// place).  Never used for a ratio of two quantities that can both be tiny.
+template <typename T>
+__device__ __forceinline__ T rcpT(T x) { return T(1.0) / x; }
+template <>
+__device__ __forceinline__ float rcpT<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename T>
+__device__ __forceinline__ T sqrtT(T x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float sqrtT<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // flang's MIN/MAX: a < b ? a : b, a > b ? a : b
 template <typename T>
 __device__ __forceinline__ T fminF(T a, T b) { return a < b ? a : b; }

@@ -217,13 +217,13 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 ppb = (gb - T(1.0)) * (epb - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
             }
             if (!(qa[0] > T(0.0)) || !(qb[0] > T(0.0))) report_t(A, 1, I, pg, ph);
-            const T rhsqrtl = sqrt(qa[0]), rhsqrtr = sqrt(qb[0]);
+            const T rhsqrtl = sqrtT(qa[0]), rhsqrtr = sqrtT(qb[0]);
             const T rhsq2 = rhsqrtl + rhsqrtr;
             // reciprocals taken once where rpn3euX.f:159-216 divides by the
             // same value several times (the products differ from the
             // quotients by at most one rounding)
-            const T il = T(1.0) / rhsqrtl, ir = T(1.0) / rhsqrtr, i2 = T(1.0) / rhsq2;
-            const T iqa = T(1.0) / qa[0], iqb = T(1.0) / qb[0];
+            const T il = rcpT(rhsqrtl), ir = rcpT(rhsqrtr), i2 = rcpT(rhsq2);
+            const T iqa = rcpT(qa[0]), iqb = rcpT(qb[0]);
             const T gamma = (rhsqrtr * gb + rhsqrtl * ga) * i2;
             const T gamma1 = gamma - T(1.0);
             const T pl = pressure_nc(ga, qa[4], qa[1], qa[2], qa[3], iqa);
@@ -235,8 +235,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             R.u2v2w2 = R.u * R.u + R.v * R.v + R.w * R.w;
             const T a2 = gamma1 * (R.enth - T(0.5) * R.u2v2w2);
             if (!(a2 > T(0.0))) report_t(A, 2, I, pg, ph);
-            R.a = sqrt(a2);
-            R.i2a = T(1.0) / (T(2.0) * R.a);
+            R.a = sqrtT(a2);
+            R.i2a = rcpT(T(2.0) * R.a);
             R.g1a2 = gamma1 / a2;
             R.euv = R.enth - R.u2v2w2;
             const T ur = qb[1] * iqb, ul = qa[1] * iqa;
@@ -281,18 +281,18 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             {
                 const T gmm = ga, gm1 = ga - T(1.0);
                 const T pim1 = gm1 * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
-                const T cim1 = sqrt(gmm * pim1 * iqa);
+                const T cim1 = sqrtT(gmm * pim1 * iqa);
                 const T s0 = qa[1] * iqa - cim1;
                 if (!(s0 >= T(0.0) && s1 > T(0.0))) {
-                    const T is1 = T(1.0) / s1;
+                    const T is1 = rcpT(s1);
                     const T rho1 = qa[0] + fw1[0] * is1;
                     const T rhou1 = qa[1] + fw1[1] * is1;
                     const T rhov1 = qa[2] + fw1[2] * is1;
                     const T rhow1 = qa[3] + fw1[3] * is1;
                     const T en1 = qa[4] + fw1[4] * is1;
-                    const T ir1 = T(1.0) / rho1;
+                    const T ir1 = rcpT(rho1);
                     const T p1 = gm1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) * ir1);
-                    const T cc1 = sqrt(gmm * p1 * ir1);
+                    const T cc1 = sqrtT(gmm * p1 * ir1);
                     const T sa = rhou1 * ir1 - cc1;
                     T sfract;
                     if (s0 < T(0.0) && sa > T(0.0)) sfract = s0 * (sa - s1) / (sa - s0);
@@ -305,17 +305,17 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                         for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + fw2[m];
                         const T gm = gb, g1 = gb - T(1.0);
                         const T pi_ = g1 * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
-                        const T ci = sqrt(gm * pi_ * iqb);
+                        const T ci = sqrtT(gm * pi_ * iqb);
                         const T s3r = qb[1] * iqb + ci;
-                        const T is3 = T(1.0) / s3;
+                        const T is3 = rcpT(s3);
                         const T rho2 = qb[0] - fw3[0] * is3;
                         const T rhou2 = qb[1] - fw3[1] * is3;
                         const T rhov2 = qb[2] - fw3[2] * is3;
                         const T rhow2 = qb[3] - fw3[3] * is3;
                         const T en2 = qb[4] - fw3[4] * is3;
-                        const T ir2 = T(1.0) / rho2;
+                        const T ir2 = rcpT(rho2);
                         const T p2 = g1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) * ir2);
-                        const T cc2 = sqrt(gm * p2 * ir2);
+                        const T cc2 = sqrtT(gm * p2 * ir2);
                         const T sb = rhou2 * ir2 + cc2;
                         bool add = true;
                         if (sb < T(0.0) && s3r > T(0.0)) sfract = sb * (s3r - s3) / (s3r - sb);

@@ -60,8 +60,14 @@ __device__ __forceinline__ T fminF(T a, T b) { return a < b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T fmaxF(T a, T b) { return a > b ? a : b; }
 
+// x**y for 0 <= x < 1, y > 0 (suppow's rfrac**pp): the library pow in FP64;
+// in the FP32 form 2**(y log2 x) from the hardware's exp2 and log2
+// (log2 0 = -inf gives 0, as pow does).
 __device__ __forceinline__ double powT(double x, double y) { return pow(x, y); }
-__device__ __forceinline__ float powT(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ float powT(float x, float y)
+{
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 
 // rpn3euX.f:686-705
 template <typename T>
@@ -69,10 +75,12 @@ __device__ __forceinline__ T cflsupbee(T theta, T beta, T r, T cfl, T cflup)
 {
     T cfle = fmaxF(T(0.001), fminF(T(0.999), cfl));
     T cflupe = fmaxF(T(0.001), fminF(T(0.999), cflup));
-    T weight = (T(1.0) - cflupe) / (T(1.0) - cfle);
-    T s1 = theta * weight * T(2.0) / cfle;
-    T s2 = (T(1.0) + cfle) / T(3.0);
-    T phimax = theta * T(2.0) / (T(1.0) - cfle);
+    // cfle is clamped to [0.001, 0.999]: its reciprocals are taken once
+    const T i1mc = rcpT(T(1.0) - cfle), icfle = rcpT(cfle);
+    T weight = (T(1.0) - cflupe) * i1mc;
+    T s1 = theta * weight * T(2.0) * icfle;
+    T s2 = (T(1.0) + cfle) * (T(1.0) / T(3.0));
+    T phimax = theta * T(2.0) * i1mc;
     T ultra = fmaxF(T(0.0), fminF(s1 * r, phimax));
     T c1 = T(1.0) + (s2 - beta / T(2.0)) * (r - T(1.0));
     T c2 = T(1.0) + (s2 + beta / T(2.0)) * (r - T(1.0));
@@ -91,12 +99,13 @@ __device__ __forceinline__ T suppow(T r, T cfl, T cflup)
     }
     T cfle = fmaxF(T(0.001), fminF(T(0.999), cfl));
     T cflupe = fmaxF(T(0.001), fminF(T(0.999), cflup));
-    T s2 = (T(1.0) + cfle) / T(3.0);
+    const T i1mc = rcpT(T(1.0) - cfle), icfle = rcpT(cfle);
+    T s2 = (T(1.0) + cfle) * (T(1.0) / T(3.0));
     T s3 = T(1.0) - s2;
-    T weight = (T(1.0) - cflupe) / (T(1.0) - cfle);
-    T pp = (r <= T(1.0)) ? weight * (T(2.0) / cfle) * T(2.0) * s3 : fabs(T(2.0) / (T(1.0) - cfle)) * T(2.0) * s2;
+    T weight = (T(1.0) - cflupe) * i1mc;
+    T pp = (r <= T(1.0)) ? weight * (T(2.0) * icfle) * T(2.0) * s3 : fabs(T(2.0) * i1mc) * T(2.0) * s2;
     T rabs = fabs(r);
-    T rfrac = fabs((T(1.0) - rabs) / (T(1.0) + rabs));
+    T rfrac = fabs((T(1.0) - rabs) * rcpT(T(1.0) + rabs));
     T signum = fmaxF(T(0.0), copysign(T(1.0), r));
     return signum * (s3 + s2 * r) * (T(1.0) - powT(rfrac, pp));
 }

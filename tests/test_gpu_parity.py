@@ -139,3 +139,21 @@ def test_src3_columns_equal_planes():
         out.append(d.get_q())
         d.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_step3_pencil_kernel(case, oracle, monkeypatch):
+    """The second sweep implementation (one workgroup per pencil, selected
+    with MAGIC_SWEEP=pencil) against the oracle."""
+    monkeypatch.setenv("MAGIC_SWEEP", "pencil")
+    (mx, my, mz), mthbc = CASES[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=case + 20)
+    print(f"[step3 pencil] case {case}", flush=True)
+    qo, cfl_o, st = oracle.step3(g, q, aux, 2.0, p.nspecadv)
+    assert st == 0
+    d = _dev(g, p, aux, q)
+    cfl_g = d.step3(2.0)
+    qg = d.get_q()
+    d.close()
+    assert abs(cfl_g - cfl_o) <= 1e-12 * cfl_o
+    assert rel_err(qg, qo) <= 1e-12

@@ -159,7 +159,10 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         const int I = S + c;                    // Fortran interface index
         const bool act1 = pen && I <= n + 2;
         const bool act2 = act1 && c >= 1 && c <= CH - 2 && I >= 1 && I <= n + 1;
-        __syncthreads();   // previous chunk's readers of s_* and T are done
+        // No barrier here: the previous chunk's last readers of the s_*
+        // arrays (part 2 and the CFLup carry) are behind its sub-step
+        // barriers, and its flush's reads of the tile are behind this
+        // chunk's barrier after part 1.
 
         // ---------------- part 1: Roe solve of interface I (rpn3euX.f:138-421)
         RoeT<T> R = {0, 0, 0, 0, 1, 0, 0, 0, 0};
@@ -511,6 +514,15 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                     }
                 }
                 __syncthreads();
+                if (phase == 0 && da == 0 && t < TA) {
+                    // CFLup carried into the next chunk: the last assignment
+                    // among this chunk's interfaces (every lane is past part
+                    // 2, which reads s_cprev, once this barrier is passed)
+                    for (int j = CH - 2; j >= 1; j--) {
+                        const int tt = PM ? t * CH + j : j * TA + t;
+                        if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
+                    }
+                }
             }
         }
 
@@ -582,13 +594,6 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 #pragma unroll
                     for (int k = 0; k < PPT; k++) *(pok[k] ? base + poff[k] : sink) = val[m][db][k];
                 }
-            }
-        }
-        // CFLup carried into the next chunk: last assignment among this chunk's interfaces
-        if (t < TA) {
-            for (int j = CH - 2; j >= 1; j--) {
-                const int tt = PM ? t * CH + j : j * TA + t;
-                if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
             }
         }
     }

@@ -296,9 +296,12 @@ static int launch_src(magic_batch *b, int s)
                 const int S = c->mz - c->mx;
                 k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->cs>>>(d, c->ut32, S, gk);
             }
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
-                d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
-            k_copy_interior<<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, 1, gate(b, s, GATE_COND_MORE, nn, kk));
+            if (kk < h.ndtrc_cap) {   // at kk = ndtrc_cap >= ndtrc these never run
+                k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
+                    d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
+                k_copy_interior<<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, 1,
+                                                                  gate(b, s, GATE_COND_MORE, nn, kk));
+            }
             k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
                 d, q5, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_LAST, nn, kk, FAC_COND + 2));
             TRY(launch_check());

@@ -143,12 +143,14 @@ static inline int ntracer(const magic_ctx *c) { return c->meqn - 5; }
 static inline dim3 cell_block() { return dim3(64, 4, 1); }
 static inline dim3 interior_grid(const Dims &d) { return dim3(blocks(d.mx, 64), blocks(d.my, 4), d.mz); }
 static inline dim3 padded_cells(const Dims &d) { return dim3(blocks(d.NX, 64), blocks(d.NY, 4), d.NZ); }
-// k_diff_xyz: layers per workgroup so that a launch has about 1024
-// workgroups (at least 4 layers each), and its grid
+// k_diff_xyz: layers per workgroup so that a launch has about 8192
+// workgroups (at least 4 layers each; each workgroup recomputes the x and y
+// passes of two extra layers, and its layers run one after the other, so
+// many short workgroups hide the latency better), and its grid
 static inline int diff_kb(const Dims &d)
 {
     const long long cols = (long long)blocks(d.mx, 64) * blocks(d.my, 4);
-    const long long chunks = (1024 + cols - 1) / cols;
+    const long long chunks = (8192 + cols - 1) / cols;
     const int kb = (int)((d.mz + chunks - 1) / chunks);
     return kb < 4 ? 4 : kb;
 }

@@ -99,7 +99,9 @@ def build_fortran() -> str:
     objs = []
     for src in ("magic_hip.f90", "claw3_hip.f90", "claw3ez_hip.f90", "out3_h5.f90"):
         obj = os.path.join(mod, src.replace(".f90", ".o"))
-        _run([FC, "-c", "-O2", "-fPIC", "-module-dir", mod, "-I", mod,
+        # no contraction: the host's background arithmetic rounds as the
+        # reference's setaux.f does
+        _run([FC, "-c", "-O2", "-fPIC", "-ffp-contract=off", "-module-dir", mod, "-I", mod,
               os.path.join(FORTRAN, src), "-o", obj])
         objs.append(obj)
     exe = os.path.join(BIN, "claw3ez_hip")

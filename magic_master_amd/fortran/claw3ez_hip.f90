@@ -10,6 +10,8 @@
 ! Backgrounds (setprob loadprof):
 !   0  isothermal, rho = dens*exp(-z*gamma*grav/speed**2), R = 287
 !      (the formulas of setaux.f:55-87)
+!   1  the vertical profile of profile.data in the working directory, as
+!      setaux.f:38-212 builds it (no analytic winds: amp0 = amp1 = 0)
 !   2  the standard temperature column integrated hydrostatically
 !      (same as magic_master_amd/problem.py:setaux_column)
 ! Frames are written by out3 as the reference's HDF5 files
@@ -165,6 +167,10 @@ contains
     double precision :: nu(1-mbc:mz+mbc), kap(1-mbc:mz+mbc), p(1-mbc:mz+mbc), tt(1-mbc:mz+mbc)
     double precision :: rho1, z, pl, pr, aa, dz0
     a = 0.d0
+    if (loadprof == 1) then
+       call profile_background(a)
+       return
+    end if
     if (loadprof == 0) then
        rho1 = dens * dexp(-(zlower + 0.5d0 * dz) * gamma * grav / (speed**2))
        do k = 1-mbc, mz+mbc
@@ -197,7 +203,7 @@ contains
           kap(k) = gamma * 1.8d-5 / (0.7d0 * rho(k))
        end do
     else
-       write(6, *) 'claw3ez_hip: loadprof must be 0 (isothermal) or 2 (standard column)'
+       write(6, *) 'claw3ez_hip: loadprof must be 0 (isothermal), 1 (profile.data) or 2 (standard column)'
        stop 1
     end if
     ! virtual gravity (setaux.f:176-187)
@@ -222,6 +228,81 @@ contains
        end do
     end do
   end subroutine background
+
+  ! loadprof = 1 (setaux.f:38-212 without analytic winds): for each cell
+  ! centre height the profile.data row with exactly that height in km (rows
+  ! read in order; a missing height stops the run), columns height, O, N2,
+  ! O2 number densities (CGS), density, temperature, H and electron number
+  ! densities, x and y wind.  Gas constant and gamma from the composition,
+  ! density from the number densities, energy from the temperature,
+  ! composition ratios (slots 6-10), molecular and thermal diffusivities
+  ! (slots 11, 12; layers above the 400th take the 400th's) and the virtual
+  ! gravity from the pressure jumps (slot 17; 0 on the lowest ghost layer).
+  ! The literals keep the reference's kinds (3.43, 0.69, ... are default
+  ! REAL), so the values are the reference's bit for bit.
+  subroutine profile_background(a)
+    double precision, intent(inout) :: a(maux, 1-mbc:mx+mbc, 1-mbc:my+mbc, 1-mbc:mz+mbc)
+    double precision :: col(19, 1-mbc:mz+mbc)
+    double precision :: height, dox, dnit2, dox2, rhof, temp, dhyd, dne, windx, windy
+    double precision :: zc, rho, gm1, cnv, cp, dmolec, dtherm, pl, pr
+    integer :: k, ios
+    col = 0.d0
+    cnv = 1.d3                  ! profile.data in CGS (setaux.f:20-21, 111-116)
+    open(7, file='profile.data', status='old', form='formatted')
+    do k = 1-mbc, mz+mbc
+       zc = zlower + (k - 0.5d0) * dz
+       do
+          read(7, *, iostat=ios) height, dox, dnit2, dox2, rhof, temp, dhyd, dne, windx, windy
+          if (ios /= 0) then
+             write(6, *) 'claw3ez_hip: profile.data has no row at height', zc / 1.d3, 'km'
+             stop 1
+          end if
+          if (height >= zc / 1.d3) exit
+       end do
+       if (height > zc / 1.d3) then
+          write(6, *) 'claw3ez_hip: profile.data height', height, '.GT.', zc / 1.d3
+          stop 1
+       end if
+       col(18, k) = 8.31d3 / ((dox*16.d0 + dnit2*28.d0 + dox2*32.d0) / (dox + dnit2 + dox2))
+       col(19, k) = (7*(dox2 + dnit2) + 5*dox) / (5*(dox2 + dnit2) + 3*dox)
+       gm1 = col(19, k) - 1.d0
+       rho = (dox*16.d0 + dox2*32.d0 + dnit2*28.d0) * cnv
+       rho = rho / (6.022d23)
+       col(1, k) = rho
+       col(2, k) = rho * windx
+       col(3, k) = rho * windy
+       col(4, k) = 0.d0
+       col(5, k) = rho * temp * col(18, k) / gm1 + .5*(col(2, k)**2) / rho + .5*(col(3, k)**2) / rho
+       col(6, k) = dox*16.d0*cnv / (6.022d23*rho)
+       col(7, k) = dox2*32.d0*cnv / (6.022d23*rho)
+       col(8, k) = dne / (dox + dox2 + dnit2)
+       col(9, k) = dhyd / (dox + dox2 + dnit2)
+       col(10, k) = 0.d0
+       cp = 1/(1 - 1/col(19, k)) * col(18, k)
+       dmolec = (3.43*dnit2 + 4.03*dox2 + 3.90*dox) * 1d-7
+       dmolec = dmolec * (temp**(0.69)) / (dox + dnit2 + dox2)
+       dtherm = (56.0*(dnit2 + dox2) + 75.9*dox) * 1d-5
+       dtherm = dtherm * (temp**(0.69)) / (dox + dnit2 + dox2)
+       col(11, k) = dmolec * (1/rho)
+       col(12, k) = dtherm * (1/(rho*cp)) * col(19, k)
+       if (k > 400) col(11:12, k) = col(11:12, 400)
+       if (k > 1-mbc) then
+          pl = (col(19, k-1) - 1.d0) * (col(5, k-1) - 0.5d0*(col(2, k-1)**2 + col(3, k-1)**2 &
+               + col(4, k-1)**2) / col(1, k-1))
+          pr = (col(19, k) - 1.d0) * (col(5, k) - 0.5d0*(col(2, k)**2 + col(3, k)**2 &
+               + col(4, k)**2) / col(1, k))
+          col(17, k) = (pr - pl) / (.5d0*(col(1, k) + col(1, k-1)))
+       end if
+    end do
+    close(7)
+    do k = 1-mbc, mz+mbc
+       do j = 1-mbc, my+mbc
+          do i = 1-mbc, mx+mbc
+             a(1:19, i, j, k) = col(:, k)
+          end do
+       end do
+    end do
+  end subroutine profile_background
 
   subroutine write_frame(iframe, t)
     integer, intent(in) :: iframe

@@ -307,6 +307,88 @@ def setaux_isothermal(g: Grid, p: ProbData) -> np.ndarray:
     return _fill_aux_from_column(g, col)
 
 
+def read_profile(path: str) -> np.ndarray:
+    """profile.data (setaux.f:90-93): one row per height, columns height (km),
+    O, N2, O2 number densities, density, temperature (K), H and electron
+    number densities, x and y wind (m/s); shape (rows, 10)."""
+    rows = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    if rows.shape[1] < 10:
+        raise ValueError(f"{path}: {rows.shape[1]} columns, profile.data has 10")
+    return rows[:, :10]
+
+
+# Fortran REAL (single precision) literals of setaux.f's diffusivity and
+# kinetic-energy formulas, promoted to double as the reference promotes them
+_F4 = lambda v: float(np.float32(v))
+
+
+def setaux_profile(g: Grid, p: ProbData, prof: np.ndarray, cgs: bool = True) -> np.ndarray:
+    """setaux.f:38-212 for loadprof = 1 (the MSIS-style vertical profile from
+    profile.data) without analytic winds (amp0 = amp1 = 0, the winds come
+    from the profile): per cell centre height the row with exactly that
+    height (in km, rows read in order as the reference reads them; a missing
+    height is an error, setaux.f:94-103), then gas constant and gamma from
+    the O/N2/O2 composition, density from the number densities (cgs = True:
+    the file is in CGS, setaux.f:20-21), energy from the temperature,
+    composition ratios in slots 6-10, molecular and thermal diffusivities
+    (slots 11, 12; held at the 400th layer above it) and the virtual gravity
+    of slot 17 from the pressure jumps (setaux.f:176-187; the lowest ghost
+    layer has none, it is 0 here).  Same operations in the same order as the
+    reference, including its single-precision literals."""
+    if p.amp0 != 0.0 or p.amp1 != 0.0:
+        raise NotImplementedError("analytic background winds (amp0/amp1) are not supported")
+    prof = np.asarray(prof, dtype=np.float64)
+    nzp = g.mz + 2 * g.mbc
+    cnv = 1.0e3 if cgs else 1.0e-3
+    col = np.zeros((nzp, g.maux))
+    pos = 0
+    c343, c403, c390, c759, c069 = _F4(3.43), _F4(4.03), _F4(3.90), _F4(75.9), _F4(0.69)
+    for kk in range(nzp):
+        k = kk + 1 - g.mbc
+        target = g.zcell(k) / 1.0e3
+        while pos < len(prof) and prof[pos, 0] < target:
+            pos += 1
+        if pos == len(prof) or prof[pos, 0] > target:
+            raise ValueError(f"profile.data has no row at height {target} km (cell centres must match exactly)")
+        height, dox, dnit2, dox2, _, temp, dhyd, dne, windx, windy = (float(v) for v in prof[pos])
+        pos += 1
+        a = col[kk]
+        a[17] = 8.31e3 / ((dox * 16.0 + dnit2 * 28.0 + dox2 * 32.0) / (dox + dnit2 + dox2))
+        a[18] = (7.0 * (dox2 + dnit2) + 5.0 * dox) / (5.0 * (dox2 + dnit2) + 3.0 * dox)
+        roverm, gamma = a[17], a[18]
+        gamma1 = gamma - 1.0
+        rho = (dox * 16.0 + dox2 * 32.0 + dnit2 * 28.0) * cnv
+        rho = rho / 6.022e23
+        a[0] = rho
+        a[1] = rho * windx
+        a[2] = rho * windy
+        a[3] = 0.0
+        a[4] = rho * temp * roverm / gamma1 + 0.5 * (a[1] ** 2) / rho + 0.5 * (a[2] ** 2) / rho
+        a[5] = dox * 16.0 * cnv / (6.022e23 * rho)
+        a[6] = dox2 * 32.0 * cnv / (6.022e23 * rho)
+        a[7] = dne / (dox + dox2 + dnit2)
+        a[8] = dhyd / (dox + dox2 + dnit2)
+        a[9] = 0.0
+        cp = 1.0 / (1.0 - 1.0 / gamma) * roverm
+        dmolec = (c343 * dnit2 + c403 * dox2 + c390 * dox) * 1e-7
+        dmolec = dmolec * (temp ** c069) / (dox + dnit2 + dox2)
+        dtherm = (56.0 * (dnit2 + dox2) + c759 * dox) * 1e-5
+        dtherm = dtherm * (temp ** c069) / (dox + dnit2 + dox2)
+        a[10] = dmolec * (1.0 / rho)
+        a[11] = (dtherm * (1.0 / (rho * cp))) * gamma
+        if k > 400:
+            a[10] = col[400 + g.mbc - 1, 10]
+            a[11] = col[400 + g.mbc - 1, 11]
+        if kk > 0:
+            b = col[kk - 1]
+            auxpl = (b[18] - 1.0) * (b[4] - 0.5 * (b[1] ** 2 + b[2] ** 2 + b[3] ** 2) / b[0])
+            auxpr = (a[18] - 1.0) * (a[4] - 0.5 * (a[1] ** 2 + a[2] ** 2 + a[3] ** 2) / a[0])
+            a[16] = (auxpr - auxpl) / (0.5 * (a[0] + b[0]))
+    aux = np.empty(g.shape_aux)
+    aux[...] = col[:, None, None, :]
+    return aux
+
+
 def temperature_standard(z: float, ts: float = 288.0, lapse: float = 6.5) -> float:
     """Analytic temperature column (K) of the full-domain workload:
     ts (288 K) at the surface, -lapse (6.5 K/km) to 11 km, isothermal to

@@ -252,8 +252,9 @@ class Reference:
                        d(t), d(dt), (C.c_int * 6)(*mthbc))
 
     @_big_stack
-    @_big_stack
     def setaux(self, g):
+        """The reference setaux (setaux.f:4-212); with loadprof = 1 it reads
+        profile.data from the working directory."""
         aux = np.zeros(g.shape_aux)
         i, d = self._i, self._d
         self.lib.setaux_(i(g.mbc), i(g.mx), i(g.my), i(g.mz), d(g.xlower), d(g.ylower),

@@ -71,3 +71,39 @@ def test_claw3ez_driver_matches_oracle(tmp_path):
         dtv[0] = dtv[4]
     assert abs(t - to) <= 1e-12 * to
     assert rel_err(qg, qo[interior(g)]) <= 1e-10
+
+
+def test_claw3ez_driver_profile_background(tmp_path):
+    """loadprof = 1: the driver builds the background from profile.data (the
+    reference's own file) as setaux.f does; 10 components as in the
+    reference's deck.  Against the oracle started from problem.setaux_profile
+    (itself equal bit for bit to the compiled reference's setaux)."""
+    import shutil
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks", "profile.data")
+    shutil.copy(src, tmp_path / "profile.data")
+    c, p, g, aux, q = P.config_plain(mx=12, my=10, mz=20, noise=0.0, meqn=10)
+    c.zlower, c.zupper = 0.0, 20e3
+    c.nout = c.nstop = c.nstepout = 3
+    p.loadprof, p.nspecadv = 1, 5
+    g = P.Grid.from_claw(c)
+    aux = P.setaux_profile(g, p, P.read_profile(src))
+    q = P.qinit(g, aux)
+    c.write(tmp_path / "claw3ez.data")
+    p.write(tmp_path / "setprob.data")
+    out = _run([os.path.join(BIN, "claw3ez_hip")], tmp_path)
+    assert "Frame    1" in out
+    from magic_master_amd import h5
+    f = h5.in3(str(tmp_path / h5.frame_name(1)))
+    t = float(f["params"][2])
+    assert f["dims"] == (g.mz, g.my, g.mx, g.meqn)
+    O = COracle()
+    qo = q.copy()
+    dtv = list(c.dtv)
+    to = 0.0
+    for _ in range(3):
+        st, to, dtv, _, _, _ = O.claw3(g, p, qo, aux, to, to - 1.0, dtv, c.cflv, [c.nv[0], 0],
+                                       c.method, c.mthbc)
+        assert st == 0
+        dtv[0] = dtv[4]
+    assert abs(t - to) <= 1e-12 * to
+    assert rel_err(f["data"], qo[interior(g)]) <= 1e-10

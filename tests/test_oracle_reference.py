@@ -85,3 +85,25 @@ def test_tracer_counts_bitwise(meqn, nspec, both):
         dto[0] = dto[4]
         dtr[0] = dtr[4]
     assert np.array_equal(qo, qr)
+
+
+def test_setaux_profile_bitwise(both, tmp_path, monkeypatch):
+    """problem.setaux_profile (the loadprof = 1 background from profile.data)
+    against the reference's setaux on the reference's own profile.data."""
+    import os
+    import shutil
+    from magic_master_amd import problem as P
+    O, R = both
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks", "profile.data")
+    shutil.copy(src, tmp_path / "profile.data")
+    monkeypatch.chdir(tmp_path)
+    p = P.ProbData(loadprof=1, nspecadv=5)
+    R.setup(p, [1, 1, 1, 1, 3, 1])
+    for mz, zlower in ((40, 0.0), (12, 100e3)):
+        g = P.Grid(mx=3, my=2, mz=mz, meqn=10, mbc=2, maux=32, xlower=0.0, ylower=0.0, zlower=zlower,
+                   dx=2e3, dy=2e3, dz=1e3)
+        ref = R.setaux(g)
+        mine = P.setaux_profile(g, p, P.read_profile(src))
+        slots = list(range(12)) + [17, 18]
+        assert np.array_equal(mine[..., slots], ref[..., slots])
+        assert np.array_equal(mine[1:, ..., 16], ref[1:, ..., 16])   # aux(17) above the lowest ghost layer

@@ -5,6 +5,10 @@
 
 namespace magic {
 
+// A compile-time int passed as a value (to generic lambdas).
+template <int V>
+struct IConst { static constexpr int value = V; };
+
 // One directional sweep of step3 (lib/step3.f:121-621): pencils along the
 // normal direction n, with the "g" transverse direction (gadd, flux3fw.f:419)
 // and the "h" transverse direction (hadd, flux3fw.f:545).

@@ -155,6 +155,27 @@ __device__ __forceinline__ void tsplit(const RoeT<T> &R, const T (&in)[NC], T (&
     }
 }
 
+// The split of components M0 .. M0+G-1 alone.  A tracer's split involves
+// only its own component and the contact speed (the v or w of the
+// interface), so a group of tracers (M0 >= 5) is split without the Euler
+// components; M0 = 0 is the split above.
+template <int M0, int G, int ICOOR, bool QUIRK, typename T>
+__device__ __forceinline__ void tsplit_group(const RoeT<T> &R, const T (&in)[G], T (&bm)[G], T (&bp)[G])
+{
+    if constexpr (M0 == 0) {
+        tsplit<G, ICOOR, QUIRK>(R, in, bm, bp);
+    } else {
+        static_assert(M0 >= 5, "a group without the Euler components holds tracers only");
+        const T s = ICOOR == 2 ? R.v : R.w;
+        const T m2 = fminF(s, T(0.0)), p2 = fmaxF(s, T(0.0));
+#pragma unroll
+        for (int m = 0; m < G; m++) {
+            bm[m] = m2 * in[m];
+            bp[m] = p2 * in[m];
+        }
+    }
+}
+
 // Accumulators D[da+1][db+1][m] of the increments a cell receives, for the
 // target cell offset (da, db) in the (g, h) plane (step3.f:189-226).
 template <int NC, typename T = double>

@@ -134,7 +134,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     // the current chunk's transverse terms are computed (registers), so the
     // HBM latency of part 1 is hidden behind part 3.  Not in the FP64
     // pencil-major kernels, where the registers would spill.
-    constexpr bool PREFETCH = !(PM && sizeof(T) == 8);
+    constexpr bool PREFETCH = !(PM && sizeof(T) == 8) && !(sizeof(T) == 8 && NTR > 2);
     T pqa[NC], pqb[NC], pga = T(0.0), pgb = T(0.0), pvg = T(0.0);
     auto fetch = [&](int Iq) {
         if (pen && Iq <= n + 2) {
@@ -443,85 +443,190 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         // each increment to the tile with shared-memory atomics instead, with
         // or without a two-waves-per-SIMD register budget, measured 3-5 %
         // slower in FP64 and 7x slower in FP32 on the full-domain workload.)
-        for (int phase = 0; phase < 2; phase++) {
-            // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
-            const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
-            const int ci = phase == 0 ? c - 1 : c;  // cell index within the chunk's tile
-            auto transverse = [&](auto &acc) {
+        // In FP64 with two or more tracers the components go in groups, the
+        // Euler components and then the tracers, at most three at a time,
+        // with one accumulator alive at a time (at five components it fits
+        // the registers; at ten it spilled a kilobyte per lane, and the
+        // full-domain step with five tracers ran 2.8x slower).  Groups add to
+        // disjoint tile entries, so a group's first sub-step needs no barrier
+        // after the previous group's last one.  FP32 accumulators fit
+        // without groups, and the extra barriers cost 3 % there.
+        if constexpr (NTR <= 1 || sizeof(T) == 4) {
+            for (int phase = 0; phase < 2; phase++) {
+                // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
+                const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
+                const int ci = phase == 0 ? c - 1 : c;  // cell index within the chunk's tile
+                auto transverse = [&](auto &acc) {
+                    const T sgn = phase == 0 ? T(1.0) : -T(1.0);   // +3 cqxx mixing for amdq, -3 for apdq
+                    const T cc = phase == 0 ? -d : d;        // sign of the correction-wave term
+                    T asd[NC];
+#pragma unroll
+                    for (int m = 0; m < NC; m++) asd[m] = phase == 0 ? amdq[m] : apdq[m];
+                    {
+                        // C splits (h direction): 2-D part, correction, then the
+                        // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
+                        T Cm[NC], Cp[NC], Cmq[NC], Cpq[NC];
+                        tsplit<NC, 3, false>(R, cqxx, Cmq, Cpq);
+                        tsplit<NC, 3, false>(R, asd, Cm, Cp);
+                        acc.h(1, 0, -T(0.5) * d * ch, Cm); acc.h(2, 0, -T(0.5) * d * ch, Cp);
+                        acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
+                        axpy3(Cp, Cp, T(3.0) * sgn, Cpq);
+                        axpy3(Cm, Cm, T(3.0) * sgn, Cmq);
+                        tsplit<NC, 2, true>(R, Cp, Cmq, Cpq);      // B-, B+ of c+asdq2
+                        acc.g(2, 0, k6g * cg, Cpq);  acc.g(1, 0, k6g * cg, Cmq);
+                        acc.g(2, 1, -k6g * cg, Cpq); acc.g(1, 1, -k6g * cg, Cmq);
+                        tsplit<NC, 2, true>(R, Cm, Cmq, Cpq);      // B-, B+ of c-asdq2
+                        acc.g(2, 0, -k6g * cg, Cpq); acc.g(1, 0, -k6g * cg, Cmq);
+                        acc.g(2, -1, k6g * cg, Cpq); acc.g(1, -1, k6g * cg, Cmq);
+                    }
+                    {
+                        // B splits (g direction): 2-D part, correction, then the
+                        // h-direction corner terms of b*asdq +- 3 b*cqxx (flux3fw.f:510-538)
+                        T Bm[NC], Bp[NC], Bmq[NC], Bpq[NC];
+                        tsplit<NC, 2, false>(R, cqxx, Bmq, Bpq);
+                        tsplit<NC, 2, false>(R, asd, Bm, Bp);
+                        acc.g(1, 0, -T(0.5) * d * cg, Bm); acc.g(2, 0, -T(0.5) * d * cg, Bp);
+                        acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
+                        axpy3(Bp, Bp, T(3.0) * sgn, Bpq);
+                        axpy3(Bm, Bm, T(3.0) * sgn, Bmq);
+                        tsplit<NC, 3, true>(R, Bp, Bmq, Bpq);      // C-, C+ of b+asdq
+                        acc.h(2, 0, k6h * ch, Bpq);  acc.h(1, 0, k6h * ch, Bmq);
+                        acc.h(2, 1, -k6h * ch, Bpq); acc.h(1, 1, -k6h * ch, Bmq);
+                        tsplit<NC, 3, true>(R, Bm, Bmq, Bpq);      // C-, C+ of b-asdq
+                        acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
+                        acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
+                    }
+#pragma unroll
+                    for (int m = 0; m < NC; m++)
+                        acc.c11(m, phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
+                };
+                Acc<NC, T> acc;
+                acc.zero();
+                if (on) transverse(acc);
+                // one sub-step per offset along a: lanes then add to distinct T entries
+#pragma unroll
+                for (int da = 0; da < 3; da++) {
+                    if (on) {
+                        // the accumulator is indexed [g offset][h offset]; both
+                        // branches keep every index a compile-time constant so
+                        // it stays in registers
+                        if (a_is_g) {
+#pragma unroll
+                            for (int db = 0; db < 3; db++)
+#pragma unroll
+                                for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
+                        } else {
+#pragma unroll
+                            for (int db = 0; db < 3; db++)
+#pragma unroll
+                                for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
+                        }
+                    }
+                    __syncthreads();
+                    if (phase == 0 && da == 0 && t < TA) {
+                        // CFLup carried into the next chunk: the last assignment
+                        // among this chunk's interfaces (every lane is past part
+                        // 2, which reads s_cprev, once this barrier is passed)
+                        for (int j = CH - 2; j >= 1; j--) {
+                            const int tt = PM ? t * CH + j : j * TA + t;
+                            if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int phase = 0; phase < 2; phase++) {
+                // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
+                const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);
+                const int ci = phase == 0 ? c - 1 : c;  // cell index within the chunk's tile
                 const T sgn = phase == 0 ? T(1.0) : -T(1.0);   // +3 cqxx mixing for amdq, -3 for apdq
                 const T cc = phase == 0 ? -d : d;        // sign of the correction-wave term
-                T asd[NC];
+                // components M0 .. M0+GW-1; `last`: a barrier after the third sub-step
+                auto group = [&](auto m0c, auto gwc, bool last) {
+                    constexpr int M0 = decltype(m0c)::value, GW = decltype(gwc)::value;
+                    Acc<GW, T> acc;
+                    acc.zero();
+                    if (on) {
+                        T asd[GW];
 #pragma unroll
-                for (int m = 0; m < NC; m++) asd[m] = phase == 0 ? amdq[m] : apdq[m];
-                {
-                    // C splits (h direction): 2-D part, correction, then the
-                    // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
-                    T Cm[NC], Cp[NC], Cmq[NC], Cpq[NC];
-                    tsplit<NC, 3, false>(R, cqxx, Cmq, Cpq);
-                    tsplit<NC, 3, false>(R, asd, Cm, Cp);
-                    acc.h(1, 0, -T(0.5) * d * ch, Cm); acc.h(2, 0, -T(0.5) * d * ch, Cp);
-                    acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
-                    axpy3(Cp, Cp, T(3.0) * sgn, Cpq);
-                    axpy3(Cm, Cm, T(3.0) * sgn, Cmq);
-                    tsplit<NC, 2, true>(R, Cp, Cmq, Cpq);      // B-, B+ of c+asdq2
-                    acc.g(2, 0, k6g * cg, Cpq);  acc.g(1, 0, k6g * cg, Cmq);
-                    acc.g(2, 1, -k6g * cg, Cpq); acc.g(1, 1, -k6g * cg, Cmq);
-                    tsplit<NC, 2, true>(R, Cm, Cmq, Cpq);      // B-, B+ of c-asdq2
-                    acc.g(2, 0, -k6g * cg, Cpq); acc.g(1, 0, -k6g * cg, Cmq);
-                    acc.g(2, -1, k6g * cg, Cpq); acc.g(1, -1, k6g * cg, Cmq);
-                }
-                {
-                    // B splits (g direction): 2-D part, correction, then the
-                    // h-direction corner terms of b*asdq +- 3 b*cqxx (flux3fw.f:510-538)
-                    T Bm[NC], Bp[NC], Bmq[NC], Bpq[NC];
-                    tsplit<NC, 2, false>(R, cqxx, Bmq, Bpq);
-                    tsplit<NC, 2, false>(R, asd, Bm, Bp);
-                    acc.g(1, 0, -T(0.5) * d * cg, Bm); acc.g(2, 0, -T(0.5) * d * cg, Bp);
-                    acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
-                    axpy3(Bp, Bp, T(3.0) * sgn, Bpq);
-                    axpy3(Bm, Bm, T(3.0) * sgn, Bmq);
-                    tsplit<NC, 3, true>(R, Bp, Bmq, Bpq);      // C-, C+ of b+asdq
-                    acc.h(2, 0, k6h * ch, Bpq);  acc.h(1, 0, k6h * ch, Bmq);
-                    acc.h(2, 1, -k6h * ch, Bpq); acc.h(1, 1, -k6h * ch, Bmq);
-                    tsplit<NC, 3, true>(R, Bm, Bmq, Bpq);      // C-, C+ of b-asdq
-                    acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
-                    acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
-                }
+                        for (int m = 0; m < GW; m++) asd[m] = phase == 0 ? amdq[M0 + m] : apdq[M0 + m];
+                        const T(&cq)[GW] = *reinterpret_cast<const T(*)[GW]>(cqxx + M0);
+                        {
+                            // C splits (h direction): 2-D part, correction, then the
+                            // g-direction corner terms of c*asdq +- 3 c*cqxx (flux3fw.f:388-412)
+                            T Cm[GW], Cp[GW], Cmq[GW], Cpq[GW];
+                            tsplit_group<M0, GW, 3, false>(R, cq, Cmq, Cpq);
+                            tsplit_group<M0, GW, 3, false>(R, asd, Cm, Cp);
+                            acc.h(1, 0, -T(0.5) * d * ch, Cm); acc.h(2, 0, -T(0.5) * d * ch, Cp);
+                            acc.h(1, 0, cc * ch, Cmq);      acc.h(2, 0, cc * ch, Cpq);
+                            axpy3(Cp, Cp, T(3.0) * sgn, Cpq);
+                            axpy3(Cm, Cm, T(3.0) * sgn, Cmq);
+                            tsplit_group<M0, GW, 2, true>(R, Cp, Cmq, Cpq);      // B-, B+ of c+asdq2
+                            acc.g(2, 0, k6g * cg, Cpq);  acc.g(1, 0, k6g * cg, Cmq);
+                            acc.g(2, 1, -k6g * cg, Cpq); acc.g(1, 1, -k6g * cg, Cmq);
+                            tsplit_group<M0, GW, 2, true>(R, Cm, Cmq, Cpq);      // B-, B+ of c-asdq2
+                            acc.g(2, 0, -k6g * cg, Cpq); acc.g(1, 0, -k6g * cg, Cmq);
+                            acc.g(2, -1, k6g * cg, Cpq); acc.g(1, -1, k6g * cg, Cmq);
+                        }
+                        {
+                            // B splits (g direction): 2-D part, correction, then the
+                            // h-direction corner terms of b*asdq +- 3 b*cqxx (flux3fw.f:510-538)
+                            T Bm[GW], Bp[GW], Bmq[GW], Bpq[GW];
+                            tsplit_group<M0, GW, 2, false>(R, cq, Bmq, Bpq);
+                            tsplit_group<M0, GW, 2, false>(R, asd, Bm, Bp);
+                            acc.g(1, 0, -T(0.5) * d * cg, Bm); acc.g(2, 0, -T(0.5) * d * cg, Bp);
+                            acc.g(1, 0, cc * cg, Bmq);      acc.g(2, 0, cc * cg, Bpq);
+                            axpy3(Bp, Bp, T(3.0) * sgn, Bpq);
+                            axpy3(Bm, Bm, T(3.0) * sgn, Bmq);
+                            tsplit_group<M0, GW, 3, true>(R, Bp, Bmq, Bpq);      // C-, C+ of b+asdq
+                            acc.h(2, 0, k6h * ch, Bpq);  acc.h(1, 0, k6h * ch, Bmq);
+                            acc.h(2, 1, -k6h * ch, Bpq); acc.h(1, 1, -k6h * ch, Bmq);
+                            tsplit_group<M0, GW, 3, true>(R, Bm, Bmq, Bpq);      // C-, C+ of b-asdq
+                            acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
+                            acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
+                        }
 #pragma unroll
-                for (int m = 0; m < NC; m++)
-                    acc.c11(m, phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
-            };
-            Acc<NC, T> acc;
-            acc.zero();
-            if (on) transverse(acc);
-            // one sub-step per offset along a: lanes then add to distinct T entries
-#pragma unroll
-            for (int da = 0; da < 3; da++) {
-                if (on) {
-                    // the accumulator is indexed [g offset][h offset]; both
-                    // branches keep every index a compile-time constant so
-                    // it stays in registers
-                    if (a_is_g) {
-#pragma unroll
-                        for (int db = 0; db < 3; db++)
-#pragma unroll
-                            for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[da][db][m];
-                    } else {
-#pragma unroll
-                        for (int db = 0; db < 3; db++)
-#pragma unroll
-                            for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
+                        for (int m = 0; m < GW; m++)
+                            acc.c11(m, phase == 0 ? -d * amdq[M0 + m] - d * cqxx[M0 + m] : -d * apdq[M0 + m] + d * cqxx[M0 + m]);
                     }
-                }
-                __syncthreads();
-                if (phase == 0 && da == 0 && t < TA) {
-                    // CFLup carried into the next chunk: the last assignment
-                    // among this chunk's interfaces (every lane is past part
-                    // 2, which reads s_cprev, once this barrier is passed)
-                    for (int j = CH - 2; j >= 1; j--) {
-                        const int tt = PM ? t * CH + j : j * TA + t;
-                        if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
+                    // one sub-step per offset along a: lanes then add to distinct T entries
+#pragma unroll
+                    for (int da = 0; da < 3; da++) {
+                        if (on) {
+                            // the accumulator is indexed [g offset][h offset]; both
+                            // branches keep every index a compile-time constant so
+                            // it stays in registers
+                            if (a_is_g) {
+#pragma unroll
+                                for (int db = 0; db < 3; db++)
+#pragma unroll
+                                    for (int m = 0; m < GW; m++) LT[tix(M0 + m, db, pa + da, ci)] += acc.D[da][db][m];
+                            } else {
+#pragma unroll
+                                for (int db = 0; db < 3; db++)
+#pragma unroll
+                                    for (int m = 0; m < GW; m++) LT[tix(M0 + m, db, pa + da, ci)] += acc.D[db][da][m];
+                            }
+                        }
+                        if (da < 2 || last) __syncthreads();
+                        if (M0 == 0 && phase == 0 && da == 0 && t < TA) {
+                            // CFLup carried into the next chunk: the last assignment
+                            // among this chunk's interfaces (every lane is past part
+                            // 2, which reads s_cprev, once this barrier is passed)
+                            for (int j = CH - 2; j >= 1; j--) {
+                                const int tt = PM ? t * CH + j : j * TA + t;
+                                if (s_flag[tt]) { s_cprev[t] = s_cfo[tt]; break; }
+                            }
+                        }
                     }
+                };
+                if constexpr (NTR <= 3) {
+                    group(IConst<0>{}, IConst<5>{}, false);
+                    group(IConst<5>{}, IConst<NTR>{}, phase == 1);
+                } else {
+                    group(IConst<0>{}, IConst<5>{}, false);
+                    group(IConst<5>{}, IConst<3>{}, false);
+                    group(IConst<8>{}, IConst<NTR - 3>{}, phase == 1);
                 }
             }
         }

@@ -43,3 +43,85 @@ def form_error(g: Grid, q: np.ndarray, q_ref: np.ndarray, aux: np.ndarray) -> di
                   "max_rel": mx / peak if peak > 0 else float("inf") if mx > 0 else 0.0,
                   "rms_rel": rms / peak if peak > 0 else float("inf") if rms > 0 else 0.0}
     return out
+
+
+def compare_forms(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, every: float,
+                  on_row=None) -> list:
+    """Advance the FP64 and the FP32 perturbation form from the same state
+    under claw3's step control, to every multiple of `every` model seconds
+    up to t_end (each claw3 call stops exactly at the output time, as the
+    claw3ez frame loop does), and compare them there.  Returns one row per
+    output time: {"t", "steps" (per form), field: form_error entry}."""
+    from .hip import FORM_FP32P, FORM_FP64, MagicHIP
+    forms = (FORM_FP64, FORM_FP32P)
+    devs = {}
+    rows = []
+    try:
+        for f in forms:
+            devs[f] = MagicHIP(g, p, aux, q, form=f)
+        t = {f: 0.0 for f in forms}
+        dtv = {f: list(c.dtv) for f in forms}
+        steps = {f: 0 for f in forms}
+        k = 1
+        while k * every <= t_end * (1 + 1e-12):
+            t_out = k * every
+            for f in forms:
+                while t[f] < t_out * (1 - 1e-12):
+                    t[f], dtv[f], _, nv, _ = devs[f].claw3(t[f], t_out, dtv[f], c.cflv, [1 << 30, 0],
+                                                           c.method, c.mthbc)
+                    steps[f] += nv[1]
+                    dtv[f][0] = dtv[f][4]
+            row = {"t": t_out, "steps": [steps[f] for f in forms]}
+            row.update(form_error(g, devs[FORM_FP32P].get_q(), devs[FORM_FP64].get_q(), aux))
+            rows.append(row)
+            if on_row:
+                on_row(row)
+            k += 1
+    finally:
+        for d in devs.values():
+            d.close()
+    return rows
+
+
+def main(argv=None):
+    """Error report of the FP32 perturbation form on the full-domain
+    workload (250x200x300), one run per plume amplitude: max and RMS of
+    rho', T' and w against the FP64 form every `--every` model seconds."""
+    import argparse
+    import copy
+    import json
+    from . import problem as P
+    ap = argparse.ArgumentParser(prog="python -m magic_master_amd.report", description=main.__doc__)
+    ap.add_argument("--amplitudes", default="5,20,60", help="plume peak amplitudes, mm/10 min")
+    ap.add_argument("--t-end", type=float, default=3000.0, help="model seconds")
+    ap.add_argument("--every", type=float, default=60.0, help="output interval, model seconds")
+    ap.add_argument("--tcenter", type=float, default=None,
+                    help="plume peak time (default: the workload's 2400 s)")
+    ap.add_argument("--json", default=None, help="also write every row to this JSON file")
+    a = ap.parse_args(argv)
+    c, p0, g, aux, q = P.config_full()
+    out = {}
+    for amp in [float(v) for v in a.amplitudes.split(",")]:
+        p = copy.copy(p0)
+        p.amplitude = amp
+        if a.tcenter is not None:
+            p.tcenter = a.tcenter
+        print(f"amplitude {amp} mm/10 min, plume peak at t = {p.tcenter} s", flush=True)
+        print("   t(s) steps(64/32) | w: peak(m/s) max/peak rms/peak | T': peak(K) max/peak rms/peak | "
+              "rho': peak(kg/m3) max/peak rms/peak", flush=True)
+
+        def show(r):
+            print(f"{r['t']:7.0f} {r['steps'][0]:5d}/{r['steps'][1]:5d} | " + " | ".join(
+                f"{r[k]['peak']:.3e} {r[k]['max_rel']:.2e} {r[k]['rms_rel']:.2e}" for k in ("w", "T'", "rho'")),
+                flush=True)
+        out[amp] = compare_forms(c, p, g, aux, q, a.t_end, a.every, on_row=show)
+        worst = {k: max(r[k]["max_rel"] for r in out[amp]) for k in ("w", "T'", "rho'")}
+        print("  worst max/peak over the run: " + ", ".join(f"{k} {v:.3e}" for k, v in worst.items()),
+              flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump({str(k): v for k, v in out.items()}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -99,11 +99,13 @@ class _Writer(threading.Thread):
 
 class Server:
     def __init__(self, budget_bytes: float = 250e9, check_every: int = 4, snapshots: str = "hdf5",
-                 outdir: Optional[str] = None, keep_final: bool = False, tile_shapes=None):
+                 outdir: Optional[str] = None, keep_final: bool = False, tile_shapes=None,
+                 progress: Optional[float] = None):
         """snapshots: "hdf5" (frames written under outdir), "download" (copied
         to the host and dropped) or "none".  tile_shapes: the sweeps' tile
         shapes (x, y, z) for the whole run (None: the library's default),
-        fixed so that a request's result does not depend on when it ran."""
+        fixed so that a request's result does not depend on when it ran.
+        progress: print a status line every that many wall-clock seconds."""
         if snapshots not in ("hdf5", "download", "none"):
             raise ValueError(snapshots)
         if snapshots == "hdf5" and not outdir:
@@ -114,6 +116,7 @@ class Server:
         self.outdir = outdir
         self.keep_final = keep_final
         self.tile_shapes = tile_shapes
+        self.progress = progress
 
     def run(self, requests: List[Request]) -> dict:
         from .hip import MagicBatch
@@ -136,9 +139,15 @@ class Server:
         max_active = 0
         max_used = 0.0
         t0 = time.perf_counter()
+        shown = 0.0
         try:
             while pending or waiting or active:
                 now = time.perf_counter() - t0
+                if self.progress and now - shown >= self.progress:
+                    shown = now
+                    done = len(reqs) - len(pending) - len(waiting) - len(active)
+                    print(f"  [{now:7.1f} s] done {done}, running {len(active)}, waiting {len(waiting)}, "
+                          f"not yet arrived {len(pending)}", flush=True)
                 while pending and pending[0].arrival <= now:
                     waiting.append(pending.pop(0))
                 while waiting:
@@ -238,3 +247,39 @@ def poisson_requests(rate: float, window: float, seed: int = 7, t_end=(600.0, 60
     scen = P.ensemble(len(times), seed=seed + 1, nmin=nmin, nmax=nmax)
     return [Request(k, times[k], c, p, g, col, float(rng.uniform(*t_end)), cadence)
             for k, (c, p, g, col) in enumerate(scen)]
+
+
+def main(argv=None):
+    """Serving runs at several arrival rates: Poisson requests drawn like the
+    ensemble workload, one Server run per rate, one summary line per rate
+    (requests, makespan, aggregate cell-updates/s, p50/p99 of time to
+    completion and queue wait, peak concurrency and footprint)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m magic_master_amd.serve", description=main.__doc__)
+    ap.add_argument("--rates", default="0.5,2,8", help="arrival rates, requests per wall-clock second")
+    ap.add_argument("--window", type=float, default=600.0, help="arrival window, wall-clock seconds")
+    ap.add_argument("--t-end", default="600,6000", help="request end time range, model seconds")
+    ap.add_argument("--cadence", type=float, default=60.0, help="snapshot interval, model seconds")
+    ap.add_argument("--budget", type=float, default=250e9, help="admission budget, bytes")
+    ap.add_argument("--snapshots", default="download", choices=("hdf5", "download", "none"))
+    ap.add_argument("--outdir", default=None, help="HDF5 frames go here (--snapshots hdf5)")
+    ap.add_argument("--shapes", default=None, help="tile shapes of the x, y, z sweeps, e.g. 1,0,0")
+    ap.add_argument("--seed", type=int, default=7)
+    a = ap.parse_args(argv)
+    t_end = tuple(float(v) for v in a.t_end.split(","))
+    shapes = [int(v) for v in a.shapes.split(",")] if a.shapes else None
+    for rate in [float(v) for v in a.rates.split(",")]:
+        rq = poisson_requests(rate, a.window, seed=a.seed, t_end=t_end, cadence=a.cadence)
+        print(f"rate {rate} req/s: {len(rq)} requests over {a.window} s", flush=True)
+        res = Server(budget_bytes=a.budget, snapshots=a.snapshots, tile_shapes=shapes, progress=30.0,
+                     outdir=os.path.join(a.outdir, f"rate{rate:g}") if a.outdir else None).run(rq)
+        failed = [o.rid for o in res["outcomes"] if o.status != 0]
+        line = {"rate": rate, "window_s": a.window, "t_end": list(t_end), "failed": failed}
+        line.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items() if k != "outcomes"})
+        line["steps"] = int(sum(o.steps for o in res["outcomes"]))
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -278,7 +278,6 @@ class Reference:
         return info.value, te.value, list(dtv_a), list(cflv_a), list(nv_a)
 
     @_big_stack
-    @_big_stack
     def rpn3(self, ixyz, q1d, a_full, dt, dx):
         """Calls rpn3 (rpn3euX.f:3) directly; a_full is (N, maux) aux pencil."""
         N, meqn = q1d.shape

@@ -156,6 +156,18 @@ int magic_set_tile_shapes(magic_ctx *ctx, const int shapes[3]);
 int magic_get_tile_shapes(magic_ctx *ctx, int shapes[3]);
 int magic_tune(magic_ctx *ctx, double dt, int reps, int shapes_out[3], float *ms_out);
 
+/* ---- Order of the sweeps (no counterpart in the reference) -------------
+ * on = 1 (default): the y-sweep of step3 runs on a second stream next to
+ * the x- and z-sweeps and adds into a scratch copy of the state's shape,
+ * added to q once both are done (the compute units the last workgroups of
+ * one stream's launches leave idle take the other stream's workgroups).
+ * on = 0: the three sweeps add into q in turn on the context's stream, the
+ * order of additions of the batched driver, whose scenarios then equal the
+ * same grids stepped alone bit for bit.  Either way the result is the same
+ * on every run; the two orders differ in the last bits.  The environment
+ * variable MAGIC_SWEEP_OVERLAP=0 turns the overlap off for every context. */
+int magic_set_sweep_overlap(magic_ctx *ctx, int on);
+
 /* ---- Batched driver (FP32 form): many grids advanced together ----------
  * Each scenario is a grid with its own state (a magic_ctx reachable through
  * magic_batch_ctx for parameters, aux and q); its step control is claw3's

@@ -123,6 +123,11 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal); hipFree(c->sink);
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
     hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64);
+    if (c->side) hipStreamSynchronize(c->side);
+    hipFree(c->dq);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->hscal) hipHostFree(c->hscal);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
@@ -482,7 +487,7 @@ static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
 }
 
 template <typename T, bool PERT, int NTR>
-static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl)
+static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s)
 {
     const void *fn = tile_fn<T, PERT, NTR>(sh);
     if (!fn) return MAGIC_EARG;
@@ -496,20 +501,20 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *n
             const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
             if (gx == 0) continue;
             void *kargs[] = {&a, &a_is_g};
-            HIPCHECK(hipLaunchKernel(fn, dim3(gx, gy), dim3(256), kargs, 0, c->stream));
+            HIPCHECK(hipLaunchKernel(fn, dim3(gx, gy), dim3(256), kargs, 0, s));
             ++*nl;
         }
     return MAGIC_OK;
 }
 
 template <typename T, bool PERT>
-static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl)
+static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl, hipStream_t s)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl);
-    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl);
-    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl);
-    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl);
+    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s);
+    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s);
+    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s);
+    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -565,8 +570,10 @@ static void dir_args(const magic_ctx *c, SweepArgsT<T> &a, int dir, double dt)
 // kernel of the FP64 form).  The tile axis is g for the x- and z-sweeps and
 // x = h for the y-sweep, so that the y- and z-sweeps tile along rows of x.
 template <typename T, bool PERT>
-static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, bool tile, int *nl)
+static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, bool tile, int *nl,
+                     hipStream_t s = nullptr)
 {
+    if (!s) s = c->stream;
     dir_args(c, a, dir, dt);
     if (tile) {
         int a_is_g = 1;
@@ -575,18 +582,98 @@ static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, 
             a.sA = 1; a.sB = (long long)c->d.NX * c->d.NY;
             a_is_g = 0;
         }
-        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl);
+        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl, s);
     }
+    if (s != c->stream) return MAGIC_EUNSUPPORTED;
     if constexpr (!PERT) return sweep_dispatch(c, a, nl);
     return MAGIC_EUNSUPPORTED;
 }
 
+// q += dq, dq = 0 over n elements
+template <typename T>
+__global__ void k_merge(T *__restrict__ q, T *__restrict__ dq, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        q[i] = q[i] + dq[i];
+        dq[i] = T(0.0);
+    }
+}
+
+template <typename T>
+__global__ void k_fill_zero(T *__restrict__ p, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T(0.0);
+}
+
+// Whether the y-sweep runs on the second stream (a context that owns its
+// stream and has not turned the overlap off; MAGIC_SWEEP_OVERLAP=0 keeps
+// the three sweeps on one stream for every context).
+static bool overlap_sweeps(const magic_ctx *c)
+{
+    const char *e = getenv("MAGIC_SWEEP_OVERLAP");
+    return c->overlap && c->own_stream && !(e && strcmp(e, "0") == 0);
+}
+
+template <typename T>
+static int side_setup(magic_ctx *c)
+{
+    if (!c->side) {
+        HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    if (!c->dq) {
+        const long long n = (long long)c->meqn * c->P;
+        if (hipMalloc(&c->dq, (size_t)n * sizeof(T)) != hipSuccess) {
+            c->dq = nullptr;
+            return MAGIC_ENOMEM;
+        }
+        k_fill_zero<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>((T *)c->dq, n);
+        TRY(launch_check());
+    }
+    return MAGIC_OK;
+}
+
 // The three sweeps of step3 (lib/step3.f:121-621) on argument block a,
 // whose state pointers and form-specific fields are already set.
+//
+// The sweeps only add into q, each from the backup: the x- and z-sweeps add
+// into q on the context's stream while the y-sweep adds into dq on the
+// second stream, and q += dq once both are done.  Each launch's last wave of
+// workgroups leaves most compute units idle (6 launches of 4-7 rounds of one
+// workgroup per CU per direction); with two streams the other stream's
+// workgroups fill them.  The order of the additions into every element is
+// fixed (x, z, then the y increments summed alone), so the result is the
+// same on every run.
 template <typename T, bool PERT>
 static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
 {
     const bool tile = PERT || use_tile();
+    if (tile && overlap_sweeps(c)) {
+        TRY(side_setup<T>(c));
+        SweepArgsT<T> ay = a;
+        ay.qnew = (T *)c->dq;
+        HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
+        HIPCHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        {
+            Span sp(c, TC_SWY, 0, c->side);
+            TRY((sweep_dir<T, PERT>(c, ay, 1, dt, c->shape[1], true, &sp.n, c->side)));
+        }
+        for (int dir = 0; dir < 3; dir += 2) {
+            Span sp(c, TC_SWX + dir, 0);
+            TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], true, &sp.n)));
+        }
+        HIPCHECK(hipEventRecord(c->ev_join, c->side));
+        HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        const long long n = (long long)c->meqn * c->P;
+        Span sp(c, TC_OTHER, 1);
+        k_merge<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>(a.qnew, (T *)c->dq,
+                                                                                           n);
+        TRY(launch_check());
+        return MAGIC_OK;
+    }
     for (int dir = 0; dir < 3; dir++) {
         Span sp(c, TC_SWX + dir, 0);
         TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], tile, &sp.n)));
@@ -681,6 +768,13 @@ extern "C" int magic_get_tile_shapes(magic_ctx *c, int shapes[3])
 {
     if (!c || !shapes) return MAGIC_EARG;
     for (int k = 0; k < 3; k++) shapes[k] = c->shape[k];
+    return MAGIC_OK;
+}
+
+extern "C" int magic_set_sweep_overlap(magic_ctx *c, int on)
+{
+    if (!c) return MAGIC_EARG;
+    c->overlap = on != 0;
     return MAGIC_OK;
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,6 +62,13 @@ struct magic_ctx {
     double *col64 = nullptr;
     bool aux_layered = false;
     bool own_stream = true;   // false when a batch lends its stream
+    // The y-sweep of a single grid runs on a second stream next to the x-
+    // and z-sweeps and adds into dq (meqn planes of the form's type, zero
+    // between steps), merged into q after both; allocated on first use.
+    bool overlap = true;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void *dq = nullptr;
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps
@@ -92,10 +100,11 @@ static inline hipEvent_t ev_get(magic_ctx *c)
 struct Span {
     magic_ctx *c;
     int cls, n;
+    hipStream_t s;
     hipEvent_t a = nullptr;
-    Span(magic_ctx *c_, int cls_, int n_) : c(c_), cls(cls_), n(n_)
+    Span(magic_ctx *c_, int cls_, int n_, hipStream_t s_ = nullptr) : c(c_), cls(cls_), n(n_), s(s_ ? s_ : c_->stream)
     {
-        if (c->timing && (a = ev_get(c))) hipEventRecord(a, c->stream);
+        if (c->timing && (a = ev_get(c))) hipEventRecord(a, s);
     }
     ~Span()
     {
@@ -104,7 +113,7 @@ struct Span {
         }
         hipEvent_t b = ev_get(c);
         if (a && b) {
-            hipEventRecord(b, c->stream);
+            hipEventRecord(b, s);
             c->pending.push_back({cls, a, b, n});
         }
     }

@@ -708,15 +708,17 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 
 
 // Waves per SIMD the register allocation aims at: two for the FP32 form
-// with tile-axis-major lanes and no tracer, and for the work-list form with
-// one tracer (left to itself the allocator can miss 256 registers by one and
-// halve the occupancy), one otherwise.  The one-grid kernels with one tracer
-// reach two waves without being asked, and asking changes their schedule
-// for the worse (6 % on the full-domain FP32 step).
+// with tile-axis-major lanes and no tracer, and with one tracer for the
+// work-list form and the one-grid form of shape 1 (left to themselves they
+// miss 256 registers by one and halve the occupancy: 50 % slower on the
+// full-domain FP32 step), one otherwise.  The one-grid kernel of shape 0
+// with one tracer reaches two waves without being asked, and asking changes
+// its schedule for the worse (6 % on the full-domain FP32 step).
 template <typename T, int NTR, int SH, bool LIST>
 constexpr int tile_waves()
 {
-    return sizeof(T) == 4 && !TileShape<SH>::PM && (NTR == 0 || (LIST && NTR == 1)) ? 2 : 1;
+    if (sizeof(T) != 4 || TileShape<SH>::PM) return 1;
+    return NTR == 0 || (NTR == 1 && (LIST || SH == 1)) ? 2 : 1;
 }
 
 template <typename T, bool PERT, int NTR, int SH>

@@ -40,7 +40,8 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_timing_reset", "magic_set_aux_column", "magic_qinit_rest", "magic_batch_create", "magic_batch_destroy",
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
            "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
-           "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune"]
+           "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune",
+           "magic_set_sweep_overlap"]
 
 TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
 # (pencils per tile, lane order) of each tile shape
@@ -124,6 +125,7 @@ def load_library(path: str = LIB_PATH):
     L.magic_batch_set_tile_shapes.argtypes = [vp, _ip]
     L.magic_batch_get_tile_shapes.argtypes = [vp, _ip]
     L.magic_batch_tune.argtypes = [vp, C.c_int, _ip, _fp]
+    L.magic_set_sweep_overlap.argtypes = [vp, C.c_int]
     _LIB = L
     return L
 
@@ -279,6 +281,11 @@ class MagicHIP:
         sh = (C.c_int * 3)()
         self._check(self.L.magic_get_tile_shapes(self.h, sh), "magic_get_tile_shapes")
         return list(sh)
+
+    def set_sweep_overlap(self, on: bool):
+        """y-sweep on a second stream (True, the default) or the three sweeps
+        in turn (False: the batched driver's order of additions)."""
+        self._check(self.L.magic_set_sweep_overlap(self.h, 1 if on else 0), "magic_set_sweep_overlap")
 
     def tune(self, dt: float, reps: int = 3):
         """Autotuner: times every tile shape of every sweep direction on the

@@ -2,7 +2,8 @@
 
 * Each scenario of a batch, stepped by the device-resident step control and
   the captured graph, equals the same grid stepped alone by magic_claw3 in
-  the FP32 form: the same dt of every attempt (a rejected first attempt
+  the FP32 form with the sweeps in turn (magic_set_sweep_overlap 0, the
+  batch's order of additions): the same dt of every attempt (a rejected first attempt
   included) and the same state bit for bit, with the plume forcing off (with
   it on, the forcing's time factor is evaluated by the host's exp for a
   single grid and by the device's exp in a batch: rho', T', w within 1e-5
@@ -36,6 +37,7 @@ def _scenario(k, forcing):
 def _single(c, p, g, aux, q, mthbc, nsteps, dt0):
     from magic_master_amd.hip import FORM_FP32P, MagicHIP
     d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
+    d.set_sweep_overlap(False)   # the batched driver's order of additions
     dtv = [dt0, c.dtv[1], 0, 0, 0]
     t = 0.0
     log = []
@@ -99,6 +101,7 @@ def test_batch_interval_stops_at_tend():
     for k, (c, p, g, aux, col, q) in enumerate(scen):
         assert states[k]["t"] == 30.0
         d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
+        d.set_sweep_overlap(False)
         try:
             te, dtv, _, nv, _ = d.claw3(0.0, 30.0, [1.0, c.dtv[1], 0, 0, 0], c.cflv, [100000, 0], c.method,
                                          c.mthbc)

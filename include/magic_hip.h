@@ -157,9 +157,10 @@ int magic_get_tile_shapes(magic_ctx *ctx, int shapes[3]);
 int magic_tune(magic_ctx *ctx, double dt, int reps, int shapes_out[3], float *ms_out);
 
 /* ---- Order of the sweeps (no counterpart in the reference) -------------
- * on = 1 (default): the y-sweep of step3 runs on a second stream next to
- * the x- and z-sweeps and adds into a scratch copy of the state's shape,
- * added to q once both are done (the compute units the last workgroups of
+ * on = 1 (default): the y-sweep of step3 and half of the z-sweep's colour
+ * launches run on a second stream next to the x-sweep and the other z
+ * launches, adding into a scratch copy of the state's shape that is added
+ * to q once both streams are done (the compute units the last workgroups of
  * one stream's launches leave idle take the other stream's workgroups).
  * on = 0: the three sweeps add into q in turn on the context's stream, the
  * order of additions of the batched driver, whose scenarios then equal the

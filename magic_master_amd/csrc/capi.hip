@@ -124,7 +124,8 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
     hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64);
     if (c->side) hipStreamSynchronize(c->side);
-    hipFree(c->dq);
+    hipFree(c->dq[0]);
+    hipFree(c->dq[1]);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->side) hipStreamDestroy(c->side);
@@ -486,8 +487,9 @@ static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
     return MAGIC_OK;
 }
 
+// Colour launches k0 .. k1-1 (k = 3 ca + cb) of one direction.
 template <typename T, bool PERT, int NTR>
-static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s)
+static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s, int k0, int k1)
 {
     const void *fn = tile_fn<T, PERT, NTR>(sh);
     if (!fn) return MAGIC_EARG;
@@ -495,6 +497,7 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *n
     const int ntiles = (a.nA + 2 + TA - 1) / TA;
     for (int ca = 0; ca < 2; ca++)
         for (int cb = 0; cb < 3; cb++) {
+            if (3 * ca + cb < k0 || 3 * ca + cb >= k1) continue;
             a.cA = ca;
             a.cB = cb;
             const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
@@ -508,13 +511,14 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *n
 }
 
 template <typename T, bool PERT>
-static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl, hipStream_t s)
+static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl, hipStream_t s, int k0,
+                         int k1)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s);
-    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s);
-    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s);
-    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s);
+    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s, k0, k1);
+    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s, k0, k1);
+    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s, k0, k1);
+    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s, k0, k1);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -571,7 +575,7 @@ static void dir_args(const magic_ctx *c, SweepArgsT<T> &a, int dir, double dt)
 // x = h for the y-sweep, so that the y- and z-sweeps tile along rows of x.
 template <typename T, bool PERT>
 static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, bool tile, int *nl,
-                     hipStream_t s = nullptr)
+                     hipStream_t s = nullptr, int k0 = 0, int k1 = 6)
 {
     if (!s) s = c->stream;
     dir_args(c, a, dir, dt);
@@ -582,22 +586,45 @@ static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, 
             a.sA = 1; a.sB = (long long)c->d.NX * c->d.NY;
             a_is_g = 0;
         }
-        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl, s);
+        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl, s, k0, k1);
     }
-    if (s != c->stream) return MAGIC_EUNSUPPORTED;
+    if (s != c->stream || k0 != 0 || k1 != 6) return MAGIC_EUNSUPPORTED;
     if constexpr (!PERT) return sweep_dispatch(c, a, nl);
     return MAGIC_EUNSUPPORTED;
 }
 
-// q += dq, dq = 0 over n elements
+// 16-byte vectors of the state's type
 template <typename T>
-__global__ void k_merge(T *__restrict__ q, T *__restrict__ dq, long long n)
+struct Vec16;
+template <>
+struct Vec16<double> { typedef double2 type; static constexpr int N = 2; };
+template <>
+struct Vec16<float> { typedef float4 type; static constexpr int N = 4; };
+
+// q += dq over n elements (both 16-byte aligned), 16-byte loads and stores
+template <typename T>
+__global__ void k_merge(T *__restrict__ q, const T *__restrict__ dq, long long n)
 {
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::N;
+    const long long nv = n / N;
+    V *qv = reinterpret_cast<V *>(q);
+    const V *dv = reinterpret_cast<const V *>(dq);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        q[i] = q[i] + dq[i];
-        dq[i] = T(0.0);
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long i = t0; i < nv; i += stride) {
+        V a = qv[i];
+        const V b = dv[i];
+        a.x = a.x + b.x;
+        a.y = a.y + b.y;
+        if constexpr (N == 4) {
+            a.z = a.z + b.z;
+            a.w = a.w + b.w;
+        }
+        qv[i] = a;
     }
+    const long long i = nv * N + t0;   // the last n mod N elements
+    if (i < n) q[i] = q[i] + dq[i];
 }
 
 template <typename T>
@@ -624,13 +651,14 @@ static int side_setup(magic_ctx *c)
         HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
-    if (!c->dq) {
+    for (int k = 0; k < 2; k++) {
+        if (c->dq[k]) continue;
         const long long n = (long long)c->meqn * c->P;
-        if (hipMalloc(&c->dq, (size_t)n * sizeof(T)) != hipSuccess) {
-            c->dq = nullptr;
+        if (hipMalloc(&c->dq[k], (size_t)n * sizeof(T)) != hipSuccess) {
+            c->dq[k] = nullptr;
             return MAGIC_ENOMEM;
         }
-        k_fill_zero<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>((T *)c->dq, n);
+        k_fill_zero<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>((T *)c->dq[k], n);
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -639,39 +667,56 @@ static int side_setup(magic_ctx *c)
 // The three sweeps of step3 (lib/step3.f:121-621) on argument block a,
 // whose state pointers and form-specific fields are already set.
 //
-// The sweeps only add into q, each from the backup: the x- and z-sweeps add
-// into q on the context's stream while the y-sweep adds into dq on the
-// second stream, and q += dq once both are done.  Each launch's last wave of
-// workgroups leaves most compute units idle (6 launches of 4-7 rounds of one
-// workgroup per CU per direction); with two streams the other stream's
-// workgroups fill them.  The order of the additions into every element is
-// fixed (x, z, then the y increments summed alone), so the result is the
-// same on every run.
+// The sweeps only add into q, each from the backup, so they can run on two
+// streams into two buffers: the x-sweep and the first three colour launches
+// of the z-sweep add into q on the context's stream, the y-sweep and the
+// other three z launches into a zeroed buffer dq on the second stream (9
+// launches each), and q += dq once both are done.  Each launch's last wave
+// of workgroups leaves most compute units idle (4-7 rounds of one workgroup
+// per CU per launch); with two streams the other stream's workgroups fill
+// them.  Two buffers alternate: the one the previous call added to q is
+// cleared on the second stream while this call's sweeps run.  The order of
+// the additions into every element is fixed (x and the first z colours,
+// then dq's sum of y and the other z colours), so the result is the same
+// on every run.
 template <typename T, bool PERT>
 static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
 {
     const bool tile = PERT || use_tile();
     if (tile && overlap_sweeps(c)) {
         TRY(side_setup<T>(c));
+        const long long n = (long long)c->meqn * c->P;
+        T *dq = (T *)c->dq[c->dq_cur], *dq_next = (T *)c->dq[c->dq_cur ^ 1];
         SweepArgsT<T> ay = a;
-        ay.qnew = (T *)c->dq;
+        ay.qnew = dq;
         HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
         HIPCHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        // dq_next was added to q by the previous call, before the fork
+        k_fill_zero<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->side>>>(dq_next, n);
+        TRY(launch_check());
         {
             Span sp(c, TC_SWY, 0, c->side);
             TRY((sweep_dir<T, PERT>(c, ay, 1, dt, c->shape[1], true, &sp.n, c->side)));
         }
-        for (int dir = 0; dir < 3; dir += 2) {
-            Span sp(c, TC_SWX + dir, 0);
-            TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], true, &sp.n)));
+        {
+            Span sp(c, TC_SWZ, 0, c->side);
+            TRY((sweep_dir<T, PERT>(c, ay, 2, dt, c->shape[2], true, &sp.n, c->side, 3, 6)));
+        }
+        {
+            Span sp(c, TC_SWX, 0);
+            TRY((sweep_dir<T, PERT>(c, a, 0, dt, c->shape[0], true, &sp.n)));
+        }
+        {
+            Span sp(c, TC_SWZ, 0);
+            TRY((sweep_dir<T, PERT>(c, a, 2, dt, c->shape[2], true, &sp.n, c->stream, 0, 3)));
         }
         HIPCHECK(hipEventRecord(c->ev_join, c->side));
         HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-        const long long n = (long long)c->meqn * c->P;
         Span sp(c, TC_OTHER, 1);
-        k_merge<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>(a.qnew, (T *)c->dq,
-                                                                                           n);
+        k_merge<T><<<(unsigned)std::min<long long>(blocks(n / Vec16<T>::N + 1, 256), 8192), 256, 0, c->stream>>>(
+            a.qnew, dq, n);
         TRY(launch_check());
+        c->dq_cur ^= 1;
         return MAGIC_OK;
     }
     for (int dir = 0; dir < 3; dir++) {

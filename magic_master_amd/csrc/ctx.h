@@ -63,12 +63,14 @@ struct magic_ctx {
     bool aux_layered = false;
     bool own_stream = true;   // false when a batch lends its stream
     // The y-sweep of a single grid runs on a second stream next to the x-
-    // and z-sweeps and adds into dq (meqn planes of the form's type, zero
-    // between steps), merged into q after both; allocated on first use.
+    // and z-sweeps and adds into dq[dq_cur] (meqn planes of the form's
+    // type), added to q after both; the other buffer is cleared meanwhile
+    // for the next step3.  Allocated on first use.
     bool overlap = true;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    void *dq = nullptr;
+    void *dq[2] = {nullptr, nullptr};
+    int dq_cur = 0;
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps

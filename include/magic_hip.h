@@ -149,6 +149,9 @@ int magic_timing_reset(magic_ctx *ctx);
  * set, dt > 0: one untimed and reps timed sweeps each), keeps the fastest
  * per direction and returns them in shapes_out (and, if ms_out is not NULL,
  * the mean ms per sweep in ms_out[3 * MAGIC_TILE_NSHAPE], direction-major).
+ * With the sweep overlap on (magic_set_sweep_overlap), it then varies each
+ * direction's shape in turn, timing the overlapped sweeps of step3 as a
+ * whole, and keeps an assignment that is at least 1 % faster.
  * q is unchanged bit for bit; the backup copy then holds q.
  * MAGIC_EUNSUPPORTED for the FP64 pencil kernel (MAGIC_SWEEP=pencil). */
 #define MAGIC_TILE_NSHAPE 4

@@ -44,14 +44,17 @@ SWEEP_CLASSES = ["sweep_x", "sweep_y", "sweep_z"]   # timing classes of the thre
 SWEEP_KERNEL = "magic::tile_sweep<{t}, {pert}, {ntr}, {sh}>"
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of kernel from the committed rocprofv3 counter
-    summary (profiles/hbm_traffic_*.json), or None when there is none."""
+def measured_traffic(kernel, shapes):
+    """HBM bytes per launch of kernel, run with the tile shape assignment
+    shapes, from the committed rocprofv3 counter summaries
+    (profiles/hbm_traffic_*.json: one entry per kernel and assignment), or
+    None when none matches."""
     import glob
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.json"))):
         d = json.load(open(f))
-        if d.get("kernel") == kernel:
-            return d["hbm_bytes_per_launch"], os.path.relpath(f, ROOT)
+        for e in d.get("entries", []):
+            if e["kernel"] == kernel and list(e["shapes"]) == list(shapes):
+                return e["hbm_bytes_per_launch"], os.path.relpath(f, ROOT)
     return None, None
 
 
@@ -281,7 +284,7 @@ def main():
                     else (HYP_BYTES_PER_CELL, SRC_BYTES_PER_CELL))
     bytes_per_launch = hyp_b * cells / launches_per_step3
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic, traffic_src = measured_traffic(kname) if a.workload == "full" else (None, None)
+    traffic, traffic_src = measured_traffic(kname, r["shapes"]) if a.workload == "full" else (None, None)
     st = np.array(r["step_ms"])
     out = {
         "metric": METRIC,

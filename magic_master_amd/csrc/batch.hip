@@ -274,8 +274,8 @@ static int launch_src(magic_batch *b, int s)
         const StepGate gd = gate(b, s, GATE_DIFF, nn);
         TRY(launch_bc(b, s, mthbc, gd));
         k_zghost_uvw<<<gface, 64, 0, b->cs>>>(d, c->q32, gd);
-        k_diff_cross<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->u32, c->q32, nu, 0.f, 0.f, 0.f,
-                                                                       gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
+        k_diff_cross<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->cs>>>(
+            d, c->u32, c->q32, nu, 0.f, 0.f, 0.f, diff_kb(d), gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             k_diff_xyz<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->cs>>>(

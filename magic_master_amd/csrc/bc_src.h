@@ -71,7 +71,7 @@ __global__ void k_src_convert(Dims d, double *q, const Aux a);
 template <typename T, class G>
 __global__ void k_zghost_uvw(Dims d, T *q, G gate);
 template <typename T, bool COLC, class G>
-__global__ void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz, G gate);
+__global__ void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz, int kb, G gate);
 template <typename T, bool COLC, class G>
 __global__ void k_diff_xyz(Dims d, T *q, const T *u, const T *coef, T fx, T fy, T fz, int kb, G gate);
 template <typename T, class G>

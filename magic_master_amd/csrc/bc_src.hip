@@ -156,13 +156,13 @@ template __global__ void k_src_finish<true>(Dims, double *, int, const Aux, Forc
 template __global__ void k_diff_xyz<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,
                                                          double, int, NoGate);
 template __global__ void k_diff_cross<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,
-                                                           double, NoGate);
+                                                           double, int, NoGate);
 template __global__ void k_pass<double, true, NoGate>(Dims, double *, const double *, const double *, const double *, double,
                                                      int, NoGate);
 template __global__ void k_diff_xyz<double, false, NoGate>(Dims, double *, const double *, const double *, double, double,
                                                           double, int, NoGate);
 template __global__ void k_diff_cross<double, false, NoGate>(Dims, double *, const double *, const double *, double, double, double,
-                                                        NoGate);
+                                                        int, NoGate);
 template __global__ void k_copy_interior<double, NoGate>(Dims, double *, const double *, int, NoGate);
 template __global__ void k_pass<double, false, NoGate>(Dims, double *, const double *, const double *, const double *, double, int,
                                                   NoGate);

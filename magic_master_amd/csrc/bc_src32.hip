@@ -23,7 +23,7 @@ template __global__ void k_zghost_uvw<float, NoGate>(Dims, float *, NoGate);
 template __global__ void k_diff_xyz<float, true, NoGate>(Dims, float *, const float *, const float *, float, float, float,
                                                         int, NoGate);
 template __global__ void k_diff_cross<float, true, NoGate>(Dims, float *, const float *, const float *, float, float, float,
-                                                        NoGate);
+                                                        int, NoGate);
 template __global__ void k_copy_interior<float, NoGate>(Dims, float *, const float *, int, NoGate);
 template __global__ void k_pass<float, true, NoGate>(Dims, float *, const float *, const float *, const float *, float, int,
                                                   NoGate);
@@ -159,7 +159,7 @@ template __global__ void k_zghost_uvw<float, StepGate>(Dims, float *, StepGate);
 template __global__ void k_diff_xyz<float, true, StepGate>(Dims, float *, const float *, const float *, float, float,
                                                           float, int, StepGate);
 template __global__ void k_diff_cross<float, true, StepGate>(Dims, float *, const float *, const float *, float,
-                                                            float, float, StepGate);
+                                                            float, float, int, StepGate);
 template __global__ void k_copy_interior<float, StepGate>(Dims, float *, const float *, int, StepGate);
 template __global__ void k_pass<float, true, StepGate>(Dims, float *, const float *, const float *, const float *,
                                                       float, int, StepGate);

@@ -952,10 +952,10 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q32, NoGate{});
         TRY(launch_check());
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<float, true, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u32, c->q32, nu,
+        k_diff_cross<float, true, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(d, c->u32, c->q32, nu,
                                                              (float)(dtr32 / (c->dx * c->dy)),
                                                              (float)(dtr32 / (c->dx * c->dz)),
-                                                             (float)(dtr32 / (c->dy * c->dz)), NoGate{});
+                                                             (float)(dtr32 / (c->dy * c->dz)), diff_kb(d), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             if (cc == 1) {
@@ -1036,8 +1036,9 @@ static int src3_passes64(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
         k_zghost_uvw<<<gface, 64, 0, c->stream>>>(d, c->q, NoGate{});
         TRY(launch_check());
         const double dtr32 = dtr / 6.0;
-        k_diff_cross<double, COL, NoGate><<<gint, cell_block(), 0, c->stream>>>(d, c->u, c->q, nu, dtr32 / (c->dx * c->dy),
-                                                 dtr32 / (c->dx * c->dz), dtr32 / (c->dy * c->dz), NoGate{});
+        k_diff_cross<double, COL, NoGate><<<diff_grid(d), cell_block(), 0, c->stream>>>(d, c->u, c->q, nu,
+                                                 dtr32 / (c->dx * c->dy), dtr32 / (c->dx * c->dz),
+                                                 dtr32 / (c->dy * c->dz), diff_kb(d), NoGate{});
         TRY(launch_check());
         for (int cc = 1; cc <= 3; cc++) {
             if (cc == 1) {

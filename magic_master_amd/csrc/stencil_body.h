@@ -117,27 +117,72 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
 // diffuse.f:70-118: cross-derivative terms of q(2:4) (after the z-ghost
 // fill), written to the interior of the three planes of u; q is not
 // changed (its planes keep the values the reference's copy holds, whose
-// ghost layers the directional passes read).
+// ghost layers the directional passes read).  A workgroup covers 64 x-cells
+// by 4 y-rows and marches over kb layers, holding q(2:4) of three
+// consecutive layers with a one-cell halo in x and y in LDS, so each value
+// is read from memory once per workgroup.  Launch: block (64, 4), grid
+// (ceil(mx/64), ceil(my/4), ceil(mz/kb)).
 template <typename T, bool COLC, class G>
-__global__ void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz, G gate)
+__global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz,
+                                                    int kb, G gate)
 {
     if (!gate.on()) return;
     dxy = gate.f(dxy, 0);
     dxz = gate.f(dxz, 1);
     dyz = gate.f(dyz, 2);
-    INTERIOR_IJK
-    const long long sx = 1, sy = d.NX, sz = (long long)d.NX * d.NY;
+    constexpr int TX = 66, TL = TX * 6;     // (64 + 2) x (4 + 2) cells per layer
+    __shared__ T s[3][3][TL];               // [layer k mod 3][component][cell]
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;
+    const int i0 = (int)blockIdx.x * 64 + 1, j0 = (int)blockIdx.y * 4 + 1;
+    const int k0 = (int)blockIdx.z * kb + 1;
+    const int k1 = k0 + kb - 1 < d.mz ? k0 + kb - 1 : d.mz;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i <= d.mx && j <= d.my;
     const T *u1 = q + d.P, *u2 = q + 2 * d.P, *u3 = q + 3 * d.P;
-    auto X = [&](const T *f, long long a, long long b) {
-        return f[c + a + b] - f[c - a + b] + f[c - a - b] - f[c + a - b];
+    // layer kk (0 <= kk <= mz + 1) of the tile and its halo; cells past
+    // mx + 1 or my + 1 are not read by any interior cell
+    auto load = [&](int kk) {
+        const int sl = kk % 3;
+        for (int e = t; e < TL; e += 256) {
+            const int ii = i0 - 1 + e % TX, jj = j0 - 1 + e / TX;
+            if (ii <= d.mx + 1 && jj <= d.my + 1) {
+                const long long c = cid(d, ii, jj, kk);
+                s[sl][0][e] = u1[c];
+                s[sl][1][e] = u2[c];
+                s[sl][2][e] = u3[c];
+            }
+        }
     };
-    const T ctxxy = X(u2, sx, sy), ctxxz = X(u3, sx, sz);
-    const T ctyxy = X(u1, sx, sy), ctyyz = X(u3, sy, sz);
-    const T ctzxz = X(u1, sx, sz), ctzyz = X(u2, sy, sz);
-    const T co = coef[COLC ? (long long)(k + d.mbc - 1) : c];
-    u[c] = u1[c] + co * (dxy * ctxxy + dxz * ctxxz);
-    u[d.P + c] = u2[c] + co * (dxy * ctyxy + dyz * ctyyz);
-    u[2 * d.P + c] = u3[c] + co * (dxz * ctzxz + dyz * ctzyz);
+    load(k0 - 1);
+    load(k0);
+    const int e = (ty + 1) * TX + tx + 1;   // this lane's cell in the tile
+    for (int k = k0; k <= k1; k++) {
+        load(k + 1);   // into the slot of layer k - 2, whose readers passed the barrier below
+        __syncthreads();
+        if (own) {
+            const int lm = (k - 1) % 3, l0 = k % 3, lp = (k + 1) % 3;
+            // f[c+a+b] - f[c-a+b] + f[c-a-b] - f[c+a-b] for the pairs of axes
+            auto Xxy = [&](int m) {
+                const T *f = s[l0][m];
+                return f[e + 1 + TX] - f[e - 1 + TX] + f[e - 1 - TX] - f[e + 1 - TX];
+            };
+            auto Xxz = [&](int m) {
+                return s[lp][m][e + 1] - s[lp][m][e - 1] + s[lm][m][e - 1] - s[lm][m][e + 1];
+            };
+            auto Xyz = [&](int m) {
+                return s[lp][m][e + TX] - s[lp][m][e - TX] + s[lm][m][e - TX] - s[lm][m][e + TX];
+            };
+            const T ctxxy = Xxy(1), ctxxz = Xxz(2);
+            const T ctyxy = Xxy(0), ctyyz = Xyz(2);
+            const T ctzxz = Xxz(0), ctzyz = Xyz(1);
+            const long long c = cid(d, i, j, k);
+            const T co = coef[COLC ? (long long)(k + d.mbc - 1) : c];
+            u[c] = s[l0][0][e] + co * (dxy * ctxxy + dxz * ctxxz);
+            u[d.P + c] = s[l0][1][e] + co * (dxy * ctyxy + dyz * ctyyz);
+            u[2 * d.P + c] = s[l0][2][e] + co * (dxz * ctzxz + dyz * ctzyz);
+        }
+        __syncthreads();
+    }
 }
 
 // diffuse.f:153-208 for one velocity component: the x, y and z passes of

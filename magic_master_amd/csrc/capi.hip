@@ -601,30 +601,50 @@ struct Vec16<double> { typedef double2 type; static constexpr int N = 2; };
 template <>
 struct Vec16<float> { typedef float4 type; static constexpr int N = 4; };
 
-// q += dq over n elements (both 16-byte aligned), 16-byte loads and stores
+// q += dq over n elements (both 16-byte aligned): 16-byte loads and stores,
+// four of each pair in flight per lane
+template <typename T>
+__device__ __forceinline__ void vadd(typename Vec16<T>::type &a, const typename Vec16<T>::type &b)
+{
+    a.x = a.x + b.x;
+    a.y = a.y + b.y;
+    if constexpr (Vec16<T>::N == 4) {
+        a.z = a.z + b.z;
+        a.w = a.w + b.w;
+    }
+}
+
 template <typename T>
 __global__ void k_merge(T *__restrict__ q, const T *__restrict__ dq, long long n)
 {
     using V = typename Vec16<T>::type;
-    constexpr int N = Vec16<T>::N;
+    constexpr int N = Vec16<T>::N, U = 4;
     const long long nv = n / N;
     V *qv = reinterpret_cast<V *>(q);
     const V *dv = reinterpret_cast<const V *>(dq);
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (long long i = t0; i < nv; i += stride) {
-        V a = qv[i];
-        const V b = dv[i];
-        a.x = a.x + b.x;
-        a.y = a.y + b.y;
-        if constexpr (N == 4) {
-            a.z = a.z + b.z;
-            a.w = a.w + b.w;
+    long long i = t0;
+    for (; i + (U - 1) * stride < nv; i += U * stride) {
+        V a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            a[u] = qv[i + u * stride];
+            b[u] = dv[i + u * stride];
         }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            vadd<T>(a[u], b[u]);
+            qv[i + u * stride] = a[u];
+        }
+    }
+    for (; i < nv; i += stride) {
+        V a = qv[i];
+        vadd<T>(a, dv[i]);
         qv[i] = a;
     }
-    const long long i = nv * N + t0;   // the last n mod N elements
-    if (i < n) q[i] = q[i] + dq[i];
+    const long long r = nv * N + t0;   // the last n mod N elements
+    if (r < n) q[r] = q[r] + dq[r];
 }
 
 template <typename T>

@@ -157,3 +157,28 @@ def test_step3_pencil_kernel(case, oracle, monkeypatch):
     d.close()
     assert abs(cfl_g - cfl_o) <= 1e-12 * cfl_o
     assert rel_err(qg, qo) <= 1e-12
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_copyq3_backup_restore(form):
+    """copyq3 (lib/copyq3.f:21-26) as claw3 uses it (mpilib/claw3_mpi.f:807,
+    869): the backup taken before a step restores the state bit for bit
+    after it, and a step from the restored state repeats the first one."""
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
+    (mx, my, mz), mthbc = CASES[1]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=5, vel=0.3)
+    print(f"[copyq3] {form} grid {mx}x{my}x{mz}", flush=True)
+    d = MagicHIP(g, p, aux, q, form=FORM_FP64 if form == "fp64" else FORM_FP32P)
+    try:
+        d.bc3(mthbc)
+        before = d.get_q()
+        d.backup()
+        d.step3(2.0)
+        after = d.get_q()
+        assert not np.array_equal(after, before)
+        d.restore()
+        np.testing.assert_array_equal(d.get_q(), before)
+        d.step3(2.0)
+        np.testing.assert_array_equal(d.get_q(), after)
+    finally:
+        d.close()

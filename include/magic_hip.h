@@ -214,6 +214,14 @@ int magic_batch_state(magic_batch *b, int s, double real_out[5], int int_out[6])
 int magic_batch_set_tile_shapes(magic_batch *b, const int shapes[3]);
 int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3]);
 int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], float *ms_out);
+/* The batch's sweeps in the same two orders as magic_set_sweep_overlap.
+ * Off by default: the batch's persistent launches leave few compute units
+ * idle, and the per-scenario merge costs more than the overlap gains.
+ * While on, each scenario holds one more buffer of its state's size.
+ * Setting it captures the graph again.  A scenario equals the same grid
+ * stepped alone bit for bit when both use the same order and tile shapes.
+ * No counterpart in the reference. */
+int magic_batch_set_sweep_overlap(magic_batch *b, int on);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@
 
 namespace magic {
 
-enum { GATE_RUN = 0, GATE_ACCEPTED, GATE_REJECTED, GATE_DIFF, GATE_COND, GATE_COND_MORE, GATE_COND_LAST };
+enum { GATE_RUN = 0, GATE_ACCEPTED, GATE_REJECTED, GATE_DIFF, GATE_COND, GATE_COND_MORE, GATE_COND_LAST, GATE_SWEPT };
 
 // factor slots of ScenCtl::fac (source pass, src3.f90:314-322 and the
 // passes of diffuse.f / conduct.f)
@@ -25,6 +25,7 @@ struct ScenCtl {
     int active;      // still stepping in the current interval
     int newstep;     // the next attempt starts a step (else it retries one)
     int run;         // this attempt runs
+    int swept;       // this attempt's sweeps ran (run before the decision, which may clear run)
     int accepted;    // this attempt was accepted
     int rejected;    // this attempt was rejected
     int nsteps, maxsteps, attempts, rejects, status;
@@ -53,6 +54,7 @@ struct StepGate {
         case GATE_COND: return c->accepted != 0 && nn <= c->ndtr && kk <= c->ndtrc;
         case GATE_COND_MORE: return c->accepted != 0 && nn <= c->ndtr && kk < c->ndtrc;
         case GATE_COND_LAST: return c->accepted != 0 && nn <= c->ndtr && kk == c->ndtrc;
+        case GATE_SWEPT: return c->swept != 0;
         default: return false;
         }
     }

@@ -27,9 +27,11 @@ struct magic_batch {
     hipStream_t stream = nullptr;
     // The per-scenario kernel chains of an attempt (boundaries and backup,
     // then restore and source pass) are independent across scenarios: the
-    // capture spreads them over NSIDE side streams (graph branches), so small
-    // scenarios' kernels overlap instead of running one after the other.
-    static constexpr int NSIDE = 4;
+    // capture spreads them over nside side streams (graph branches, at most
+    // NSIDE; MAGIC_BATCH_BRANCHES sets the count), so small scenarios'
+    // kernels overlap instead of running one after the other.
+    static constexpr int NSIDE = 16;
+    int nside = 4;
     hipStream_t side[NSIDE] = {};
     hipEvent_t fork_ev[2] = {}, join_ev[2][NSIDE] = {};
     hipStream_t cs = nullptr;            // stream of the per-scenario launches being recorded
@@ -46,6 +48,16 @@ struct magic_batch {
     int shape[3] = {0, 0, 0};            // tile shape of the x-, y- and z-sweeps
     int grid[3] = {0, 0, 0};             // persistent grid size of each direction's kernel
     int *one_d = nullptr;                // a device 1: the run flag of every tile while tuning
+    // Sweep overlap, as for a single grid (off by default): the y-sweep and
+    // the last three colour launches of the z-sweep add into each
+    // scenario's dq (ctx dq[0]) on side[0], the rest into q on the main
+    // stream; each scenario then adds dq into q and clears it.  The same
+    // order of additions as a single grid with the overlap on.  The batch's
+    // persistent launches leave few compute units idle, and on the
+    // 64-scenario ensemble the merge's traffic outweighs the gain (3 %
+    // slower), hence the default.
+    bool overlap = false;
+    hipEvent_t sw_fork = nullptr, sw_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;                   // control blocks changed on the host
@@ -62,6 +74,7 @@ __global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, in
     c.accepted = 0;
     c.rejected = 0;
     c.run = c.active != 0 && c.status == 0;
+    c.swept = c.run;
     if (!c.run) return;
     if (c.newstep) {                                   // claw3_mpi.f:741
         c.told = c.t;
@@ -86,6 +99,7 @@ __global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, in
     if (ndtr > c.ndtr_cap || ndtrc > c.ndtrc_cap) {   // beyond the captured substeps
         c.status = MAGIC_EUNSUPPORTED;
         c.run = 0;
+        c.swept = 0;
         return;
     }
     c.ndtr = ndtr;
@@ -178,7 +192,7 @@ static int src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
 }
 
 template <int NTR>
-static int launch_lists(magic_batch *b, int dir, int k)
+static int launch_lists(magic_batch *b, int dir, int k, hipStream_t st)
 {
     int lo = b->item_off[6 * dir + k], n = b->item_off[6 * dir + k + 1] - lo;
     if (n <= 0) return MAGIC_OK;
@@ -189,18 +203,20 @@ static int launch_lists(magic_batch *b, int dir, int k)
     const int *const *run = b->run_d;
     int *cnt = b->counters_d + 6 * dir + k;
     void *kargs[] = {&args, &items, &n, &run, &cnt};
-    HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, b->stream));
+    HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, st));
     return MAGIC_OK;
 }
 
-static int launch_dir(magic_batch *b, int dir)
+// colour launches k0 .. k1-1 of direction dir on stream st
+static int launch_dir(magic_batch *b, int dir, hipStream_t st = nullptr, int k0 = 0, int k1 = 6)
 {
-    for (int k = 0; k < 6; k++) {
+    if (!st) st = b->stream;
+    for (int k = k0; k < k1; k++) {
         switch (b->meqn - 5) {
-        case 0: TRY(launch_lists<0>(b, dir, k)); break;
-        case 1: TRY(launch_lists<1>(b, dir, k)); break;
-        case 2: TRY(launch_lists<2>(b, dir, k)); break;
-        case 5: TRY(launch_lists<5>(b, dir, k)); break;
+        case 0: TRY(launch_lists<0>(b, dir, k, st)); break;
+        case 1: TRY(launch_lists<1>(b, dir, k, st)); break;
+        case 2: TRY(launch_lists<2>(b, dir, k, st)); break;
+        case 5: TRY(launch_lists<5>(b, dir, k, st)); break;
         default: return MAGIC_EUNSUPPORTED;
         }
         TRY(launch_check());
@@ -210,8 +226,46 @@ static int launch_dir(magic_batch *b, int dir)
 
 static int launch_sweeps(magic_batch *b)
 {
+    if (b->overlap) {
+        HIPCHECK(hipEventRecord(b->sw_fork, b->stream));
+        HIPCHECK(hipStreamWaitEvent(b->side[0], b->sw_fork, 0));
+        TRY(launch_dir(b, 1, b->side[0]));
+        TRY(launch_dir(b, 2, b->side[0], 3, 6));
+        TRY(launch_dir(b, 0));
+        TRY(launch_dir(b, 2, b->stream, 0, 3));
+        HIPCHECK(hipEventRecord(b->sw_join, b->side[0]));
+        HIPCHECK(hipStreamWaitEvent(b->stream, b->sw_join, 0));
+        return MAGIC_OK;
+    }
     for (int dir = 0; dir < 3; dir++) TRY(launch_dir(b, dir));
     return MAGIC_OK;
+}
+
+// q += dq, dq = 0 (n elements, both 16-byte aligned) for a scenario whose
+// sweeps ran in this attempt; the additions round as a single grid's merge
+__global__ void k_batch_merge(float *__restrict__ q, float *__restrict__ dq, long long n, StepGate g)
+{
+    if (!g.on()) return;
+    const long long nv = n / 4;
+    float4 *qv = reinterpret_cast<float4 *>(q);
+    float4 *dv = reinterpret_cast<float4 *>(dq);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long i = t0; i < nv; i += stride) {
+        float4 a = qv[i];
+        const float4 b = dv[i];
+        a.x = a.x + b.x;
+        a.y = a.y + b.y;
+        a.z = a.z + b.z;
+        a.w = a.w + b.w;
+        qv[i] = a;
+        dv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const long long r = nv * 4 + t0;   // the last n mod 4 elements
+    if (r < n) {
+        q[r] = q[r] + dq[r];
+        dq[r] = 0.f;
+    }
 }
 
 static StepGate gate(const magic_batch *b, int s, int mode, int nn = 0, int kk = 0, int fslot = -1)
@@ -326,13 +380,13 @@ static int launch_src(magic_batch *b, int s)
 static int fork_side(magic_batch *b, int k)
 {
     HIPCHECK(hipEventRecord(b->fork_ev[k], b->stream));
-    for (int i = 0; i < magic_batch::NSIDE; i++) HIPCHECK(hipStreamWaitEvent(b->side[i], b->fork_ev[k], 0));
+    for (int i = 0; i < b->nside; i++) HIPCHECK(hipStreamWaitEvent(b->side[i], b->fork_ev[k], 0));
     return MAGIC_OK;
 }
 
 static int join_side(magic_batch *b, int k)
 {
-    for (int i = 0; i < magic_batch::NSIDE; i++) {
+    for (int i = 0; i < b->nside; i++) {
         HIPCHECK(hipEventRecord(b->join_ev[k][i], b->side[i]));
         HIPCHECK(hipStreamWaitEvent(b->stream, b->join_ev[k][i], 0));
     }
@@ -351,7 +405,7 @@ static int record_attempt(magic_batch *b)
     for (int s = 0, k = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (!c) continue;
-        b->cs = b->side[k++ % magic_batch::NSIDE];
+        b->cs = b->side[k++ % b->nside];
         const StepGate g = gate(b, s, GATE_RUN);
         TRY(launch_bc(b, s, &b->mthbc[6 * s], g));
         if (c->meqn > 5) {
@@ -368,7 +422,15 @@ static int record_attempt(magic_batch *b)
     for (int s = 0, k = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (!c) continue;
-        b->cs = b->side[k++ % magic_batch::NSIDE];
+        b->cs = b->side[k++ % b->nside];
+        if (b->overlap) {
+            const long long n = c->P * c->meqn;
+            long long nb = (n / 4 + 255) / 256;
+            if (nb > 8192) nb = 8192;
+            k_batch_merge<<<(unsigned)(nb > 0 ? nb : 1), 256, 0, b->cs>>>(c->q32, (float *)c->dq[0], n,
+                                                                          gate(b, s, GATE_SWEPT));
+            TRY(launch_check());
+        }
         TRY(launch_copy(b, c->q32, c->qold32, c->P * c->meqn, gate(b, s, GATE_REJECTED)));
         TRY(launch_src(b, s));
     }
@@ -445,6 +507,7 @@ static int build_lists(magic_batch *b, bool tuning = false)
                     SweepArgsT<float> t = a;
                     t.cA = ca;
                     t.cB = cb;
+                    if (!tuning && b->overlap && (dir == 1 || (dir == 2 && k >= 3))) t.qnew = (float *)c->dq[0];
                     args[ai] = t;
                     run[ai] = tuning ? b->one_d : &b->ctl_d[s].run;
                     const int gx = (ntiles - ca + 1) / 2, gy = (a.nB + 1 - cb) / 3 + 1;
@@ -502,6 +565,16 @@ static int build_graph(magic_batch *b)
     b->graph = nullptr;
     b->graph_ok = false;
     TRY(list_grids(b));
+    if (b->overlap)
+        for (magic_ctx *c : b->sc) {
+            if (!c || c->dq[0]) continue;
+            const size_t bytes = (size_t)c->P * c->meqn * sizeof(float);
+            if (hipMalloc(&c->dq[0], bytes) != hipSuccess) {
+                c->dq[0] = nullptr;
+                return MAGIC_ENOMEM;
+            }
+            HIPCHECK(hipMemsetAsync(c->dq[0], 0, bytes, b->stream));   // before the capture
+        }
     TRY(build_lists(b));
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = record_attempt(b);
@@ -599,14 +672,21 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
     b->mbc = mbc;
     b->maux = maux;
     b->form = form;
+    if (const char *e = getenv("MAGIC_BATCH_BRANCHES")) {
+        const int v = atoi(e);
+        if (v >= 1 && v <= magic_batch::NSIDE) b->nside = v;
+    }
     int st = MAGIC_OK;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
     b->cs = b->stream;
-    for (int i = 0; i < magic_batch::NSIDE && st == MAGIC_OK; i++)
+    for (int i = 0; i < b->nside && st == MAGIC_OK; i++)
         if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
+    if (st == MAGIC_OK && (hipEventCreateWithFlags(&b->sw_fork, hipEventDisableTiming) != hipSuccess ||
+                           hipEventCreateWithFlags(&b->sw_join, hipEventDisableTiming) != hipSuccess))
+        st = MAGIC_EHIP;
     for (int k = 0; k < 2 && st == MAGIC_OK; k++) {
         if (hipEventCreateWithFlags(&b->fork_ev[k], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
-        for (int i = 0; i < magic_batch::NSIDE && st == MAGIC_OK; i++)
+        for (int i = 0; i < b->nside && st == MAGIC_OK; i++)
             if (hipEventCreateWithFlags(&b->join_ev[k][i], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
     }
     if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
@@ -639,6 +719,8 @@ extern "C" int magic_batch_destroy(magic_batch *b)
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
     hipFree(b->one_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
+    if (b->sw_fork) hipEventDestroy(b->sw_fork);
+    if (b->sw_join) hipEventDestroy(b->sw_join);
     for (int k = 0; k < 2; k++) {
         if (b->fork_ev[k]) hipEventDestroy(b->fork_ev[k]);
         for (int i = 0; i < magic_batch::NSIDE; i++)
@@ -772,6 +854,14 @@ extern "C" int magic_batch_set_tile_shapes(magic_batch *b, const int shapes[3])
 {
     if (!b || !shapes || !shapes_valid(shapes)) return MAGIC_EARG;
     for (int k = 0; k < 3; k++) b->shape[k] = shapes[k];
+    b->graph_ok = false;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_batch_set_sweep_overlap(magic_batch *b, int on)
+{
+    if (!b) return MAGIC_EARG;
+    b->overlap = on != 0;
     b->graph_ok = false;
     return MAGIC_OK;
 }

@@ -41,7 +41,7 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
            "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
            "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune",
-           "magic_set_sweep_overlap"]
+           "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap"]
 
 TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
 # (pencils per tile, lane order) of each tile shape
@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH):
     L.magic_batch_get_tile_shapes.argtypes = [vp, _ip]
     L.magic_batch_tune.argtypes = [vp, C.c_int, _ip, _fp]
     L.magic_set_sweep_overlap.argtypes = [vp, C.c_int]
+    L.magic_batch_set_sweep_overlap.argtypes = [vp, C.c_int]
     _LIB = L
     return L
 
@@ -394,6 +395,10 @@ class MagicBatch:
     def set_tile_shapes(self, shapes):
         self._check(self.L.magic_batch_set_tile_shapes(self.h, (C.c_int * 3)(*shapes)),
                     "magic_batch_set_tile_shapes")
+
+    def set_sweep_overlap(self, on: bool):
+        """The sweeps' order of additions, as MagicHIP.set_sweep_overlap."""
+        self._check(self.L.magic_batch_set_sweep_overlap(self.h, 1 if on else 0), "magic_batch_set_sweep_overlap")
 
     def tile_shapes(self):
         sh = (C.c_int * 3)()

@@ -65,10 +65,10 @@ class Outcome:
 
 
 def footprint_bytes(g: P.Grid) -> int:
-    """Device memory of one scenario of the FP32 form: q' and its backup
-    (2 x 4 B x meqn), the source pass's scratch (5 x 4 B), the FP64 staging
-    of q for upload and download (8 B x meqn), per padded cell, plus the
-    columns."""
+    """Device memory of one scenario of the FP32 form in a batch (sweeps in
+    turn, the batch's default): q' and its backup (2 x 4 B x meqn), the
+    source pass's scratch (5 x 4 B), the FP64 staging of q for upload and
+    download (8 B x meqn), per padded cell, plus the columns."""
     cells = (g.mx + 2 * g.mbc) * (g.my + 2 * g.mbc) * (g.mz + 2 * g.mbc)
     return cells * (8 * g.meqn + 20 + 8 * g.meqn) + 12 * (g.mz + 2 * g.mbc) * 8
 

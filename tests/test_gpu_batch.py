@@ -2,8 +2,8 @@
 
 * Each scenario of a batch, stepped by the device-resident step control and
   the captured graph, equals the same grid stepped alone by magic_claw3 in
-  the FP32 form with the sweeps in turn (magic_set_sweep_overlap 0, the
-  batch's order of additions): the same dt of every attempt (a rejected first attempt
+  the FP32 form with the same order of the sweeps' additions (the default
+  overlap, and the sweeps in turn): the same dt of every attempt (a rejected first attempt
   included) and the same state bit for bit, with the plume forcing off (with
   it on, the forcing's time factor is evaluated by the host's exp for a
   single grid and by the device's exp in a batch: rho', T', w within 1e-5
@@ -34,10 +34,10 @@ def _scenario(k, forcing):
     return c, p, g, aux, col, q
 
 
-def _single(c, p, g, aux, q, mthbc, nsteps, dt0):
+def _single(c, p, g, aux, q, mthbc, nsteps, dt0, overlap=False):
     from magic_master_amd.hip import FORM_FP32P, MagicHIP
     d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
-    d.set_sweep_overlap(False)   # the batched driver's order of additions
+    d.set_sweep_overlap(overlap)
     dtv = [dt0, c.dtv[1], 0, 0, 0]
     t = 0.0
     log = []
@@ -51,10 +51,11 @@ def _single(c, p, g, aux, q, mthbc, nsteps, dt0):
         d.close()
 
 
-def _batch(scen, nsteps, dt0, tend=None):
+def _batch(scen, nsteps, dt0, tend=None, overlap=False):
     from magic_master_amd.hip import MagicBatch
     b = MagicBatch([s[2] for s in scen], [s[1] for s in scen], [s[4] for s in scen], [s[5] for s in scen])
     try:
+        b.set_sweep_overlap(overlap)
         for k, (c, p, g, aux, col, q) in enumerate(scen):
             mthbc = c.mthbc
             # nsteps accepted steps (an end time that is never reached), or up to tend
@@ -67,13 +68,14 @@ def _batch(scen, nsteps, dt0, tend=None):
         b.close()
 
 
-def test_batch_equals_single_grids_bitwise():
+@pytest.mark.parametrize("overlap", [True, False])
+def test_batch_equals_single_grids_bitwise(overlap):
     scen = [_scenario(k, forcing=False) for k in range(3)]
     nsteps = 6
-    print("[batch] 3 scenarios, 6 steps, first dt 50 s (rejected)", flush=True)
-    states, qb = _batch(scen, nsteps, 50.0)
+    print(f"[batch] 3 scenarios, 6 steps, first dt 50 s (rejected), overlap {overlap}", flush=True)
+    states, qb = _batch(scen, nsteps, 50.0, overlap=overlap)
     for k, (c, p, g, aux, col, q) in enumerate(scen):
-        t, dt_next, qs, log = _single(c, p, g, aux, q, c.mthbc, nsteps, 50.0)
+        t, dt_next, qs, log = _single(c, p, g, aux, q, c.mthbc, nsteps, 50.0, overlap=overlap)
         st = states[k]
         assert st["status"] == 0 and st["active"] == 0
         assert st["nsteps"] == nsteps
@@ -101,7 +103,7 @@ def test_batch_interval_stops_at_tend():
     for k, (c, p, g, aux, col, q) in enumerate(scen):
         assert states[k]["t"] == 30.0
         d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
-        d.set_sweep_overlap(False)
+        d.set_sweep_overlap(False)   # the batch's default order of additions
         try:
             te, dtv, _, nv, _ = d.claw3(0.0, 30.0, [1.0, c.dtv[1], 0, 0, 0], c.cflv, [100000, 0], c.method,
                                          c.mthbc)

@@ -13,9 +13,9 @@ magic_batch_set_tile_shapes, magic_batch_tune).
   state on every run, bit for bit, and the two agree within the FP64 bar
   after 6 claw3 steps (FP32: 1e-5 of each component's max).
 * A batch with a non-default assignment, and a batch after magic_batch_tune,
-  equals each grid stepped alone with the same shapes (and the sweeps in
-  turn, magic_set_sweep_overlap 0), bit for bit, and the tuner leaves the
-  scenarios' states unchanged.
+  equals each grid stepped alone with the same shapes and order of the
+  sweeps (in turn, the batch's default), bit for bit, and the tuner leaves
+  the scenarios' states unchanged.
 """
 import numpy as np
 import pytest
@@ -142,7 +142,7 @@ def _single(scen, shapes, nsteps, dt0):
         d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
         try:
             d.set_tile_shapes(shapes)
-            d.set_sweep_overlap(False)   # the batched driver's order of additions
+            d.set_sweep_overlap(False)   # the batch's default order of additions
             dtv = [dt0, c.dtv[1], 0, 0, 0]
             t = 0.0
             for _ in range(nsteps):

@@ -9,7 +9,7 @@ form), i.e. every sweep instantiation the library carries.
 * FP32 form against the oracle after 10 claw3 steps: w, T', rho' within 1e-4
   of their peaks, tracers within 1e-4 (their range is [0, 1]).
 * Batched driver with 0 and 2 tracers: each scenario equals the same grid
-  stepped alone (sweeps in turn), bit for bit.
+  stepped alone (sweeps in turn, the batch's default), bit for bit.
 """
 import numpy as np
 import pytest
@@ -128,7 +128,7 @@ def test_batch_tracer_counts(meqn, nspec):
         b.close()
     for k, (c, p, g, aux, col, q) in enumerate(scen):
         d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
-        d.set_sweep_overlap(False)   # the batched driver's order of additions
+        d.set_sweep_overlap(False)   # the batch's default order of additions
         try:
             dtv = [1.0, c.dtv[1], 0, 0, 0]
             t = 0.0

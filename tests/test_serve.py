@@ -28,6 +28,7 @@ def test_footprint_counts_state_scratch_and_staging():
     g = P.Grid(mx=100, my=100, mz=100, meqn=6, mbc=2, maux=32)
     cells = 104 ** 3
     fp = serve.footprint_bytes(g)
+    # q' and backup (2 x 4 B x 6), scratch (5 x 4 B), FP64 staging (8 B x 6)
     assert cells * (4 * 12 + 20 + 8 * 6) <= fp <= cells * (4 * 12 + 20 + 8 * 6) + 2e4
 
 

@@ -671,16 +671,17 @@ static int side_setup(magic_ctx *c)
         HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
+    if (c->dq_clear) return MAGIC_OK;
+    const long long n = (long long)c->meqn * c->P;
     for (int k = 0; k < 2; k++) {
-        if (c->dq[k]) continue;
-        const long long n = (long long)c->meqn * c->P;
-        if (hipMalloc(&c->dq[k], (size_t)n * sizeof(T)) != hipSuccess) {
+        if (!c->dq[k] && hipMalloc(&c->dq[k], (size_t)n * sizeof(T)) != hipSuccess) {
             c->dq[k] = nullptr;
             return MAGIC_ENOMEM;
         }
         k_fill_zero<T><<<(unsigned)std::min<long long>(blocks(n, 256), 8192), 256, 0, c->stream>>>((T *)c->dq[k], n);
         TRY(launch_check());
     }
+    c->dq_clear = true;
     return MAGIC_OK;
 }
 
@@ -705,6 +706,8 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
     const bool tile = PERT || use_tile();
     if (tile && overlap_sweeps(c)) {
         TRY(side_setup<T>(c));
+        // a launch that fails below leaves the buffers unknown: cleared again next time
+        c->dq_clear = false;
         const long long n = (long long)c->meqn * c->P;
         T *dq = (T *)c->dq[c->dq_cur], *dq_next = (T *)c->dq[c->dq_cur ^ 1];
         SweepArgsT<T> ay = a;
@@ -737,6 +740,7 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
             a.qnew, dq, n);
         TRY(launch_check());
         c->dq_cur ^= 1;
+        c->dq_clear = true;
         return MAGIC_OK;
     }
     for (int dir = 0; dir < 3; dir++) {

@@ -71,6 +71,7 @@ struct magic_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void *dq[2] = {nullptr, nullptr};
     int dq_cur = 0;
+    bool dq_clear = false;   // dq[dq_cur] holds zeros (false until set up, and after a failed overlapped step)
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps

@@ -128,8 +128,11 @@ int magic_claw3(magic_ctx *ctx, double tstart, double *tend, double dtv[5],
  * (rpn3euX.f:145-150). */
 int magic_error_location(magic_ctx *ctx, int *sweep, int *i, int *j, int *k);
 
-/* Per-kernel-class timing with HIP events on the library's stream
- * (0 = off).  Classes: 0 sweep x, 1 sweep y, 2 sweep z, 3 bc3, 4 src3,
+/* Per-kernel-class timing with HIP events on the library's streams
+ * (0 = off): each class's launches are bracketed on the stream they run on
+ * (the second stream for the y-sweep and half of the z-sweep while the
+ * sweep overlap is on, so those spans include time shared with the other
+ * stream).  Classes: 0 sweep x, 1 sweep y, 2 sweep z, 3 bc3, 4 src3,
  * 5 other.  magic_timing_read returns milliseconds and launch counts
  * accumulated since the last reset. */
 int magic_timing_enable(magic_ctx *ctx, int on);

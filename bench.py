@@ -10,9 +10,11 @@ case: 250x200x300 cells (500x400x300 km), standard temperature column,
 plume at (250, 200) km.  Inputs are resident in HBM before timing starts.
 
 Prints one JSON line: Mcell-updates/s = mx*my*mz*K / T / 1e6, the roofline
-of the dominant kernel (the sweep kernel, HIP events on the library's
-stream) and a CPU baseline (the C port of the step on the host, one core,
-on a bounded slab of the same workload).
+of the dominant kernel (the sweep kernel, HIP events on the streams it runs
+on) and a CPU baseline: the reference's own Fortran step (compiled from its
+sources into oracle/_ref where they were mounted at build time; else the C
+port of the step in oracle/), one process per host core (at most 16), each
+on a bounded slab of the same workload.
 """
 import argparse
 import json
@@ -183,26 +185,56 @@ def _cpu_slab(budget_s):
     return g.ncell * steps, el, steps
 
 
-def cpu_baseline(budget_s=20.0, workers=None):
-    """The C port (oracle) of the same step on the host's cores: one process
-    per core (at most 16, the host share of one GPU), each stepping its own
-    250 x 16 x 300 slab of the full-domain workload with the same column and
-    physics; the throughput is the sum over the processes.  Runs before the
-    process touches the GPU (the workers are forked)."""
+def _cpu_slab_reference(budget_s):
+    """One host core: the reference's own Fortran step (claw3 with step3,
+    flux3, rpn3euX/rpt3euX/rptt3euX, bc3_mpi, src3 as compiled from its
+    sources into oracle/_ref, flang -O2, one MPI rank) on the same slab;
+    its console lines go to /dev/null."""
+    from magic_master_amd import problem as P
+    from oracle.oracle import Reference
+
+    null = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(null, 1)
+    R = Reference.get()
+    c, p, g, aux, q = P.config_full(mx=250, my=16, mz=300)
+    R.setup(p, c.mthbc)
+    dtv = list(c.dtv)
+    t = 0.0
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        info, t, dtv, cflv, nv = R.claw3(g, q, aux, t, t - 1.0, dtv, c.cflv, [c.nv[0], 0], c.method, c.mthbc)
+        if info != 0:
+            raise RuntimeError(f"reference info {info}")
+        dtv[0] = dtv[4]
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    return g.ncell * steps, el, steps
+
+
+def cpu_baseline(budget_s=20.0, workers=None, kind="port"):
+    """The same step on the host's cores: one process per core (at most 16,
+    the host share of one GPU), each stepping its own 250 x 16 x 300 slab of
+    the full-domain workload with the same column and physics; the
+    throughput is the sum over the processes.  kind "port": the C port
+    (oracle); "reference": the reference's Fortran compiled into oracle/_ref.
+    Runs before the process touches the GPU (the workers are forked)."""
     import multiprocessing as mp
     n = workers or max(1, min(16, os.cpu_count() or 1))
-    if n == 1:
-        res = [_cpu_slab(budget_s)]
-    else:
-        with mp.get_context("fork").Pool(n) as pool:
-            res = pool.map(_cpu_slab, [budget_s] * n)
+    fn = _cpu_slab if kind == "port" else _cpu_slab_reference
+    with mp.get_context("fork").Pool(n) as pool:
+        res = pool.map(fn, [budget_s] * n)
     value = sum(cu / el for cu, el, _ in res) / 1e6
     steps = sum(st for _, _, st in res)
     el = max(el for _, el, _ in res)
-    return dict(value=value, unit="Mcell-updates/s", cores=n, kind="port",
-                sample=f"C port of the step (oracle/magic_oracle.c, gcc -O2), {n} processes on one core each, "
-                       f"each on a 250x16x300 slab of the full-domain workload: {steps} accepted steps in all "
-                       f"within {el:.1f} s")
+    what = ("C port of the step (oracle/magic_oracle.c, gcc -O2)" if kind == "port" else
+            "the reference's Fortran step (claw3/step3/flux3/rpn3euX/src3 compiled from its sources into "
+            "oracle/_ref, flang -O2, one MPI rank per process)")
+    return dict(value=value, unit="Mcell-updates/s", cores=n, kind=kind,
+                sample=f"{what}, {n} processes on one core each, each on a 250x16x300 slab of the full-domain "
+                       f"workload: {steps} accepted steps in all within {el:.1f} s")
 
 
 def copy_calibration():
@@ -233,6 +265,10 @@ def main():
                     help="tile shapes of the x, y, z sweeps of the single-grid run, e.g. 2,1,1 "
                          "(instead of the autotuner's choice)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-kind", default="auto", choices=["auto", "port", "reference"],
+                    help="CPU baseline: the reference's own Fortran compiled into oracle/_ref, or the C port "
+                         "of the step (oracle); auto: the reference when oracle/_ref holds it")
+    ap.add_argument("--cpu-only", action="store_true", help="only run the CPU baseline and print it")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the FP32 batched ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=20)
@@ -259,7 +295,14 @@ def main():
               "supported (the domain is not decomposed across GPUs)", file=sys.stderr)
         sys.exit(2)
     # the CPU baseline forks its workers, so it runs before anything touches the GPU
-    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_budget)
+    kind = a.cpu_kind
+    if kind == "auto":
+        from oracle.oracle import Reference
+        kind = "reference" if Reference.available() else "port"
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_budget, kind=kind)
+    if a.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu}), flush=True)
+        return
     shapes = [int(v) for v in a.shapes.split(",")] if a.shapes else None
     r = run_gpu(a.workload, a.steps, a.warmup, a.form, tune=not a.no_tune, shapes=shapes)
     g = r["g"]

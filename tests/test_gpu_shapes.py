@@ -194,11 +194,18 @@ def test_batch_tune_then_equals_single_grids():
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32"])
-def test_sweep_overlap_order(form):
+@pytest.mark.parametrize("grid", ["case0", "odd"])
+def test_sweep_overlap_order(form, grid):
     from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
-    (mx, my, mz), mthbc = CASES[0]
-    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=77, vel=0.3)
-    print(f"[sweep overlap] {form} grid {mx}x{my}x{mz}", flush=True)
+    if grid == "case0":
+        (mx, my, mz), mthbc = CASES[0]
+        c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=77, vel=0.3)
+    else:
+        # no tracer and 13 x 15 x 11 padded cells: meqn * cells = 10725 is odd,
+        # so the merge's last elements are past its 16-byte vectors
+        (mx, my, mz), mthbc = (9, 11, 7), [1, 1, 2, 2, 3, 1]
+        c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=78, vel=0.3, meqn=5, nspecadv=0)
+    print(f"[sweep overlap] {form} grid {mx}x{my}x{mz} meqn {g.meqn}", flush=True)
 
     def run(on):
         d = MagicHIP(g, p, aux, q, form=FORM_FP64 if form == "fp64" else FORM_FP32P)

@@ -98,6 +98,58 @@ def compare_forms(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, e
     return compare_runs(c, p, g, aux, q, t_end, every, (FORM_FP64, True), (FORM_FP32P, True), on_row)
 
 
+def compare_ensemble(nscen: int = 64, t_end: float = 400.0, check: int = 4, seed: int = 2024, shapes=None,
+                     on_row=None):
+    """The batched ensemble workload (problem.ensemble(nscen, seed)): every
+    scenario advanced by the batched driver in the FP32 form from rest to
+    model time t_end (claw3 intervals, clipped there), then the first
+    `check` scenarios stepped alone in the FP64 form to the same time and
+    compared.  Returns (summary of the batch run, one row per checked
+    scenario: {"scenario", "cells", "steps" (FP32, FP64), field: form_error
+    entry})."""
+    import time
+    from . import problem as P
+    from .hip import FORM_FP64, MagicBatch, MagicHIP
+    ens = P.ensemble(nscen, seed)
+    b = MagicBatch([e[2] for e in ens], [e[1] for e in ens], [e[3] for e in ens], None)
+    try:
+        if shapes is not None:
+            b.set_tile_shapes(shapes)
+        for s, (c, p, g, col) in enumerate(ens):
+            b.start(s, 0.0, t_end, c.dtv[0], c.dtv, c.cflv, 1 << 30, c.mthbc)
+        t0 = time.perf_counter()
+        attempts = b.run(max_attempts=1 << 30, check_every=8)
+        wall = time.perf_counter() - t0
+        states = [b.state(s) for s in range(nscen)]
+        q32 = [b.get_q(s) for s in range(min(check, nscen))]
+    finally:
+        b.close()
+    cell_steps = sum(e[2].mx * e[2].my * e[2].mz * st["nsteps"] for e, st in zip(ens, states))
+    summary = {"scenarios": nscen, "t_end": t_end, "attempts": attempts, "wall_s": wall,
+               "steps": [min(st["nsteps"] for st in states), max(st["nsteps"] for st in states)],
+               "rejected": sum(st["rejects"] for st in states), "mcell_updates_per_s": cell_steps / wall / 1e6}
+    rows = []
+    for s in range(min(check, nscen)):
+        c, p, g, col = ens[s]
+        aux = np.ascontiguousarray(np.broadcast_to(col[:, None, None, :], g.shape_aux))
+        d = MagicHIP(g, p, aux, P.qinit(g, aux), form=FORM_FP64)
+        try:
+            t, dtv, n64 = 0.0, list(c.dtv), 0
+            while t < t_end * (1 - 1e-12):
+                t, dtv, _, nv, _ = d.claw3(t, t_end, dtv, c.cflv, [1 << 30, 0], c.method, c.mthbc)
+                n64 += nv[1]
+                dtv[0] = dtv[4]
+            q64 = d.get_q()
+        finally:
+            d.close()
+        row = {"scenario": s, "cells": [g.mx, g.my, g.mz], "steps": [states[s]["nsteps"], n64]}
+        row.update(form_error(g, q32[s], q64, aux))
+        rows.append(row)
+        if on_row:
+            on_row(row)
+    return summary, rows
+
+
 def main(argv=None):
     """Error report of the FP32 perturbation form on the full-domain
     workload (250x200x300), one run per plume amplitude: max and RMS of
@@ -119,8 +171,24 @@ def main(argv=None):
                     help="fp32: the FP32 form against the FP64 form; order: the FP64 form with the sweeps "
                          "in turn against the FP64 form with the sweep overlap (the same arithmetic in "
                          "another order of additions: how far rounding alone moves the solution)")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="instead: the batched FP32 ensemble of this many seeded scenarios to --t-end, the "
+                         "first --check of them against the FP64 form stepped alone")
+    ap.add_argument("--check", type=int, default=4)
     a = ap.parse_args(argv)
     from .hip import FORM_FP32P, FORM_FP64
+    if a.ensemble > 0:
+        def show_s(r):
+            print(f"scenario {r['scenario']} cells {r['cells']} steps (fp32/fp64) {r['steps']}: " + " | ".join(
+                f"{k}: peak {r[k]['peak']:.3e} max/peak {r[k]['max_rel']:.2e} rms/peak {r[k]['rms_rel']:.2e}"
+                for k in ("w", "T'", "rho'")), flush=True)
+        summary, rows = compare_ensemble(a.ensemble, a.t_end, a.check, on_row=show_s)
+        print("ensemble: " + json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in summary.items()}),
+              flush=True)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump({"summary": summary, "rows": rows}, f, indent=1)
+        return
     pair = ((FORM_FP64, True), (FORM_FP32P, True)) if a.pair == "fp32" else ((FORM_FP64, True), (FORM_FP64, False))
     c, p0, g, aux, q = P.config_full()
     out = {}

@@ -99,18 +99,18 @@ def compare_forms(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, e
 
 
 def compare_ensemble(nscen: int = 64, t_end: float = 400.0, check: int = 4, seed: int = 2024, shapes=None,
-                     on_row=None):
+                     on_row=None, nmin=(64, 64, 96), nmax=(200, 200, 300)):
     """The batched ensemble workload (problem.ensemble(nscen, seed)): every
     scenario advanced by the batched driver in the FP32 form from rest to
     model time t_end (claw3 intervals, clipped there), then the first
     `check` scenarios stepped alone in the FP64 form to the same time and
-    compared.  Returns (summary of the batch run, one row per checked
-    scenario: {"scenario", "cells", "steps" (FP32, FP64), field: form_error
-    entry})."""
+    compared (nmin, nmax: the grid-size range of problem.ensemble).  Returns
+    (summary of the batch run, one row per checked scenario: {"scenario",
+    "cells", "steps" (FP32, FP64), field: form_error entry})."""
     import time
     from . import problem as P
     from .hip import FORM_FP64, MagicBatch, MagicHIP
-    ens = P.ensemble(nscen, seed)
+    ens = P.ensemble(nscen, seed, nmin=nmin, nmax=nmax)
     b = MagicBatch([e[2] for e in ens], [e[1] for e in ens], [e[3] for e in ens], None)
     try:
         if shapes is not None:

@@ -879,7 +879,7 @@ extern "C" int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3])
 // control blocks' reduction fields, which the next attempt's prepare clears.
 static int tune_rounds(magic_batch *b, int reps, TuneClock &clk)
 {
-    for (int r = 0; r <= reps; r++) {   // round 0 is not timed
+    for (int r = 0; r <= reps; r++) {   // repetition 0 warms up and is not timed
         for (int sh = 0; sh < TILE_NSHAPE; sh++) {
             for (int k = 0; k < 3; k++) b->shape[k] = sh;
             TRY(list_grids(b));

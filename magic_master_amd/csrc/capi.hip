@@ -855,7 +855,7 @@ extern "C" int magic_set_sweep_overlap(magic_ctx *c, int on)
 // backup then holds the current state).
 static int tune_rounds(magic_ctx *c, double dt, int reps, TuneClock &clk)
 {
-    for (int r = 0; r <= reps; r++) {   // round 0 is not timed
+    for (int r = 0; r <= reps; r++) {   // repetition 0 warms up and is not timed
         for (int dir = 0; dir < 3; dir++)
             for (int sh = 0; sh < TILE_NSHAPE; sh++) {
                 int nl = 0;

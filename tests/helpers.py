@@ -103,3 +103,27 @@ def rel_err_mom(a, b):
     scale[1:4] = scale[1:4].max()
     scale[scale == 0] = 1.0
     return float((np.abs(a - b).max(axis=0) / scale).max())
+
+
+# Cells a plume_window run must lie inside its edges to equal the full
+# domain's run after 10 claw3 steps (the oracle's runs of a 40- and a 64-cell
+# window agree exactly from 10 cells in; 2 more for safety).
+WINDOW_MARGIN = 12
+
+
+def plume_window(i0, j0, nx, ny, mz, full=(250, 200), amplitude=60.0):
+    """Cells i0+1..i0+nx, j0+1..j0+ny of the full-domain workload
+    (problem.config_full on full[0] x full[1] x mz) with its plume on from
+    t = 0: the same background columns, the same cell centres (the window's
+    lower edges move, dx and dy stay 2 km) and the plume at the full domain's
+    centre.  plume_window(0, 0, *full, mz) is the full domain itself."""
+    c, p, g, aux, q = P.config_full(mx=nx, my=ny, mz=mz)
+    c.xlower, c.xupper = 2e3 * i0, 2e3 * (i0 + nx)
+    c.ylower, c.yupper = 2e3 * j0, 2e3 * (j0 + ny)
+    p.xpos, p.ypos = 0.5 * 2e3 * full[0], 0.5 * 2e3 * full[1]
+    p.amplitude = amplitude
+    p.tcenter = 0.0
+    g = P.Grid.from_claw(c)
+    aux = P.setaux_column(g)
+    q = P.qinit(g, aux, tracer_blob=dict(z0=10e3, width=2e3, amp=1.0))
+    return c, p, g, aux, q

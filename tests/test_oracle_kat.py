@@ -2,7 +2,8 @@
 the reference build: textbook identities of the Roe solver and transverse
 splitting (LeVeque 2002, sec. 15.3), limiter consistency (Kemm 2011/2012
 properties), the well-balanced hydrostatic rest state, the explicit
-diffusion factor of a Fourier mode, and the claw3 dt recursion.  Each check
+diffusion factor of a Fourier mode, the claw3 dt recursion, and the domain
+of dependence the full-domain window test relies on.  Each check
 is chosen so that a dropped term, a wrong sign or a transposed index fails."""
 import math
 
@@ -11,7 +12,7 @@ import pytest
 
 from magic_master_amd import problem as P
 from oracle.oracle import COracle
-from tests.helpers import perturbed
+from tests.helpers import WINDOW_MARGIN, perturbed, plume_window
 
 
 @pytest.fixture(scope="module")
@@ -281,3 +282,32 @@ def test_claw3_dt_recursion(O):
     dt, cfl, _ = log[-1]
     assert dtv[4] == min(10.0, dt * 0.9 / cfl)
     assert t == log[-1][0]
+
+
+def test_plume_window_radius(O):
+    """After 10 claw3 steps of the plume workload, a 40-cell window and a
+    64-cell window around the plume agree exactly on every cell at least
+    WINDOW_MARGIN - 2 cells inside the smaller one's edges: the radius the
+    full-domain GPU test against an oracle window relies on.  (The 60 km top
+    keeps this quick; with the full 300 km column the two runs differ by
+    1.4e-10 of the field at the window's edge and by nothing from 10 cells
+    in.)"""
+    mz, nsteps = 60, 10
+    runs = []
+    for nw in (40, 64):
+        c, p, g, aux, q = plume_window(125 - nw // 2, 100 - nw // 2, nw, nw, mz)
+        dtv = list(c.dtv)
+        t = 0.0
+        for _ in range(nsteps):
+            st, t, dtv, _, _, _ = O.claw3(g, p, q, aux, t, t - 1.0, dtv, c.cflv, [c.nv[0], 0],
+                                          c.method, c.mthbc)
+            assert st == 0
+            dtv[0] = dtv[4]
+        runs.append((t, q[2:2 + mz, 2:2 + nw, 2:2 + nw]))
+    (ta, a), (tb, b) = runs
+    assert ta == tb
+    b = b[:, 12:52, 12:52]
+    diff = np.abs(a - b).max(axis=(0, 3))
+    r = WINDOW_MARGIN - 2
+    assert diff[r:40 - r, r:40 - r].max() == 0.0
+    assert np.abs(a[..., 3] / a[..., 0]).max() > 0.1

@@ -14,7 +14,10 @@ of the dominant kernel (the sweep kernel, HIP events on the streams it runs
 on) and a CPU baseline: the reference's own Fortran step (compiled from its
 sources into oracle/_ref where they were mounted at build time; else the C
 port of the step in oracle/), one process per host core (at most 16), each
-on a bounded slab of the same workload.
+on a bounded slab of the same workload.  `host_buffers` is the rate with q
+copied from and back to host memory around every step, as the drop-in claw3
+called from Fortran does (PCIe-inclusive; reported beside `value`, never as
+it).
 """
 import argparse
 import json
@@ -95,9 +98,22 @@ def run_gpu(workload, steps, warmup, form="fp64", tune=True, shapes=None):
         timed += len(lg.dt)
     wall = time.perf_counter() - tw0
     times = dev.timing_read()
+    dev.timing(False)
+    # The drop-in claw3 (fortran/claw3_hip.f90) hands q over in host memory
+    # on every call: q copied in, one step, q copied back into the caller's
+    # array.  Timed apart from the figure above, on the same state.
+    qh = dev.get_q()
+    inc_steps = 3
+    ti0 = time.perf_counter()
+    for _ in range(inc_steps):
+        dev.set_q(qh)
+        t, dtv, _ = dev.run_steps(c, 1, t0=t, dt0=dtv[0])
+        dev.get_q(out=qh)
+    inc_wall = time.perf_counter() - ti0
     dev.close()
     return dict(c=c, g=g, name=name, wall=wall, step_ms=step_ms, times=times, t=t, shapes=shapes, tuned=tuned,
-                attempts=len(log.dt), rejected=sum(not a for a in log.accepted), timed_attempts=timed)
+                attempts=len(log.dt), rejected=sum(not a for a in log.accepted), timed_attempts=timed,
+                inc_steps=inc_steps, inc_wall=inc_wall)
 
 
 def run_ensemble(steps, warmup, nscen=64, seed=2024, tune=True):
@@ -350,6 +366,11 @@ def main():
         "latency": {"unit": "ms per accepted step", "p50": round(float(np.percentile(st, 50)), 4),
                     "p99": round(float(np.percentile(st, 99)), 4)},
         "kernel_ms": {k: [round(v[0], 3), int(v[1])] for k, v in r["times"].items()},
+        "host_buffers": {"value": round(cells * r["inc_steps"] / r["inc_wall"] / 1e6, 3),
+                         "unit": "Mcell-updates/s", "steps": r["inc_steps"],
+                         "ms_per_step": round(r["inc_wall"] / r["inc_steps"] * 1e3, 3),
+                         "what": "q copied from and back to pageable host memory around every step, as "
+                                 "the drop-in claw3 called from Fortran does (PCIe-inclusive; not value)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": traffic_src,

@@ -194,8 +194,14 @@ class MagicHIP:
             raise ValueError(f"q shape {q.shape} != {self.g.shape_q}")
         self._check(self.L.magic_set_q(self.h, _ptr(q)), "magic_set_q")
 
-    def get_q(self) -> np.ndarray:
-        out = np.empty(self.g.shape_q)
+    def get_q(self, out: np.ndarray = None) -> np.ndarray:
+        """q in the reference layout; into `out` (C-contiguous float64 of
+        shape_q) when given, as the drop-in claw3 copies into its caller's
+        array."""
+        if out is None:
+            out = np.empty(self.g.shape_q)
+        elif out.shape != self.g.shape_q or out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be C-contiguous float64 of shape {self.g.shape_q}")
         self._check(self.L.magic_get_q(self.h, _ptr(out)), "magic_get_q")
         return out
 

@@ -33,6 +33,13 @@ sys.path.insert(0, ROOT)
 METRIC = ("Mcell-updates/s FP64 vs FP32-batched on 1 MI355X at stated error; "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
+# The step is bound by vector arithmetic, not by HBM: about 4.1 k flop per
+# interface and sweep direction (rpn3's Roe solve, entropy fix and limiters,
+# about 0.6 k; flux3's transverse and double-transverse splits and updates,
+# about 3.5 k), three directions per cell-update.  MI355X vector peaks
+# (non-matrix): FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s.
+FLOP_PER_CELL_UPDATE = 3 * 4.1e3
+VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # Algorithmic bytes per cell-update at meqn = 6.  Hyperbolic pass: read q,
 # write q (2 x 48 B) + aux(17), aux(19) (16 B), shared evenly by the three
 # sweep directions; a direction's launches (tile colours) each cover an equal
@@ -378,7 +385,12 @@ def main():
                      "launches": k_n, "sweep_launches_per_step3": launches_per_step3,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "bytes_per_cell_update": hyp_b + src_b,
-                     "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5)},
+                     "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5),
+                     "step_vector": {"flop_per_cell_update": FLOP_PER_CELL_UPDATE,
+                                     "achieved_tflops": round(value * 1e6 * FLOP_PER_CELL_UPDATE / 1e12, 3),
+                                     "peak_tflops": VECTOR_PEAK_TFLOPS[a.form],
+                                     "frac": round(value * 1e6 * FLOP_PER_CELL_UPDATE / 1e12
+                                                   / VECTOR_PEAK_TFLOPS[a.form], 4)}},
     }
     ens_shapes = None
     if not a.no_ensemble:

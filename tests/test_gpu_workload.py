@@ -41,7 +41,7 @@ def _dev(g, p, aux, q):
     return MagicHIP(g, p, aux, q)
 
 
-def MagicHIP_form(g, p, aux, q, form):
+def _dev_form(g, p, aux, q, form):
     from magic_master_amd.hip import MagicHIP
     return MagicHIP(g, p, aux, q, form=form)
 
@@ -152,7 +152,7 @@ def test_full_domain_plume_matches_oracle_window(form):
     auxw, ref = _window_run(i0, j0, nw, mz, nsteps)
     c, p, g, aux, q = plume_window(0, 0, mx, my, mz)
     print(f"[plume-window {form}] device on {mx}x{my}x{mz}", flush=True)
-    d = MagicHIP_form(g, p, aux, q, FORM_FP64 if form == "fp64" else FORM_FP32P)
+    d = _dev_form(g, p, aux, q, FORM_FP64 if form == "fp64" else FORM_FP32P)
     del q
     m = g.mbc
     r = WINDOW_MARGIN

@@ -202,7 +202,11 @@ int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx);
 int magic_batch_start(magic_batch *b, int s, double t, double tend, double dt, const double dtv[5],
                       const double cflv[4], int maxsteps, const int mthbc[6]);
 /* Replay attempts until no started scenario is active (checked every
- * check_every attempts) or max_attempts were made. */
+ * check_every attempts) or max_attempts were made.  Returns only argument
+ * and HIP errors: a scenario that fails (MAGIC_ERHO, MAGIC_EA2, or
+ * MAGIC_EUNSUPPORTED when its source pass needs more substeps than were
+ * captured) stops with that status, which magic_batch_state reports, and
+ * the other scenarios go on. */
 int magic_batch_run(magic_batch *b, int max_attempts, int check_every, int *attempts_done);
 /* real_out: t, dt (next), cfl (last attempt), cflmax, tend;
  * int_out: accepted steps, attempts, rejected attempts, active, status, ndtr. */

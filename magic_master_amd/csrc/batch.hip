@@ -94,7 +94,8 @@ __global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, in
     // src3.f90:314-322
     const int ndtr = (int)ceil((dt * c.difmax / (c.dxyzmin2)) / 0.45);
     const double dtr = dt / ndtr;
-    const int ndtrc = (int)ceil((dtr * c.kapmax / (c.dxyzmin2)) / 0.45);
+    // ndtr = 0 (no viscosity at the top): no substep runs (src3.f90:341)
+    const int ndtrc = ndtr > 0 ? (int)ceil((dtr * c.kapmax / (c.dxyzmin2)) / 0.45) : 0;
     const double dtrc = dtr / ndtrc;
     if (ndtr > c.ndtr_cap || ndtrc > c.ndtrc_cap) {   // beyond the captured substeps
         c.status = MAGIC_EUNSUPPORTED;
@@ -557,13 +558,29 @@ static int list_grids(magic_batch *b)
     return MAGIC_OK;
 }
 
-static int build_graph(magic_batch *b)
+// Retire the captured attempt: wait until no replay runs (the main stream
+// and every branch stream), then destroy the executable graph and the graph
+// itself.  Called before any device buffer its kernels address (control
+// blocks, argument blocks, work lists, a scenario's state) is freed or
+// reallocated, so that no instantiated graph ever holds a dangling pointer,
+// and before every capture.
+static int drop_graph(magic_batch *b)
 {
+    int st = MAGIC_OK;
+    if (b->stream && hipStreamSynchronize(b->stream) != hipSuccess) st = MAGIC_EHIP;
+    for (int i = 0; i < magic_batch::NSIDE; i++)
+        if (b->side[i] && hipStreamSynchronize(b->side[i]) != hipSuccess) st = MAGIC_EHIP;
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_ok = false;
+    return st;
+}
+
+static int build_graph(magic_batch *b)
+{
+    TRY(drop_graph(b));
     TRY(list_grids(b));
     if (b->overlap)
         for (magic_ctx *c : b->sc) {
@@ -600,7 +617,7 @@ static int reserve(magic_batch *b, int want)
     if (want <= b->cap) return MAGIC_OK;
     int cap = b->cap ? b->cap : 8;
     while (cap < want) cap *= 2;
-    HIPCHECK(hipStreamSynchronize(b->stream));
+    TRY(drop_graph(b));
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
     b->ctl_d = nullptr; b->args_d = nullptr; b->run_d = nullptr; b->ctl_pin = nullptr;
@@ -650,7 +667,7 @@ extern "C" int magic_batch_add(magic_batch *b, const int dims[3], const double g
 extern "C" int magic_batch_remove(magic_batch *b, int s)
 {
     if (!b || s < 0 || s >= b->n || !b->sc[s]) return MAGIC_EARG;
-    hipStreamSynchronize(b->stream);
+    TRY(drop_graph(b));
     magic_destroy(b->sc[s]);
     b->sc[s] = nullptr;
     memset(&b->ctl[s], 0, sizeof(ScenCtl));
@@ -711,9 +728,7 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
 extern "C" int magic_batch_destroy(magic_batch *b)
 {
     if (!b) return MAGIC_OK;
-    if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    drop_graph(b);
     for (auto c : b->sc)
         if (c) magic_destroy(c);
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
@@ -823,8 +838,9 @@ extern "C" int magic_batch_run(magic_batch *b, int max_attempts, int check_every
         for (int s = 0; s < b->n; s++) any = any || (b->ctl[s].active && b->ctl[s].status == 0);
     }
     if (attempts_done) *attempts_done = done;
-    for (int s = 0; s < b->n; s++)
-        if (b->sc[s] && b->ctl[s].status != 0) return b->ctl[s].status;
+    // a scenario that failed (rho <= 0, a**2 <= 0, more substeps than
+    // captured) has stopped with its status in its control block, which
+    // magic_batch_state reports; the others went on
     return MAGIC_OK;
 }
 
@@ -905,7 +921,7 @@ extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], flo
     TuneClock clk;
     if (!clk.ok) return MAGIC_EHIP;
     const int saved[3] = {b->shape[0], b->shape[1], b->shape[2]};
-    b->graph_ok = false;   // the lists and argument blocks are rebuilt before the next attempt
+    TRY(drop_graph(b));   // the lists and argument blocks are rebuilt before the next attempt
     for (int s = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (c)

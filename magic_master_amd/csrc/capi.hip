@@ -1124,7 +1124,9 @@ static int src3_launch(magic_ctx *c, double t, double dt, const int *mthbc_in, i
     const double difmax = (4.0 / 3.0) * c->aux11_top;
     const int ndtr = (int)std::ceil((dt * difmax / (dxyzmin2)) / 0.45);
     const double dtr = dt / ndtr;
-    const int ndtrc = (int)std::ceil((dtr * c->aux12_top / (dxyzmin2)) / 0.45);
+    // no viscosity at the top: ndtr = 0 and the substep loop runs no pass
+    // (src3.f90:341); the conduction count of an infinite dtr is then 0
+    const int ndtrc = ndtr > 0 ? (int)std::ceil((dtr * c->aux12_top / (dxyzmin2)) / 0.45) : 0;
     const double dtrc = dtr / ndtrc;
     *ndtr_out = ndtr;
     *ndtrc_out = ndtrc;

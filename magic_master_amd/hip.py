@@ -30,6 +30,8 @@ STATUS = {0: "ok", 1: "rho <= 0 in a Riemann problem", 2: "a**2 <= 0 in a Roe av
           3: "boundary condition needs MPI or a user routine", 4: "unsupported option",
           5: "invalid argument", 6: "HIP runtime error", 7: "device allocation failed"}
 
+MAGIC_OK, MAGIC_ERHO, MAGIC_EA2, MAGIC_EBC, MAGIC_EUNSUPPORTED = 0, 1, 2, 3, 4
+
 FORM_FP64 = 0     # MAGIC_FORM_FP64
 FORM_FP32P = 1    # MAGIC_FORM_FP32P: FP32 perturbation from a background column at rest
 
@@ -386,6 +388,8 @@ class MagicBatch:
                                              (C.c_int * 6)(*mthbc)), "magic_batch_start")
 
     def run(self, max_attempts=100000, check_every=8):
+        """Replays attempts; a failed scenario stops with its status in
+        state(s)["status"] while the others go on."""
         done = C.c_int()
         self._check(self.L.magic_batch_run(self.h, max_attempts, check_every, C.byref(done)),
                     "magic_batch_run")

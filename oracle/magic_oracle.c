@@ -1148,7 +1148,10 @@ int orc_src3(const orc_grid *g, const orc_params *p, double *q, const double *au
     double dtr = dt / ndtr;
     double conmax = AG(12, 1, 1, mz);
     double concfl = 0.45;
-    int ndtrc = (int)ceil((dtr * conmax / (dxyzmin2)) / concfl);
+    /* With no viscosity at the top (aux(11) = 0) ndtr is 0, dtr is
+       infinite and the reference's ceiling of it is undefined; its substep
+       loop (src3.f90:341) then runs no pass at all, so the count is 0 here. */
+    int ndtrc = ndtr > 0 ? (int)ceil((dtr * conmax / (dxyzmin2)) / concfl) : 0;
     double dtrc = dtr / ndtrc;
     *ndtr_out = ndtr;
     *ndtrc_out = ndtrc;

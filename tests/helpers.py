@@ -127,3 +127,11 @@ def plume_window(i0, j0, nx, ny, mz, full=(250, 200), amplitude=60.0):
     aux = P.setaux_column(g)
     q = P.qinit(g, aux, tracer_blob=dict(z0=10e3, width=2e3, amp=1.0))
     return c, p, g, aux, q
+
+
+def unshuffle_planes(b, shape):
+    """Inverse of tests/golden/make_golden.shuffle_planes: the (nz, ny, nx,
+    meqn) FP64 array stored as byte planes."""
+    nz, ny, nx, meqn = shape
+    a = np.frombuffer(np.ascontiguousarray(b.T).tobytes(), dtype="<f8").reshape(meqn, nz, ny, nx)
+    return np.ascontiguousarray(np.moveaxis(a, 0, -1))

@@ -71,6 +71,26 @@ def test_src3(case, oracle):
     assert rel_err(qg, qo, interior(g)) <= 1e-12
 
 
+def test_src3_without_viscosity(oracle):
+    """aux(11) = 0 at the top: no diffusion substep (ndtr = 0) and, with it,
+    no conduction, though aux(12) is not zero (src3.f90:314-341)."""
+    (mx, my, mz), mthbc = CASES[0]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=6, visc=3e10)
+    aux = aux.copy()
+    aux[..., 10] = 0.0
+    p.tcenter = 0.0
+    print("[src3 no viscosity]", flush=True)
+    qo = q.copy()
+    st, ndtr_o, ndtrc_o = oracle.src3(g, p, qo, aux, 0.0, 5.0, mthbc)
+    assert st == 0 and (ndtr_o, ndtrc_o) == (0, 0)
+    d = _dev(g, p, aux, q)
+    ndtr, ndtrc = d.src3(0.0, 5.0, mthbc)
+    qg = d.get_q()
+    d.close()
+    assert (ndtr, ndtrc) == (0, 0)
+    assert rel_err(qg, qo, interior(g)) <= 1e-12
+
+
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_claw3_steps(case, oracle):
     """10 accepted steps of the claw3 loop (first one rejected: dtv(1) = 50 s)."""

@@ -49,3 +49,28 @@ def test_each_result_alone_equals_in_the_mix():
         alone = serve.Server(snapshots="none", keep_final=True).run([r])["outcomes"][0]
         assert alone.steps == o.steps
         np.testing.assert_array_equal(alone.final_q, o.final_q)
+
+
+def test_failed_request_leaves_the_others_running():
+    """One request starts with a negative density in one cell: its first
+    Riemann problem there fails (MAGIC_ERHO, rpn3euX.f:140-153), it leaves the
+    batch with that status, and every other request completes as it does
+    without it."""
+    from magic_master_amd.hip import MAGIC_ERHO
+    rq = _requests()
+    ref = serve.Server(snapshots="none", keep_final=True).run(_requests())["outcomes"]
+    bad = sorted(rq, key=lambda r: r.arrival)[1]
+    g, m = bad.grid, bad.grid.mbc
+    q = np.zeros(g.shape_q)
+    q[..., 0] = bad.column[:, None, None, 0]
+    q[..., 4] = bad.column[:, None, None, 4]
+    q[m + g.mz // 2, m + g.my // 2, m + g.mx // 2, 0] = -1.0
+    bad.q0 = q
+    print(f"[serve-failure] {len(rq)} requests, request {bad.rid} fails", flush=True)
+    res = serve.Server(snapshots="none", keep_final=True).run(rq)
+    for o, o_ref in zip(res["outcomes"], ref):
+        if o.rid == bad.rid:
+            assert o.status == MAGIC_ERHO and o.final_q is None
+        else:
+            assert o.status == 0 and o.steps == o_ref.steps
+            np.testing.assert_array_equal(o.final_q, o_ref.final_q)

@@ -2,10 +2,10 @@
 
 1. Plain isothermal workload (64x64x48, problem.config_plain) for 200 claw3
    calls against the compiled reference's own run (tests/golden/plain_200.npz):
-   every step's next dt and CFL, and every 4th cell of the state after steps
-   1, 10 and 200, plus per-component sums.  Bars: 1e-12 after one step, 1e-9
-   after 200 (relative to each component's largest value; the momenta share
-   one scale).
+   every step's next dt and CFL, and every interior cell of the state after
+   steps 1, 10 and 200.  Bars: 1e-12 after one step, 1e-9 after 10 and 200
+   (relative to each component's largest value; the momenta share one
+   scale).
 2. Full-domain workload (250x200x300, problem.config_full) at its real size,
    through a size-independent property: with one horizontal axis periodic and
    the state uniform along it, every row along that axis must equal the
@@ -30,7 +30,7 @@ import pytest
 from magic_master_amd import problem as P
 from oracle.oracle import COracle
 from magic_master_amd.report import form_error
-from tests.helpers import WINDOW_MARGIN, plume_window, rel_err_mom, uniform_state
+from tests.helpers import WINDOW_MARGIN, plume_window, rel_err_mom, uniform_state, unshuffle_planes
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -50,6 +50,7 @@ def test_plain_workload_200_steps_vs_reference_run():
     z = np.load(os.path.join(GOLD, "plain_200.npz"))
     c, p, g, aux, q = P.config_plain()
     m = g.mbc
+    q0 = q[m:m + g.mz, m:m + g.my, m:m + g.mx].copy()
     print(f"[plain200] grid {g.mx}x{g.my}x{g.mz}", flush=True)
     d = _dev(g, p, aux, q)
     dtv = list(c.dtv)
@@ -65,15 +66,11 @@ def test_plain_workload_200_steps_vs_reference_run():
             if n in (1, 10, 200):
                 qg = d.get_q()
                 it = qg[m:m + g.mz, m:m + g.my, m:m + g.mx]
-                sub = qg[m:m + g.mz:4, m:m + g.my:4, m:m + g.mx:4]
+                ref = q0 + unshuffle_planes(z[f"step{n}_dq"], q0.shape)   # every interior cell of the reference's run
                 tol = 1e-12 if n == 1 else 1e-9
-                err = rel_err_mom(sub, z[f"step{n}_sub"])
-                print(f"  step {n}: field err {err:.3e}", flush=True)
+                err = rel_err_mom(it, ref)
+                print(f"  step {n}: field err {err:.3e} over {it[..., 0].size} cells", flush=True)
                 assert err <= tol, (n, err)
-                s = it.reshape(-1, g.meqn).sum(0)
-                ref = z[f"step{n}_sum"]
-                bound = np.sqrt(z[f"step{n}_sumsq"] * it[..., 0].size)   # >= sum |q|
-                assert np.all(np.abs(s - ref) <= tol * bound), (n, s, ref)
     finally:
         d.close()
     dts = np.array(dts)

@@ -107,3 +107,21 @@ def test_setaux_profile_bitwise(both, tmp_path, monkeypatch):
         slots = list(range(12)) + [17, 18]
         assert np.array_equal(mine[..., slots], ref[..., slots])
         assert np.array_equal(mine[1:, ..., 16], ref[1:, ..., 16])   # aux(17) above the lowest ghost layer
+
+
+def test_src3_without_viscosity_bitwise(both):
+    """aux(11) = 0 at the top: ndtr = 0 and src3 runs no substep, only the
+    conversion, the reassembly and the forcing (src3.f90:314-341); the
+    conduction coefficient is left non-zero."""
+    O, R = both
+    (mx, my, mz), mthbc = CASES[0]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=21, visc=1e9)
+    aux = aux.copy()
+    aux[..., 10] = 0.0
+    p.tcenter = 0.0
+    R.setup(p, mthbc)
+    q1, q2 = q.copy(), q.copy()
+    st, ndtr, ndtrc = O.src3(g, p, q1, aux, 0.0, 4.0, mthbc)
+    R.src3(g, q2, aux, 0.0, 4.0, mthbc)
+    assert st == 0 and (ndtr, ndtrc) == (0, 0)
+    assert np.array_equal(q1, q2)

@@ -123,6 +123,18 @@ int magic_claw3(magic_ctx *ctx, double tstart, double *tend, double dtv[5],
                 double cflv[4], int nv[2], const int method[7], const int mthbc[6],
                 double *log_dt, double *log_cfl, int *log_acc, int maxlog, int *nlog);
 
+/* Global sums over the interior cells in FP64 with compensated (Neumaier)
+ * summation: sums[m] = sum over i, j, k of q(m+1, i, j, k) * dx*dy*dz for
+ * m = 0 .. meqn-1 -- the mass (m = 0), the momenta, the total energy
+ * (m = 4) and the tracer masses.  In the FP32 form the background of each
+ * layer enters as an exact FP64 product and the FP32 perturbations are
+ * summed against it.  Every addition has a fixed order, so the result is
+ * the same on every run.  The first call allocates the context's partial
+ * sums (mz * meqn * 16 B).  No counterpart in the reference (it checks
+ * conservation only implicitly: periodic x and y with the fluid at rest
+ * change sum(rho) only through the z faces). */
+int magic_global_sums(magic_ctx *ctx, double *sums);
+
 /* Where the last MAGIC_ERHO / MAGIC_EA2 happened: sweep (1,2,3), the
  * Fortran indices of the pencil's cell, as the reference prints them
  * (rpn3euX.f:145-150). */

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "sweep_tile32.hip", "bc_src.hip", "bc_src32.hip", "capi.hip",
-               "batch.hip"]
+               "batch.hip", "sums.hip"]
 # the tiled sweeps are compiled once per tile shape (TILE_NSHAPE in magic_dev.h)
 PER_SHAPE = ("sweep_tile.hip", "sweep_tile32.hip")
 TILE_NSHAPE = 4
@@ -110,12 +110,13 @@ def build_fortran() -> str:
     lib = os.path.join(BIN, "libmagic_f77.so")
     _run([FC, "-shared", "-o", lib, objs[0], objs[1], "-L" + HERE, "-lmagic_hip",
           "-Wl,-rpath,$ORIGIN/.."])
-    # a caller that uses claw3 with the reference's argument list
-    chk = os.path.join(mod, "dropin_check.o")
-    _run([FC, "-c", "-O2", "-module-dir", mod, "-I", mod,
-          os.path.join(FORTRAN, "dropin_check.f90"), "-o", chk])
-    _run([FC, "-o", os.path.join(BIN, "dropin_check"), chk, "-L" + BIN, "-lmagic_f77",
-          "-L" + HERE, "-lmagic_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
+    # callers written for the reference: one that calls claw3, one that keeps
+    # its own step control and calls step3, bc3, b4step3 and src3
+    for prog in ("dropin_check", "step3_loop_check"):
+        chk = os.path.join(mod, prog + ".o")
+        _run([FC, "-c", "-O2", "-module-dir", mod, "-I", mod, os.path.join(FORTRAN, prog + ".f90"), "-o", chk])
+        _run([FC, "-o", os.path.join(BIN, prog), chk, "-L" + BIN, "-lmagic_f77",
+              "-L" + HERE, "-lmagic_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."])
     return exe
 
 

@@ -62,7 +62,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
     ok = ok && hipMalloc(&c->stage, P * meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&c->dscal, 8 * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&c->sink, 4096) == hipSuccess;
-    ok = ok && hipHostMalloc(&c->hscal, 8 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc(&c->hscal, HSCAL_WORDS * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
     if (ok && form == MAGIC_FORM_FP32P) {
         const size_t NZ = (size_t)c->d.NZ;
         ok = hipMalloc(&c->q32, P * meqn * sizeof(float)) == hipSuccess &&
@@ -122,7 +122,7 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->q); hipFree(c->qold); hipFree(c->aux); hipFree(c->zratio); hipFree(c->stage);
     hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal); hipFree(c->sink);
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
-    hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64);
+    hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64); hipFree(c->sums_d);
     if (c->side) hipStreamSynchronize(c->side);
     hipFree(c->dq[0]);
     hipFree(c->dq[1]);

@@ -32,6 +32,9 @@ using namespace magic;
         if (s_ != MAGIC_OK) return s_; \
     } while (0)
 
+// pinned host words of a context and where the global sums land in them
+constexpr int HSCAL_WORDS = 32, HSCAL_SUMS = 16;
+
 enum { AX_RHO = 0, AX_MX, AX_MY, AX_MZ, AX_E, AX_NU, AX_KAP, AX_VG, AX_R, AX_GAM, NAUXP };
 enum { TC_SWX = 0, TC_SWY, TC_SWZ, TC_BC, TC_SRC, TC_OTHER, NTC };
 
@@ -75,7 +78,8 @@ struct magic_ctx {
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps
-    unsigned long long *hscal = nullptr;   // pinned mirror
+    unsigned long long *hscal = nullptr;   // pinned host words: [0, 8) mirror dscal, [HSCAL_SUMS, +10) global sums
+    void *sums_d = nullptr;                // global sums: partials and results (first magic_global_sums)
     magic_params prm;
     bool have_params = false, have_aux = false;
     double aux11_top = 0.0, aux12_top = 0.0;

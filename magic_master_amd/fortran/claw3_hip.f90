@@ -1,8 +1,10 @@
 ! Drop-in Fortran operators on the MI355X device path.
 !
-! claw3 and step3 below keep the reference's exact dummy argument lists
-! (mpilib/claw3_mpi.f:4-7 and lib/step3.f:3-7), so a caller written for the
-! reference links against them unchanged.  They run on the GPU through the
+! claw3, step3, bc3, b4step3 and src3 below keep the reference's exact dummy
+! argument lists (mpilib/claw3_mpi.f:4-7, lib/step3.f:3-7, bc3_mpi.f:5-6,
+! b4step3.f:2-3, src3.f90:203-205), so a caller written for the reference
+! links against them unchanged: one that keeps its own claw3 and calls
+! these operators, or one that calls claw3 itself.  They run on the GPU through the
 ! C ABI (module magic_hip): the device context for the grid is created on the
 ! first call, aux is uploaded whenever a different aux array is passed, and q
 ! crosses to the device on entry and back on exit of each call.  The
@@ -19,6 +21,10 @@ module magic_hip_state
   implicit none
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: dims(6) = 0
+  ! xlower, ylower, zlower, dx, dy, dz of the context; the lower edges are
+  ! unknown while only step3 (whose arguments do not carry them) has run
+  double precision, save :: geom(6) = 0.d0
+  logical, save :: lowers_known = .false.
   integer(c_intptr_t), save :: aux_addr = 0
 contains
 
@@ -44,21 +50,33 @@ contains
   end subroutine load_params
 
   ! Create (or reuse) the device context of this grid and make sure the
-  ! device holds this aux array.
-  subroutine ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux)
+  ! device holds this aux array.  known: the caller's arguments carry the
+  ! grid's lower edges (every operator but step3); a context created without
+  ! them is created again once they are known.  The state q is not kept
+  ! between calls, so a new context loses nothing.
+  subroutine ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux, known)
     integer, intent(in) :: mx, my, mz, meqn, mbc, maux
     double precision, intent(in) :: xlower, ylower, zlower, dx, dy, dz
     double precision, intent(in), target :: aux(*)
+    logical, intent(in) :: known
     integer :: d(6)
+    logical :: same
     d = (/mx, my, mz, meqn, mbc, maux/)
-    if (c_associated(ctx) .and. any(d /= dims)) then
-       call magic_check(magic_destroy(ctx), 'magic_destroy')
-       ctx = c_null_ptr
+    if (c_associated(ctx)) then
+       same = all(d == dims) .and. geom(4) == dx .and. geom(5) == dy .and. geom(6) == dz
+       if (known) same = same .and. lowers_known .and. geom(1) == xlower .and. geom(2) == ylower &
+            .and. geom(3) == zlower
+       if (.not. same) then
+          call magic_check(magic_destroy(ctx), 'magic_destroy')
+          ctx = c_null_ptr
+       end if
     end if
     if (.not. c_associated(ctx)) then
        call magic_check(magic_create(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, &
             dx, dy, dz, ctx), 'magic_create')
        dims = d
+       geom = (/xlower, ylower, zlower, dx, dy, dz/)
+       lowers_known = known
        aux_addr = 0
     end if
     call load_params()
@@ -86,7 +104,7 @@ subroutine claw3(meqn, mwaves, mbc, mx, my, mz, maux, q, aux, xlower, ylower, zl
      write(6, *) 'claw3: the device path is the f-wave solver with rpn3 limiting only'
      stop 1
   end if
-  call ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux)
+  call ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux, .true.)
   call magic_check(magic_set_q(ctx, q), 'magic_set_q')
   st = magic_claw3(ctx, tstart, tend, dtv, cflv, nv, method, mthbc, c_null_ptr, c_null_ptr, &
        c_null_ptr, 0, nlog)
@@ -116,12 +134,63 @@ subroutine step3(maxm, meqn, mwaves, mbc, mx, my, mz, qold, qnew, aux, dx, dy, d
   dimension aux1(*), aux2(*), aux3(*), work(*)
   logical use_fwave
   real(c_double) :: c
-  call ensure_ctx(mx, my, mz, meqn, mbc, maux, 0.d0, 0.d0, 0.d0, dx, dy, dz, aux)
+  call ensure_ctx(mx, my, mz, meqn, mbc, maux, 0.d0, 0.d0, 0.d0, dx, dy, dz, aux, .false.)
   call magic_check(magic_set_q(ctx, qold), 'magic_set_q')
   call magic_check(magic_step3(ctx, dt, c), 'step3')
   call magic_check(magic_get_q(ctx, qnew), 'magic_get_q')
   cfl = c
 end subroutine step3
+
+
+! bc3: bc3_mpi.f:5-6 (same arguments): the ghost layers of q, physical
+! branches 1-3 of every face (mthbc 0 or 4 stop: they need MPI or a user
+! routine, bc3_mpi.f:100-105).
+subroutine bc3(meqn, mbc, mx, my, mz, xlower, ylower, zlower, dx, dy, dz, q, maux, aux, t, dt, mthbc)
+  use magic_hip_state
+  implicit double precision (a-h, o-z)
+  dimension q(*), aux(*), mthbc(6)
+  call ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux, .true.)
+  call magic_check(magic_set_q(ctx, q), 'magic_set_q')
+  call magic_check(magic_bc3(ctx, mthbc), 'bc3')
+  call magic_check(magic_get_q(ctx, q), 'magic_get_q')
+end subroutine bc3
+
+
+! b4step3: b4step3.f:2-3 (same arguments): tracers 6..meqn clamped to [0, 1]
+! on every cell.
+subroutine b4step3(mbc, mx, my, mz, meqn, q, xlower, ylower, zlower, dx, dy, dz, t, dt, maux, aux)
+  use magic_hip_state
+  implicit double precision (a-h, o-z)
+  dimension q(*), aux(*)
+  call ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux, .true.)
+  call magic_check(magic_set_q(ctx, q), 'magic_set_q')
+  call magic_check(magic_b4step3(ctx), 'b4step3')
+  call magic_check(magic_get_q(ctx, q), 'magic_get_q')
+end subroutine b4step3
+
+
+! src3: src3.f90:203-205 (same arguments): the source pass with diffuse.f
+! and conduct.f substeps and the plume forcing, evaluated at t over dt.
+! Prints the substep line the reference prints (src3.f90:370-374).
+subroutine src3(meqn, mbc, mx, my, mz, xlower, ylower, zlower, dx, dy, dz, q, maux, aux, t, dt, mthbc)
+  use magic_hip_state
+  implicit double precision (a-h, o-z)
+  dimension q(*), mthbc(6)
+  dimension aux(maux, 1-mbc:mx+mbc, 1-mbc:my+mbc, 1-mbc:mz+mbc)
+  integer(c_int) :: ndtr, ndtrc
+  call ensure_ctx(mx, my, mz, meqn, mbc, maux, xlower, ylower, zlower, dx, dy, dz, aux, .true.)
+  call magic_check(magic_set_q(ctx, q), 'magic_set_q')
+  call magic_check(magic_src3(ctx, t, dt, mthbc, ndtr, ndtrc), 'src3')
+  call magic_check(magic_get_q(ctx, q), 'magic_get_q')
+  ! the Courant numbers of the substeps, as src3.f90:313-325 forms them
+  dxyzmin2 = min(dx, dy, dz)**2
+  difcfl = 0.d0
+  concfl = 0.d0
+  if (ndtr > 0) difcfl = (4.d0/3.d0)*aux(11, 1, 1, mz)*(dt/ndtr)/dxyzmin2
+  if (ndtrc > 0) concfl = aux(12, 1, 1, mz)*((dt/ndtr)/ndtrc)/dxyzmin2
+  write(6, 601) ndtr, ndtrc, difcfl, concfl
+601 format('SRC3...  Vis. Steps', i5, '  Con. Sub-steps', i5, ' VisCFL =', f6.3, ' ConCFL =', f6.3)
+end subroutine src3
 
 
 ! setprob.data reader: fills the commons in the order of setprob.f:31-61 and

@@ -43,7 +43,7 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
            "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
            "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune",
-           "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap"]
+           "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap", "magic_global_sums"]
 
 TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
 # (pencils per tile, lane order) of each tile shape
@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
     L.magic_batch_tune.argtypes = [vp, C.c_int, _ip, _fp]
     L.magic_set_sweep_overlap.argtypes = [vp, C.c_int]
     L.magic_batch_set_sweep_overlap.argtypes = [vp, C.c_int]
+    L.magic_global_sums.argtypes = [vp, _dp]
     _LIB = L
     return L
 
@@ -205,6 +206,13 @@ class MagicHIP:
         elif out.shape != self.g.shape_q or out.dtype != np.float64 or not out.flags.c_contiguous:
             raise ValueError(f"out must be C-contiguous float64 of shape {self.g.shape_q}")
         self._check(self.L.magic_get_q(self.h, _ptr(out)), "magic_get_q")
+        return out
+
+    def global_sums(self) -> np.ndarray:
+        """Compensated FP64 sums of every component over the interior cells
+        times the cell volume (mass, momenta, total energy, tracer masses)."""
+        out = np.zeros(self.g.meqn)
+        self._check(self.L.magic_global_sums(self.h, _ptr(out)), "magic_global_sums")
         return out
 
     # -- operators (reference signatures in include/magic_hip.h) ----------
@@ -381,6 +389,12 @@ class MagicBatch:
         q = np.empty(self.grids[s].shape_q)
         self._check(self.L.magic_get_q(self.ctx[s], _ptr(q)), "magic_get_q")
         return q
+
+    def global_sums(self, s) -> np.ndarray:
+        """Compensated FP64 sums of scenario s's state (MagicHIP.global_sums)."""
+        out = np.zeros(self.grids[s].meqn)
+        self._check(self.L.magic_global_sums(self.ctx[s], _ptr(out)), "magic_global_sums")
+        return out
 
     def start(self, s, t, tend, dt, dtv, cflv, maxsteps, mthbc):
         self._check(self.L.magic_batch_start(self.h, s, t, tend, dt, (C.c_double * 5)(*dtv),

@@ -6,7 +6,8 @@ density perturbation rho' = rho - rho_bar, the temperature perturbation
 T' = T - T_bar with T = (gamma-1)(E - |m|^2/(2 rho)) / (rho R) (the state
 equation of src3.f90:242-304), and the vertical velocity w = m_z / rho.
 Errors are reported as max and RMS of the difference, absolute and
-relative to the field's peak |value| in the FP64 run.
+relative to the field's peak |value| in the FP64 run, next to both runs'
+global mass and energy (compensated FP64 sums over the interior).
 """
 from __future__ import annotations
 
@@ -45,6 +46,17 @@ def form_error(g: Grid, q: np.ndarray, q_ref: np.ndarray, aux: np.ndarray) -> di
     return out
 
 
+def sums_row(ref: np.ndarray, other: np.ndarray) -> dict:
+    """The compensated FP64 global mass and energy (magic_global_sums: kg and
+    J over the interior) of two runs and the relative difference of the
+    second from the first."""
+    out = {}
+    for name, m in (("mass", 0), ("energy", 4)):
+        r, o = float(ref[m]), float(other[m])
+        out[name] = {"ref": r, "other": o, "rel": (o - r) / r if r != 0 else float("inf")}
+    return out
+
+
 # Peaks below which a field is at round-off level (fluid at rest): w in m/s,
 # T' in K, rho' in kg/m^3.
 PEAK_FLOOR = {"w": 1e-6, "T'": 1e-6, "rho'": 1e-9}
@@ -80,6 +92,7 @@ def compare_runs(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, ev
                     dtv[n][0] = dtv[n][4]
             row = {"t": t_out, "steps": list(steps)}
             row.update(form_error(g, devs[1].get_q(), devs[0].get_q(), aux))
+            row.update(sums_row(devs[0].global_sums(), devs[1].global_sums()))
             rows.append(row)
             if on_row:
                 on_row(row)
@@ -122,6 +135,7 @@ def compare_ensemble(nscen: int = 64, t_end: float = 400.0, check: int = 4, seed
         wall = time.perf_counter() - t0
         states = [b.state(s) for s in range(nscen)]
         q32 = [b.get_q(s) for s in range(min(check, nscen))]
+        s32 = [b.global_sums(s) for s in range(min(check, nscen))]
     finally:
         b.close()
     cell_steps = sum(e[2].mx * e[2].my * e[2].mz * st["nsteps"] for e, st in zip(ens, states))
@@ -140,10 +154,12 @@ def compare_ensemble(nscen: int = 64, t_end: float = 400.0, check: int = 4, seed
                 n64 += nv[1]
                 dtv[0] = dtv[4]
             q64 = d.get_q()
+            s64 = d.global_sums()
         finally:
             d.close()
         row = {"scenario": s, "cells": [g.mx, g.my, g.mz], "steps": [states[s]["nsteps"], n64]}
         row.update(form_error(g, q32[s], q64, aux))
+        row.update(sums_row(s64, s32[s]))
         rows.append(row)
         if on_row:
             on_row(row)
@@ -181,7 +197,8 @@ def main(argv=None):
         def show_s(r):
             print(f"scenario {r['scenario']} cells {r['cells']} steps (fp32/fp64) {r['steps']}: " + " | ".join(
                 f"{k}: peak {r[k]['peak']:.3e} max/peak {r[k]['max_rel']:.2e} rms/peak {r[k]['rms_rel']:.2e}"
-                for k in ("w", "T'", "rho'")), flush=True)
+                for k in ("w", "T'", "rho'")) + f" | mass {r['mass']['rel']:+.2e} energy {r['energy']['rel']:+.2e}",
+                flush=True)
         summary, rows = compare_ensemble(a.ensemble, a.t_end, a.check, on_row=show_s)
         print("ensemble: " + json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in summary.items()}),
               flush=True)
@@ -199,12 +216,13 @@ def main(argv=None):
             p.tcenter = a.tcenter
         print(f"amplitude {amp} mm/10 min, plume peak at t = {p.tcenter} s", flush=True)
         print("   t(s) steps(ref/other) | w: peak(m/s) max/peak rms/peak | T': peak(K) max/peak rms/peak | "
-              "rho': peak(kg/m3) max/peak rms/peak", flush=True)
+              "rho': peak(kg/m3) max/peak rms/peak | mass(kg) other/ref-1 | energy(J) other/ref-1", flush=True)
 
         def show(r):
             print(f"{r['t']:7.0f} {r['steps'][0]:5d}/{r['steps'][1]:5d} | " + " | ".join(
-                f"{r[k]['peak']:.3e} {r[k]['max_rel']:.2e} {r[k]['rms_rel']:.2e}" for k in ("w", "T'", "rho'")),
-                flush=True)
+                f"{r[k]['peak']:.3e} {r[k]['max_rel']:.2e} {r[k]['rms_rel']:.2e}" for k in ("w", "T'", "rho'"))
+                + f" | {r['mass']['ref']:.10e} {r['mass']['rel']:+.2e} | {r['energy']['ref']:.10e} "
+                  f"{r['energy']['rel']:+.2e}", flush=True)
         out[amp] = compare_runs(c, p, g, aux, q, a.t_end, a.every, pair[0], pair[1], on_row=show)
         # a field whose FP64 peak is at round-off level (the fluid still at
         # rest) has no meaningful relative error; the worst case is taken over

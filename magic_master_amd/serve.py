@@ -13,8 +13,9 @@ or kept in memory), and the next interval starts; at its end time the
 scenario leaves the batch and its memory is freed.
 
 Reported per request: arrival, admission and completion times (queue wait
-and time to completion), accepted and rejected steps; for the run: p50/p99
-of both, and aggregate cell-updates per second over the run.
+and time to completion), accepted and rejected steps, and the global mass
+and energy at every snapshot (compensated FP64 sums); for the run: p50/p99
+of both times, and aggregate cell-updates per second over the run.
 """
 from __future__ import annotations
 
@@ -52,6 +53,7 @@ class Outcome:
     steps: int = 0
     rejects: int = 0
     snapshots: List[tuple] = field(default_factory=list)   # (model time, path or None)
+    sums: List[tuple] = field(default_factory=list)        # (model time, mass kg, energy J) per snapshot
     final_q: Optional[np.ndarray] = None
     status: int = 0
 
@@ -198,6 +200,8 @@ class Server:
                         from . import h5
                         path = os.path.join(self.outdir, f"req{r.rid:05d}", h5.frame_name(frame))
                     out[r.rid].snapshots.append((st["t"], path))
+                    gs = batch.global_sums(s)   # compensated FP64 sums of the full state
+                    out[r.rid].sums.append((st["t"], float(gs[0]), float(gs[4])))
                     if st["t"] >= r.t_end:
                         if self.keep_final:
                             out[r.rid].final_q = qh

@@ -1,5 +1,7 @@
 """The Fortran host on the GPU: (1) a caller that uses the drop-in claw3 with
-the reference's argument list (magic_master_amd/bin/dropin_check), and (2)
+the reference's argument list (magic_master_amd/bin/dropin_check), (1b) a
+caller that keeps its own step control and calls the drop-in step3, bc3,
+b4step3 and src3 (magic_master_amd/bin/step3_loop_check), and (2)
 the claw3ez_hip driver reading claw3ez.data/setprob.data and writing the
 reference's HDF5 frames; both against the C oracle on the same inputs."""
 import os
@@ -45,6 +47,42 @@ def test_dropin_claw3_matches_oracle(tmp_path):
                                       [1, 2, 22, 0, 1, 0, g.maux], mthbc)
         assert st == 0
         dtv[0] = dtv[4]
+    assert rel_err(qg, qo, interior(g)) <= 1e-10
+
+
+def test_dropin_step3_loop_matches_oracle(tmp_path):
+    """A caller that keeps its own claw3 step control and calls the drop-in
+    step3, bc3, b4step3 and src3 with the reference's argument lists
+    (magic_master_amd/bin/step3_loop_check): the same attempted steps (a
+    rejected first one included) and the same state as the oracle's claw3."""
+    (mx, my, mz), mthbc = (12, 10, 9), [2, 2, 1, 1, 3, 1]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=8, vel=0.05)
+    p.write(tmp_path / "setprob.data")
+    with open(tmp_path / "dims.txt", "w") as f:
+        f.write(f"{mx} {my} {mz} {g.meqn} {g.mbc} {g.maux}\n")
+        f.write(f"{g.xlower!r} {g.ylower!r} {g.zlower!r} {g.dx!r} {g.dy!r} {g.dz!r}\n")
+        f.write("4 50.0\n" + " ".join(map(str, mthbc)) + "\n" + str(tmp_path / "setprob.data") + "\n")
+    aux.tofile(tmp_path / "aux.bin")
+    q.tofile(tmp_path / "q.bin")
+    out = _run([os.path.join(BIN, "step3_loop_check"), "dims.txt", "aux.bin", "q.bin", "qout.bin"], tmp_path)
+    attempts = [ln.split() for ln in out.splitlines() if len(ln.split()) == 4 and ln.split()[3] in ("T", "F")]
+    assert "SRC3...  Vis. Steps" in out
+    qg = np.fromfile(tmp_path / "qout.bin").reshape(g.shape_q)
+    O = COracle()
+    qo = q.copy()
+    dtv = [50.0, 10.0, 0, 0, 0]
+    t = 0.0
+    log = []
+    for _ in range(4):
+        st, t, dtv, _, _, lg = O.claw3(g, p, qo, aux, t, t - 1.0, dtv, [0.95, 0.9, 0, 0], [10000, 0],
+                                       [1, 2, 22, 0, 1, 0, g.maux], mthbc)
+        assert st == 0
+        log += lg
+        dtv[0] = dtv[4]
+    assert len(attempts) == len(log) and not log[0][2]     # the first attempt is rejected
+    for a, (dt_o, cfl_o, acc_o) in zip(attempts, log):
+        assert abs(float(a[1]) - dt_o) <= 1e-12 * dt_o and (a[3] == "T") == acc_o
+        assert abs(float(a[2]) - cfl_o) <= 1e-12 * cfl_o
     assert rel_err(qg, qo, interior(g)) <= 1e-10
 
 

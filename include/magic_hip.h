@@ -134,6 +134,14 @@ int magic_claw3(magic_ctx *ctx, double tstart, double *tend, double dtv[5],
  * conservation only implicitly: periodic x and y with the fluid at rest
  * change sum(rho) only through the z faces). */
 int magic_global_sums(magic_ctx *ctx, double *sums);
+/* magic_global_sums without allocating anything: the same kernels and the
+ * same sums, bit for bit, with the partial sums in a static device array of
+ * the library instead of context memory.  No counterpart in the reference;
+ * for callers that cannot allocate device memory (HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Limit: mz <= 1024 interior
+ * layers (MAGIC_EARG beyond it).  Calls that overlap on one GPU race on the
+ * array. */
+int magic_global_sums_noalloc(magic_ctx *ctx, double *sums);
 
 /* Where the last MAGIC_ERHO / MAGIC_EA2 happened: sweep (1,2,3), the
  * Fortran indices of the pencil's cell, as the reference prints them

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 HIP_SOURCES = ["sweep.hip", "sweep_tile.hip", "sweep_tile32.hip", "bc_src.hip", "bc_src32.hip", "capi.hip",
-               "batch.hip", "sums.hip"]
+               "batch.hip", "sums.hip", "sums_noalloc.hip"]
 # the tiled sweeps are compiled once per tile shape (TILE_NSHAPE in magic_dev.h)
 PER_SHAPE = ("sweep_tile.hip", "sweep_tile32.hip")
 TILE_NSHAPE = 4

@@ -43,7 +43,8 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_batch_ctx", "magic_batch_start", "magic_batch_run", "magic_batch_state",
            "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
            "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune",
-           "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap", "magic_global_sums"]
+           "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap", "magic_global_sums",
+           "magic_global_sums_noalloc"]
 
 TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
 # (pencils per tile, lane order) of each tile shape
@@ -130,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     L.magic_set_sweep_overlap.argtypes = [vp, C.c_int]
     L.magic_batch_set_sweep_overlap.argtypes = [vp, C.c_int]
     L.magic_global_sums.argtypes = [vp, _dp]
+    L.magic_global_sums_noalloc.argtypes = [vp, _dp]
     _LIB = L
     return L
 

@@ -195,17 +195,24 @@ int magic_tune(magic_ctx *ctx, double dt, int reps, int shapes_out[3], float *ms
  * variable MAGIC_SWEEP_OVERLAP=0 turns the overlap off for every context. */
 int magic_set_sweep_overlap(magic_ctx *ctx, int on);
 
-/* ---- Batched driver (FP32 form): many grids advanced together ----------
+/* ---- Batched driver: many grids of one form advanced together ----------
  * Each scenario is a grid with its own state (a magic_ctx reachable through
  * magic_batch_ctx for parameters, aux and q); its step control is claw3's
  * (claw3_mpi.f:736-927) but lives on the device, and one attempted step of
  * every scenario is one replay of a HIP graph captured at the first
- * magic_batch_run.  The sweeps of all scenarios run as one persistent launch
- * per direction and tile colour over a work list of tiles.  No counterpart in
- * the reference. */
+ * magic_batch_run.  Each kernel of the attempt is one launch over all
+ * scenarios (the sweeps as persistent launches per direction and tile
+ * colour over a work list of tiles, the boundary, copy and source-pass
+ * kernels over the scenarios' grids laid end to end), so the launches per
+ * attempt do not grow with the number of scenarios.  No counterpart in the
+ * reference. */
 typedef struct magic_batch magic_batch;
 /* dims: (mx, my, mz) per scenario; geom: (xlower, ylower, zlower, dx, dy, dz)
- * per scenario (nscen may be 0); form must be MAGIC_FORM_FP32P. */
+ * per scenario (nscen may be 0); form: MAGIC_FORM_FP32P or MAGIC_FORM_FP64
+ * (every scenario of a batch has the batch's form).  An FP64 scenario's
+ * background must be constant on each layer (magic_set_aux_column, or
+ * magic_set_aux with such planes): magic_batch_start returns
+ * MAGIC_EUNSUPPORTED otherwise. */
 int magic_batch_create(int nscen, const int *dims, int meqn, int mbc, int maux, const double *geom,
                        int form, magic_batch **out);
 int magic_batch_destroy(magic_batch *b);
@@ -218,7 +225,9 @@ int magic_batch_remove(magic_batch *b, int s);
 int magic_batch_ctx(magic_batch *b, int s, magic_ctx **ctx);
 /* Start an interval of scenario s: from t to tend (clipped like claw3 when
  * t < tend) with first dt, dtv(2) = dtv[1] as the largest dt, cflv(1:2),
- * at most maxsteps accepted steps, boundary conditions mthbc. */
+ * at most maxsteps accepted steps, boundary conditions mthbc.  The source
+ * pass's substep counts are bounded from max(dt, dtv[1]); a start whose
+ * bound exceeds those of the captured graph captures it again. */
 int magic_batch_start(magic_batch *b, int s, double t, double tend, double dt, const double dtv[5],
                       const double cflv[4], int maxsteps, const int mthbc[6]);
 /* Replay attempts until no started scenario is active (checked every

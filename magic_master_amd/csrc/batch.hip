@@ -1,21 +1,28 @@
-// Batched driver: many grids (scenarios) of the FP32 perturbation form
-// advanced together on one stream, with claw3's step control on the device
-// (batch.h) and one attempted step of every scenario captured once as a HIP
-// graph and replayed.
+// Batched driver: many grids (scenarios) of one form (the FP32 perturbation
+// form or the FP64 form) advanced together on one stream, with claw3's step
+// control on the device (batch.h) and one attempted step of every scenario
+// captured once as a HIP graph and replayed.
 //
-// One attempt, for every scenario whose control block says it runs:
+// One attempt:
 //   prepare  (dt clip, per-grid sweep factors, source-pass substep counts)
-//   bc3, b4step3, copyq3 backup                 (per scenario, gated)
-//   3 directions x 6 tile colours of the sweep  (one persistent launch each
-//                                                over all scenarios' tiles)
+//   bc3 (x, y, z faces), b4step3, copyq3 backup
+//   3 directions x 6 tile colours of the sweep  (persistent work-list launches)
 //   decide   (accept / reject from the Courant number)
-//   restore of rejected scenarios, src3 of accepted ones (gated)
+//   restore of rejected scenarios, src3 of accepted ones
 //   finalize (next dt, step count, end of interval)
-// No host synchronisation inside an attempt; the host reads the control
-// blocks back every few attempts to see whether all scenarios are done.
+// Every launch covers all scenarios: the sweeps through tile work lists,
+// the other kernels through block lists that lay the scenarios' own grids
+// end to end (batch_ops.h), each scenario gated by its control block.  The
+// number of launches per attempt does not depend on the number of
+// scenarios; the source pass's substep loops run to the largest substep
+// counts any scenario was started with (each scenario's own counts gate its
+// part).  No host synchronisation inside an attempt; the host reads the
+// control blocks back every few attempts to see whether all scenarios are
+// done.
 #include <algorithm>
 
 #include "batch.h"
+#include "batch_ops.h"
 #include "ctx.h"
 
 using namespace magic;
@@ -25,22 +32,12 @@ struct magic_batch {
     int cap = 0;                         // slots the device arrays hold
     int meqn = 0, mbc = 0, maux = 0, form = 0;
     hipStream_t stream = nullptr;
-    // The per-scenario kernel chains of an attempt (boundaries and backup,
-    // then restore and source pass) are independent across scenarios: the
-    // capture spreads them over nside side streams (graph branches, at most
-    // NSIDE; MAGIC_BATCH_BRANCHES sets the count), so small scenarios'
-    // kernels overlap instead of running one after the other.
-    static constexpr int NSIDE = 16;
-    int nside = 4;
-    hipStream_t side[NSIDE] = {};
-    hipEvent_t fork_ev[2] = {}, join_ev[2][NSIDE] = {};
-    hipStream_t cs = nullptr;            // stream of the per-scenario launches being recorded
     std::vector<magic_ctx *> sc;         // nullptr: empty slot
     std::vector<ScenCtl> ctl;            // host copy of the control blocks
     std::vector<int> mthbc;              // 6 per scenario
     ScenCtl *ctl_d = nullptr;
     ScenCtl *ctl_pin = nullptr;          // pinned read-back buffer
-    SweepArgsT<float> *args_d = nullptr; // 18 per scenario: direction x colour
+    void *args_d = nullptr;              // SweepArgsT of the form, 18 per slot: direction x colour
     const int **run_d = nullptr;         // per argument block: &ctl_d[s].run
     int4 *items_d = nullptr;
     int item_off[19] = {0};              // work list of direction d, colour k at [item_off[6d+k], ..)
@@ -48,15 +45,22 @@ struct magic_batch {
     int shape[3] = {0, 0, 0};            // tile shape of the x-, y- and z-sweeps
     int grid[3] = {0, 0, 0};             // persistent grid size of each direction's kernel
     int *one_d = nullptr;                // a device 1: the run flag of every tile while tuning
+    // boundary, copy and source-pass kernels (batch_ops.h)
+    void *ops_d = nullptr;               // BatchOps of the form, one per slot
+    int *loff_d = nullptr;               // NLG block lists: cap + 1 offsets each
+    int4 *lgdim_d = nullptr;             //                  cap grid shapes each
+    long long ltotal[NLG] = {0};         // blocks of each list
+    int ndtr_cap = 0, ndtrc_cap = 0;     // substep loops of the captured attempt
     // Sweep overlap, as for a single grid (off by default): the y-sweep and
     // the last three colour launches of the z-sweep add into each
-    // scenario's dq (ctx dq[0]) on side[0], the rest into q on the main
-    // stream; each scenario then adds dq into q and clears it.  The same
-    // order of additions as a single grid with the overlap on.  The batch's
-    // persistent launches leave few compute units idle, and on the
+    // scenario's dq (ctx dq[0]) on a second stream, the rest into q on the
+    // main stream; each scenario then adds dq into q and clears it.  The
+    // same order of additions as a single grid with the overlap on.  The
+    // batch's persistent launches leave few compute units idle, and on the
     // 64-scenario ensemble the merge's traffic outweighs the gain (3 %
     // slower), hence the default.
     bool overlap = false;
+    hipStream_t side = nullptr;
     hipEvent_t sw_fork = nullptr, sw_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -64,8 +68,17 @@ struct magic_batch {
     bool graph_ok = false;
 };
 
+static bool fp32(const magic_batch *b) { return b->form == MAGIC_FORM_FP32P; }
+static size_t elt(const magic_batch *b) { return fp32(b) ? sizeof(float) : sizeof(double); }
+static void *state_q(const magic_ctx *c) { return c->form == MAGIC_FORM_FP32P ? (void *)c->q32 : (void *)c->q; }
+static void *state_qold(const magic_ctx *c)
+{
+    return c->form == MAGIC_FORM_FP32P ? (void *)c->qold32 : (void *)c->qold;
+}
+
 // ------------------------------------------------------- control kernels --
-__global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, int *counters)
+template <typename T>
+__global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<T> *args, int *counters)
 {
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (s < 18) counters[s] = 0;
@@ -85,8 +98,7 @@ __global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<float> *args, in
     const double dt = c.dt;
     const double dtdx = dt / c.dx, dtdy = dt / c.dy, dtdz = dt / c.dz;
     for (int k = 0; k < 6; k++) {
-        SweepArgsT<float> &ax = args[s * 18 + 0 * 6 + k], &ay = args[s * 18 + 1 * 6 + k],
-                          &az = args[s * 18 + 2 * 6 + k];
+        SweepArgsT<T> &ax = args[s * 18 + 0 * 6 + k], &ay = args[s * 18 + 1 * 6 + k], &az = args[s * 18 + 2 * 6 + k];
         ax.dtdn = dtdx; ax.fg = dtdy; ax.fh = dtdz;
         ay.dtdn = dtdy; ay.fg = dtdz; ay.fh = dtdx;
         az.dtdn = dtdz; az.fg = dtdx; az.fh = dtdy;
@@ -178,7 +190,9 @@ __global__ void k_batch_finalize(ScenCtl *ctl, int n)
 }
 
 // --------------------------------------------------------------- helpers --
-static int src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
+// Substep counts of the source pass for any dt up to dtbound
+// (src3.f90:314-322 at the largest dt).
+static void src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
 {
     double dmin = c->dx < c->dy ? c->dx : c->dy;
     dmin = dmin < c->dz ? dmin : c->dz;
@@ -189,263 +203,184 @@ static int src_caps(const magic_ctx *c, double dtbound, int *ndtr, int *ndtrc)
     // the conduction count is largest for the largest dtr, i.e. dt / 1
     *ndtrc = (int)std::ceil((dtbound * c->aux12_top / d2) / 0.45);
     if (*ndtrc < 1) *ndtrc = 1;
-    return MAGIC_OK;
 }
 
-template <int NTR>
-static int launch_lists(magic_batch *b, int dir, int k, hipStream_t st)
+// tracers 0, 1, 2 and 5 (the sweep kernels' instantiations)
+template <typename T, bool PERT>
+static const void *list_fn(const magic_batch *b, int dir)
 {
-    int lo = b->item_off[6 * dir + k], n = b->item_off[6 * dir + k + 1] - lo;
-    if (n <= 0) return MAGIC_OK;
-    const void *fn = tile_list_fn<NTR>(b->shape[dir]);
-    if (!fn || b->grid[dir] < 1) return MAGIC_EARG;
-    const SweepArgsT<float> *args = b->args_d;
-    const int4 *items = b->items_d + lo;
-    const int *const *run = b->run_d;
-    int *cnt = b->counters_d + 6 * dir + k;
-    void *kargs[] = {&args, &items, &n, &run, &cnt};
-    HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, st));
-    return MAGIC_OK;
+    switch (b->meqn - 5) {
+    case 0: return tile_list_fn<T, PERT, 0>(b->shape[dir]);
+    case 1: return tile_list_fn<T, PERT, 1>(b->shape[dir]);
+    case 2: return tile_list_fn<T, PERT, 2>(b->shape[dir]);
+    case 5: return tile_list_fn<T, PERT, 5>(b->shape[dir]);
+    default: return nullptr;
+    }
 }
 
 // colour launches k0 .. k1-1 of direction dir on stream st
+template <typename T, bool PERT>
 static int launch_dir(magic_batch *b, int dir, hipStream_t st = nullptr, int k0 = 0, int k1 = 6)
 {
     if (!st) st = b->stream;
+    const void *fn = list_fn<T, PERT>(b, dir);
+    if (!fn || b->grid[dir] < 1) return MAGIC_EARG;
     for (int k = k0; k < k1; k++) {
-        switch (b->meqn - 5) {
-        case 0: TRY(launch_lists<0>(b, dir, k, st)); break;
-        case 1: TRY(launch_lists<1>(b, dir, k, st)); break;
-        case 2: TRY(launch_lists<2>(b, dir, k, st)); break;
-        case 5: TRY(launch_lists<5>(b, dir, k, st)); break;
-        default: return MAGIC_EUNSUPPORTED;
-        }
+        const int lo = b->item_off[6 * dir + k];
+        int n = b->item_off[6 * dir + k + 1] - lo;
+        if (n <= 0) continue;
+        const SweepArgsT<T> *args = (const SweepArgsT<T> *)b->args_d;
+        const int4 *items = b->items_d + lo;
+        const int *const *run = b->run_d;
+        int *cnt = b->counters_d + 6 * dir + k;
+        void *kargs[] = {&args, &items, &n, &run, &cnt};
+        HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, st));
         TRY(launch_check());
     }
     return MAGIC_OK;
 }
 
+template <typename T, bool PERT>
 static int launch_sweeps(magic_batch *b)
 {
     if (b->overlap) {
         HIPCHECK(hipEventRecord(b->sw_fork, b->stream));
-        HIPCHECK(hipStreamWaitEvent(b->side[0], b->sw_fork, 0));
-        TRY(launch_dir(b, 1, b->side[0]));
-        TRY(launch_dir(b, 2, b->side[0], 3, 6));
-        TRY(launch_dir(b, 0));
-        TRY(launch_dir(b, 2, b->stream, 0, 3));
-        HIPCHECK(hipEventRecord(b->sw_join, b->side[0]));
+        HIPCHECK(hipStreamWaitEvent(b->side, b->sw_fork, 0));
+        TRY((launch_dir<T, PERT>(b, 1, b->side)));
+        TRY((launch_dir<T, PERT>(b, 2, b->side, 3, 6)));
+        TRY((launch_dir<T, PERT>(b, 0)));
+        TRY((launch_dir<T, PERT>(b, 2, b->stream, 0, 3)));
+        HIPCHECK(hipEventRecord(b->sw_join, b->side));
         HIPCHECK(hipStreamWaitEvent(b->stream, b->sw_join, 0));
         return MAGIC_OK;
     }
-    for (int dir = 0; dir < 3; dir++) TRY(launch_dir(b, dir));
+    for (int dir = 0; dir < 3; dir++) TRY((launch_dir<T, PERT>(b, dir)));
     return MAGIC_OK;
 }
 
-// q += dq, dq = 0 (n elements, both 16-byte aligned) for a scenario whose
-// sweeps ran in this attempt; the additions round as a single grid's merge
-__global__ void k_batch_merge(float *__restrict__ q, float *__restrict__ dq, long long n, StepGate g)
+// Workgroup shape of the kernels that use block list lg
+static dim3 list_block(int lg)
 {
-    if (!g.on()) return;
-    const long long nv = n / 4;
-    float4 *qv = reinterpret_cast<float4 *>(q);
-    float4 *dv = reinterpret_cast<float4 *>(dq);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (long long i = t0; i < nv; i += stride) {
-        float4 a = qv[i];
-        const float4 b = dv[i];
-        a.x = a.x + b.x;
-        a.y = a.y + b.y;
-        a.z = a.z + b.z;
-        a.w = a.w + b.w;
-        qv[i] = a;
-        dv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    const long long r = nv * 4 + t0;   // the last n mod 4 elements
-    if (r < n) {
-        q[r] = q[r] + dq[r];
-        dq[r] = 0.f;
+    switch (lg) {
+    case LG_ALL: case LG_CPY: case LG_CP1: case LG_MRG: return dim3(256);
+    case LG_PAD: case LG_DIF: case LG_INT: return cell_block();
+    default: return dim3(64);
     }
 }
 
-static StepGate gate(const magic_batch *b, int s, int mode, int nn = 0, int kk = 0, int fslot = -1)
+// One launch of kernel class op over block list lg (several when the list
+// exceeds what one dispatch may hold)
+template <typename T>
+static int launch_op(magic_batch *b, int op, int lg, GateSpec gs, int arg = 0)
 {
-    StepGate g;
-    g.c = b->ctl_d + s;
-    g.mode = mode;
-    g.nn = nn;
-    g.kk = kk;
-    g.fslot = fslot;
-    return g;
-}
-
-static int launch_bc(magic_batch *b, int s, const int *mthbc, StepGate g)
-{
-    magic_ctx *c = b->sc[s];
-    const Dims &d = c->d;
-    auto lohi = [](int m) { return m < 0 ? 0 : m; };
-    if (mthbc[0] >= 0 || mthbc[1] >= 0)
-        k_bc_x<<<dim3(blocks(d.NY, 64), d.NZ), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[0]),
-                                                                 lohi(mthbc[1]), g);
-    if (mthbc[2] >= 0 || mthbc[3] >= 0)
-        k_bc_y<<<dim3(blocks(d.NX, 64), d.NZ), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[2]),
-                                                                 lohi(mthbc[3]), g);
-    if (mthbc[4] >= 0 || mthbc[5] >= 0)
-        k32_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, b->cs>>>(d, c->q32, c->meqn, lohi(mthbc[4]),
-                                                                   lohi(mthbc[5]), c->zr32, g);
+    const long long total = b->ltotal[lg];
+    if (total <= 0) return MAGIC_OK;
+    const void *fn = batch_op_fn<T>(op);
+    if (!fn) return MAGIC_EARG;
+    const BatchOps<T> *ops = (const BatchOps<T> *)b->ops_d;
+    const int *off = b->loff_d + (size_t)lg * (b->cap + 1);
+    const int4 *gd = b->lgdim_d + (size_t)lg * b->cap;
+    int n = b->n;
+    const dim3 blk = list_block(lg);
+    // a dispatch holds fewer than 2^32 work-items
+    const long long chunk = (1ll << 31) / (blk.x * blk.y * blk.z);
+    for (long long base = 0; base < total; base += chunk) {
+        const unsigned nb = (unsigned)std::min(chunk, total - base);
+        int ib = (int)base;
+        void *kargs[] = {&ops, &off, &gd, &n, &gs, &arg, &ib};
+        HIPCHECK(hipLaunchKernel(fn, dim3(nb), blk, kargs, 0, b->stream));
+    }
     return launch_check();
 }
 
-static int launch_copy(magic_batch *b, float *dst, const float *src, long long n, StepGate g)
-{
-    long long nb = (n + 255) / 256;
-    if (nb > 8192) nb = 8192;
-    k_copy<float, StepGate><<<(unsigned)nb, 256, 0, b->cs>>>(dst, src, n, g);
-    return launch_check();
-}
+static GateSpec gs(int mode, int nn = 0, int kk = 0, int fslot = -1) { return GateSpec{mode, nn, kk, fslot}; }
 
-// src3 of scenario s with the captured substep caps (src3_passes32's sequence)
-static int launch_src(magic_batch *b, int s)
+// src3 of the accepted scenarios: src3_passes32 / src3_passes64's sequence
+// with the substep loops run to the captured caps, each scenario gated by
+// its own counts (src3.f90:203-634, diffuse.f:37-210, conduct.f:245-332)
+template <typename T>
+static int launch_src(magic_batch *b)
 {
-    magic_ctx *c = b->sc[s];
-    const ScenCtl &h = b->ctl[s];
-    const Dims &d = c->d;
-    const long long P = c->P;
-    const int NZ = d.NZ;
-    const dim3 gint = interior_grid(d);
-    const dim3 gface(blocks(d.NX, 64), d.NY);
-    const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
-    const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
-                       c->col32 + C32_R * NZ};
-    int mthbc[6];
-    memcpy(mthbc, &b->mthbc[6 * s], sizeof(mthbc));
-    mthbc[4] = -1;
-    mthbc[5] = -1;
-    k32_src_convert<<<padded_cells(d), cell_block(), 0, b->cs>>>(d, c->q32, col,
-                                                                             gate(b, s, GATE_ACCEPTED));
-    TRY(launch_check());
-    for (int nn = 1; nn <= h.ndtr_cap; nn++) {
-        const StepGate gd = gate(b, s, GATE_DIFF, nn);
-        TRY(launch_bc(b, s, mthbc, gd));
-        k_zghost_uvw<<<gface, 64, 0, b->cs>>>(d, c->q32, gd);
-        k_diff_cross<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->cs>>>(
-            d, c->u32, c->q32, nu, 0.f, 0.f, 0.f, diff_kb(d), gate(b, s, GATE_DIFF, nn, 0, FAC_CROSS));
-        TRY(launch_check());
-        for (int cc = 1; cc <= 3; cc++) {
-            k_diff_xyz<float, true, StepGate><<<diff_grid(d), cell_block(), 0, b->cs>>>(
-                d, c->q32 + cc * P, c->u32 + (cc - 1) * P, nu, 0.f, 0.f, 0.f, diff_kb(d),
-                gate(b, s, GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)));
-            TRY(launch_check());
-        }
-        float *q5 = c->q32 + 4 * P;
-        TRY(launch_copy(b, c->ut32, q5, P, gd));
-        for (int kk = 1; kk <= h.ndtrc_cap; kk++) {
-            const StepGate gk = gate(b, s, GATE_COND, nn, kk);
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->qt32, c->ut32, c->ut32, kap, 0.f, 0,
-                                                                     gate(b, s, GATE_COND, nn, kk, FAC_COND + 0));
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, c->ut32, kap, 0.f, 1,
-                                                                     gate(b, s, GATE_COND, nn, kk, FAC_COND + 1));
-            k_conduct_zghost<<<gface, 64, 0, b->cs>>>(d, c->ut32, gk);
-            if (c->mz > c->mx) {
-                const int S = c->mz - c->mx;
-                k_conduct_zspill<<<(S + 63) / 64, 64, 0, b->cs>>>(d, c->ut32, S, gk);
+    TRY(launch_op<T>(b, OP_CONVERT, LG_PAD, gs(GATE_ACCEPTED)));
+    for (int nn = 1; nn <= b->ndtr_cap; nn++) {
+        const GateSpec gd = gs(GATE_DIFF, nn);
+        // bc3 with the z faces skipped (src3.f90:333-336)
+        TRY(launch_op<T>(b, OP_BCX, LG_FX, gd));
+        TRY(launch_op<T>(b, OP_BCY, LG_FY, gd));
+        TRY(launch_op<T>(b, OP_ZGHOST_UVW, LG_FZ, gd));
+        TRY(launch_op<T>(b, OP_DIFF_CROSS, LG_DIF, gs(GATE_DIFF, nn, 0, FAC_CROSS)));
+        for (int cc = 1; cc <= 3; cc++)
+            TRY(launch_op<T>(b, OP_DIFF_XYZ, LG_DIF, gs(GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)), cc));
+        TRY(launch_op<T>(b, OP_COPY_Q5, LG_CP1, gd));
+        for (int kk = 1; kk <= b->ndtrc_cap; kk++) {
+            const GateSpec gk = gs(GATE_COND, nn, kk);
+            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND, nn, kk, FAC_COND + 0), PASS_QT_UT * 4 + 0));
+            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND, nn, kk, FAC_COND + 1), PASS_UT_QT * 4 + 1));
+            TRY(launch_op<T>(b, OP_CONDUCT_ZGHOST, LG_FZ, gk));
+            TRY(launch_op<T>(b, OP_ZSPILL, LG_SPL, gk));
+            if (kk < b->ndtrc_cap) {   // at kk = ndtrc_cap >= ndtrc these never run
+                TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND_MORE, nn, kk, FAC_COND + 2), PASS_QT_UT * 4 + 2));
+                TRY(launch_op<T>(b, OP_COPY_INTERIOR, LG_INT, gs(GATE_COND_MORE, nn, kk)));
             }
-            if (kk < h.ndtrc_cap) {   // at kk = ndtrc_cap >= ndtrc these never run
-                k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
-                    d, c->qt32, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_MORE, nn, kk, FAC_COND + 2));
-                k_copy_interior<<<gint, cell_block(), 0, b->cs>>>(d, c->ut32, c->qt32, 1,
-                                                                  gate(b, s, GATE_COND_MORE, nn, kk));
-            }
-            k_pass<float, true, StepGate><<<gint, cell_block(), 0, b->cs>>>(
-                d, q5, c->ut32, c->ut32, kap, 0.f, 2, gate(b, s, GATE_COND_LAST, nn, kk, FAC_COND + 2));
-            TRY(launch_check());
+            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND_LAST, nn, kk, FAC_COND + 2), PASS_Q5_UT * 4 + 2));
         }
     }
-    const magic_params &p = c->prm;
-    Forcing f;
-    f.forcemth = (int)p.forcemth;
-    f.t = 0.0; f.dt = 0.0; f.tgauss = 0.0;   // from the control block at run time
-    f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
-    f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
-    f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
-    f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
-    f.tcenter = p.tcenter; f.twidth = p.twidth;
-    f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
-    k32_src_finish<<<gint, cell_block(), 0, b->cs>>>(d, c->q32, c->meqn, col, f, gate(b, s, GATE_ACCEPTED));
-    return launch_check();
-}
-
-// the side streams wait for the main stream (fork) / the main stream waits
-// for them (join), at recording point k
-static int fork_side(magic_batch *b, int k)
-{
-    HIPCHECK(hipEventRecord(b->fork_ev[k], b->stream));
-    for (int i = 0; i < b->nside; i++) HIPCHECK(hipStreamWaitEvent(b->side[i], b->fork_ev[k], 0));
-    return MAGIC_OK;
-}
-
-static int join_side(magic_batch *b, int k)
-{
-    for (int i = 0; i < b->nside; i++) {
-        HIPCHECK(hipEventRecord(b->join_ev[k][i], b->side[i]));
-        HIPCHECK(hipStreamWaitEvent(b->stream, b->join_ev[k][i], 0));
-    }
-    b->cs = b->stream;
-    return MAGIC_OK;
+    return launch_op<T>(b, OP_FINISH, LG_INT, gs(GATE_ACCEPTED));
 }
 
 // one attempt of every scenario, as launched into the capture
+template <typename T, bool PERT>
 static int record_attempt(magic_batch *b)
 {
-    b->cs = b->stream;
     const unsigned nb = blocks(b->n > 18 ? b->n : 18, 256);
-    k_batch_prepare<<<nb, 256, 0, b->stream>>>(b->ctl_d, b->n, b->args_d, b->counters_d);
+    k_batch_prepare<T><<<nb, 256, 0, b->stream>>>(b->ctl_d, b->n, (SweepArgsT<T> *)b->args_d, b->counters_d);
     TRY(launch_check());
-    TRY(fork_side(b, 0));
-    for (int s = 0, k = 0; s < b->n; s++) {
-        magic_ctx *c = b->sc[s];
-        if (!c) continue;
-        b->cs = b->side[k++ % b->nside];
-        const StepGate g = gate(b, s, GATE_RUN);
-        TRY(launch_bc(b, s, &b->mthbc[6 * s], g));
-        if (c->meqn > 5) {
-            k_clamp<<<blocks(c->P, 256), 256, 0, b->cs>>>(c->q32, c->P, c->meqn, g);
-            TRY(launch_check());
-        }
-        TRY(launch_copy(b, c->qold32, c->q32, c->P * c->meqn, g));
-    }
-    TRY(join_side(b, 0));
-    TRY(launch_sweeps(b));
+    const GateSpec run = gs(GATE_RUN);
+    TRY(launch_op<T>(b, OP_BCX, LG_FX, run));
+    TRY(launch_op<T>(b, OP_BCY, LG_FY, run));
+    TRY(launch_op<T>(b, OP_BCZ, LG_BCZ, run));
+    if (b->meqn > 5) TRY(launch_op<T>(b, OP_CLAMP, LG_ALL, run));
+    TRY(launch_op<T>(b, OP_BACKUP, LG_CPY, run));
+    TRY((launch_sweeps<T, PERT>(b)));
     k_batch_decide<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     TRY(launch_check());
-    TRY(fork_side(b, 1));
-    for (int s = 0, k = 0; s < b->n; s++) {
-        magic_ctx *c = b->sc[s];
-        if (!c) continue;
-        b->cs = b->side[k++ % b->nside];
-        if (b->overlap) {
-            const long long n = c->P * c->meqn;
-            long long nb = (n / 4 + 255) / 256;
-            if (nb > 8192) nb = 8192;
-            k_batch_merge<<<(unsigned)(nb > 0 ? nb : 1), 256, 0, b->cs>>>(c->q32, (float *)c->dq[0], n,
-                                                                          gate(b, s, GATE_SWEPT));
-            TRY(launch_check());
-        }
-        TRY(launch_copy(b, c->q32, c->qold32, c->P * c->meqn, gate(b, s, GATE_REJECTED)));
-        TRY(launch_src(b, s));
-    }
-    TRY(join_side(b, 1));
+    if (b->overlap) TRY(launch_op<T>(b, OP_MERGE, LG_MRG, gs(GATE_SWEPT)));
+    TRY(launch_op<T>(b, OP_RESTORE, LG_CPY, gs(GATE_REJECTED)));
+    TRY(launch_src<T>(b));
     k_batch_finalize<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     return launch_check();
+}
+
+// Form-specific part of a scenario's sweep argument block
+static void sweep_form_args(const magic_ctx *c, SweepArgsT<float> &a)
+{
+    const int NZ = c->d.NZ;
+    a.qold = c->qold32;
+    a.qnew = c->q32;
+    a.crho = c->col32 + C32_RHO * NZ;
+    a.cE = c->col32 + C32_E * NZ;
+    a.cgam = c->col32 + C32_GAM * NZ;
+    a.cvg = c->col32 + C32_VG * NZ;
+    a.cres = c->col32 + C32_RES * NZ;
+    a.sink = (float *)c->sink;
+}
+
+static void sweep_form_args(const magic_ctx *c, SweepArgsT<double> &a)
+{
+    a.qold = c->qold;
+    a.qnew = c->q;
+    a.gam = c->aux + AX_GAM * c->P;
+    a.vgrav = c->aux + AX_VG * c->P;
+    a.sink = (double *)c->sink;
 }
 
 // Sweep argument blocks and tile work lists (geometry only; the step's dt
 // factors are written by k_batch_prepare).  For the autotuner (tuning) every
 // tile runs and the dt factors come from each scenario's next dt.
+template <typename T>
 static int build_lists(magic_batch *b, bool tuning = false)
 {
-    std::vector<SweepArgsT<float>> args((size_t)b->n * 18);
+    std::vector<SweepArgsT<T>> args((size_t)b->n * 18);
     std::vector<const int *> run((size_t)b->n * 18);
     std::vector<std::vector<int4>> lists(18);
     // tiles of longer pencils first (they take longest)
@@ -459,23 +394,15 @@ static int build_lists(magic_batch *b, bool tuning = false)
             magic_ctx *c = b->sc[s];
             const Dims &d = c->d;
             const long long NXY = (long long)d.NX * d.NY;
-            const int NZ = d.NZ;
-            SweepArgsT<float> a;
+            SweepArgsT<T> a;
             memset(&a, 0, sizeof(a));
-            a.qold = c->qold32;
-            a.qnew = c->q32;
+            sweep_form_args(c, a);
             a.P = c->P;
             a.mbc = c->mbc;
             a.nspecadv = c->have_params ? c->prm.nspecadv : 0;
             a.cfl_bits = &b->ctl_d[s].cfl_bits;
             a.status = &b->ctl_d[s].err[0];
             for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
-            a.crho = c->col32 + C32_RHO * NZ;
-            a.cE = c->col32 + C32_E * NZ;
-            a.cgam = c->col32 + C32_GAM * NZ;
-            a.cvg = c->col32 + C32_VG * NZ;
-            a.cres = c->col32 + C32_RES * NZ;
-            a.sink = (float *)c->sink;
             int a_is_g;
             const double dt = b->ctl[s].dt > 0.0 ? b->ctl[s].dt : 1.0;
             if (dir == 0) {          // step3.f:121-280 (g = y, h = z), tile axis y = g
@@ -505,10 +432,10 @@ static int build_lists(magic_batch *b, bool tuning = false)
             for (int ca = 0; ca < 2; ca++)
                 for (int cb = 0; cb < 3; cb++) {
                     const int k = ca * 3 + cb, ai = s * 18 + dir * 6 + k;
-                    SweepArgsT<float> t = a;
+                    SweepArgsT<T> t = a;
                     t.cA = ca;
                     t.cB = cb;
-                    if (!tuning && b->overlap && (dir == 1 || (dir == 2 && k >= 3))) t.qnew = (float *)c->dq[0];
+                    if (!tuning && b->overlap && (dir == 1 || (dir == 2 && k >= 3))) t.qnew = (T *)c->dq[0];
                     args[ai] = t;
                     run[ai] = tuning ? b->one_d : &b->ctl_d[s].run;
                     const int gx = (ntiles - ca + 1) / 2, gy = (a.nB + 1 - cb) / 3 + 1;
@@ -522,6 +449,7 @@ static int build_lists(magic_batch *b, bool tuning = false)
         b->item_off[k] = (int)total;
         total += lists[k].size();
     }
+    if (total > 0x7fffffff) return MAGIC_EUNSUPPORTED;
     b->item_off[18] = (int)total;
     std::vector<int4> items;
     items.reserve(total);
@@ -536,21 +464,128 @@ static int build_lists(magic_batch *b, bool tuning = false)
     return MAGIC_OK;
 }
 
+// Form-specific operands of a scenario's boundary and source-pass kernels
+static void form_ops(const magic_ctx *c, BatchOps<float> &o)
+{
+    const int NZ = c->d.NZ;
+    o.q = c->q32; o.qold = c->qold32; o.u = c->u32; o.qt = c->qt32; o.ut = c->ut32;
+    o.col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ, c->col32 + C32_R * NZ,
+             c->bg64, c->bg64 + NZ};
+    o.zr = c->zr32;
+    o.nu32 = c->col32 + C32_NU * NZ;
+    o.kap32 = c->col32 + C32_KAP * NZ;
+}
+
+static void form_ops(const magic_ctx *c, BatchOps<double> &o)
+{
+    const int NZ = c->d.NZ;
+    const long long P = c->P;
+    o.q = c->q; o.qold = c->qold; o.u = c->u; o.qt = c->qt; o.ut = c->ut;
+    const double *A = c->col64;   // the background's columns (magic_batch_start checks they exist)
+    o.ax = {A + AX_RHO * NZ, A + AX_MX * NZ, A + AX_MY * NZ, A + AX_MZ * NZ, A + AX_E * NZ, A + AX_R * NZ,
+            A + AX_GAM * NZ};
+    o.nu64 = A + AX_NU * NZ;
+    o.kap64 = A + AX_KAP * NZ;
+    o.aux1 = c->aux + AX_RHO * P;
+    o.aux2 = c->aux + AX_MX * P;
+    o.aux3 = c->aux + AX_MY * P;
+    o.aux5 = c->aux + AX_E * P;
+    o.zratio = c->zratio;
+}
+
+// Grid of list lg for one scenario (x, y, z blocks; x = 0: no blocks)
+static int4 list_grid(const magic_batch *b, const magic_ctx *c, const int *bc, int lg)
+{
+    const Dims &d = c->d;
+    const long long P = c->P;
+    auto g1 = [](long long x) { return make_int4((int)x, 1, 1, 0); };
+    switch (lg) {
+    case LG_FX: return bc[0] >= 0 || bc[1] >= 0 ? make_int4(blocks(d.NY, 64), d.NZ, 1, 0) : g1(0);
+    case LG_FY: return bc[2] >= 0 || bc[3] >= 0 ? make_int4(blocks(d.NX, 64), d.NZ, 1, 0) : g1(0);
+    case LG_BCZ: return bc[4] >= 0 || bc[5] >= 0 ? make_int4(blocks(d.NX, 64), d.NY, 1, 0) : g1(0);
+    case LG_FZ: return make_int4(blocks(d.NX, 64), d.NY, 1, 0);
+    case LG_ALL: return g1(blocks(P, 256));
+    case LG_CPY: return g1(std::min<long long>(8192, blocks(P * b->meqn, 256)));
+    case LG_CP1: return g1(std::min<long long>(8192, blocks(P, 256)));
+    case LG_PAD: { const dim3 g = padded_cells(d); return make_int4(g.x, g.y, g.z, 0); }
+    case LG_DIF: { const dim3 g = diff_grid(d); return make_int4(g.x, g.y, g.z, 0); }
+    case LG_INT: { const dim3 g = interior_grid(d); return make_int4(g.x, g.y, g.z, 0); }
+    case LG_SPL: return g1(c->mz > c->mx ? blocks(c->mz - c->mx, 64) : 0);
+    case LG_MRG: {
+        const long long nv = P * b->meqn / (long long)(16 / elt(b)) + 1;
+        return g1(std::min<long long>(8192, blocks(nv, 256)));
+    }
+    default: return g1(0);
+    }
+}
+
+// Operand blocks and block lists of the boundary, copy and source-pass
+// kernels, and the substep loops the capture records
+template <typename T>
+static int build_ops(magic_batch *b)
+{
+    std::vector<BatchOps<T>> ops(b->n);
+    std::vector<int> off((size_t)NLG * (b->cap + 1), 0);
+    std::vector<int4> gdim((size_t)NLG * b->cap, make_int4(0, 1, 1, 0));
+    b->ndtr_cap = 0;
+    b->ndtrc_cap = 0;
+    for (int lg = 0; lg < NLG; lg++) {
+        long long total = 0;
+        for (int s = 0; s < b->n; s++) {
+            off[(size_t)lg * (b->cap + 1) + s] = (int)total;
+            magic_ctx *c = b->sc[s];
+            if (!c) continue;
+            const int4 g = list_grid(b, c, &b->mthbc[6 * s], lg);
+            gdim[(size_t)lg * b->cap + s] = g;
+            total += (long long)g.x * g.y * g.z;
+            if (total > 0x7fffffff) return MAGIC_EUNSUPPORTED;
+        }
+        off[(size_t)lg * (b->cap + 1) + b->n] = (int)total;
+        b->ltotal[lg] = total;
+    }
+    for (int s = 0; s < b->n; s++) {
+        magic_ctx *c = b->sc[s];
+        BatchOps<T> &o = ops[s];
+        memset(&o, 0, sizeof(o));
+        if (!c) continue;
+        o.d = c->d;
+        form_ops(c, o);
+        o.dq = (T *)c->dq[0];
+        o.nc = c->meqn;
+        o.kb = diff_kb(c->d);
+        o.spill = c->mz > c->mx ? c->mz - c->mx : 0;
+        memcpy(o.bc, &b->mthbc[6 * s], sizeof(o.bc));
+        o.ctl = b->ctl_d + s;
+        const magic_params &p = c->prm;
+        Forcing &f = o.f;
+        f.forcemth = (int)p.forcemth;
+        f.t = 0.0; f.dt = 0.0; f.tgauss = 0.0;   // from the control block at run time
+        f.xlower = c->xlower; f.ylower = c->ylower; f.zlower = c->zlower;
+        f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+        f.amplitude = p.amplitude; f.xpos = p.xpos; f.ypos = p.ypos; f.zpos = p.zpos;
+        f.xwidth = p.xwidth; f.ywidth = p.ywidth; f.zwidth = p.zwidth;
+        f.tcenter = p.tcenter; f.twidth = p.twidth;
+        f.omega = p.omega; f.propx = p.propx; f.propz = p.propz; f.vsrcx = p.vsrcx;
+        b->ndtr_cap = std::max(b->ndtr_cap, b->ctl[s].ndtr_cap);
+        b->ndtrc_cap = std::max(b->ndtrc_cap, b->ctl[s].ndtrc_cap);
+    }
+    HIPCHECK(hipMemcpyAsync(b->ops_d, ops.data(), ops.size() * sizeof(ops[0]), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->loff_d, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->lgdim_d, gdim.data(), gdim.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return MAGIC_OK;
+}
+
 // persistent grid of each direction: as many workgroups as fit on the chip
+template <typename T, bool PERT>
 static int list_grids(magic_batch *b)
 {
     int dev = 0, ncu = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     for (int dir = 0; dir < 3; dir++) {
-        const void *fn = nullptr;
-        switch (b->meqn - 5) {
-        case 0: fn = tile_list_fn<0>(b->shape[dir]); break;
-        case 1: fn = tile_list_fn<1>(b->shape[dir]); break;
-        case 2: fn = tile_list_fn<2>(b->shape[dir]); break;
-        case 5: fn = tile_list_fn<5>(b->shape[dir]); break;
-        default: return MAGIC_EUNSUPPORTED;
-        }
+        const void *fn = list_fn<T, PERT>(b, dir);
+        if (!fn) return MAGIC_EUNSUPPORTED;
         int occ = 0;
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
         b->grid[dir] = ncu * (occ > 0 ? occ : 1);
@@ -559,17 +594,16 @@ static int list_grids(magic_batch *b)
 }
 
 // Retire the captured attempt: wait until no replay runs (the main stream
-// and every branch stream), then destroy the executable graph and the graph
-// itself.  Called before any device buffer its kernels address (control
-// blocks, argument blocks, work lists, a scenario's state) is freed or
-// reallocated, so that no instantiated graph ever holds a dangling pointer,
-// and before every capture.
+// and the sweep overlap's stream), then destroy the executable graph and
+// the graph itself.  Called before any device buffer its kernels address
+// (control blocks, argument blocks, work and block lists, a scenario's
+// state) is freed or reallocated, so that no instantiated graph ever holds
+// a dangling pointer, and before every capture.
 static int drop_graph(magic_batch *b)
 {
     int st = MAGIC_OK;
     if (b->stream && hipStreamSynchronize(b->stream) != hipSuccess) st = MAGIC_EHIP;
-    for (int i = 0; i < magic_batch::NSIDE; i++)
-        if (b->side[i] && hipStreamSynchronize(b->side[i]) != hipSuccess) st = MAGIC_EHIP;
+    if (b->side && hipStreamSynchronize(b->side) != hipSuccess) st = MAGIC_EHIP;
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -578,24 +612,25 @@ static int drop_graph(magic_batch *b)
     return st;
 }
 
-static int build_graph(magic_batch *b)
+template <typename T, bool PERT>
+static int build_graph_t(magic_batch *b)
 {
     TRY(drop_graph(b));
-    TRY(list_grids(b));
+    TRY((list_grids<T, PERT>(b)));
     if (b->overlap)
         for (magic_ctx *c : b->sc) {
             if (!c || c->dq[0]) continue;
-            const size_t bytes = (size_t)c->P * c->meqn * sizeof(float);
+            const size_t bytes = (size_t)c->P * c->meqn * sizeof(T);
             if (hipMalloc(&c->dq[0], bytes) != hipSuccess) {
                 c->dq[0] = nullptr;
                 return MAGIC_ENOMEM;
             }
             HIPCHECK(hipMemsetAsync(c->dq[0], 0, bytes, b->stream));   // before the capture
         }
-    TRY(build_lists(b));
+    TRY(build_lists<T>(b));
+    TRY(build_ops<T>(b));
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int st = record_attempt(b);
-    b->cs = b->stream;
+    const int st = record_attempt<T, PERT>(b);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (st != MAGIC_OK) {
@@ -609,6 +644,11 @@ static int build_graph(magic_batch *b)
     return MAGIC_OK;
 }
 
+static int build_graph(magic_batch *b)
+{
+    return fp32(b) ? build_graph_t<float, true>(b) : build_graph_t<double, false>(b);
+}
+
 // ------------------------------------------------------------------ ABI --
 // device arrays for at least want slots (contents are rebuilt from the host
 // copies before the next attempt)
@@ -619,13 +659,20 @@ static int reserve(magic_batch *b, int want)
     while (cap < want) cap *= 2;
     TRY(drop_graph(b));
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d);
+    hipFree(b->ops_d); hipFree(b->loff_d); hipFree(b->lgdim_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
     b->ctl_d = nullptr; b->args_d = nullptr; b->run_d = nullptr; b->ctl_pin = nullptr;
+    b->ops_d = nullptr; b->loff_d = nullptr; b->lgdim_d = nullptr;
     b->cap = 0;
+    const size_t args_bytes = fp32(b) ? sizeof(SweepArgsT<float>) : sizeof(SweepArgsT<double>);
+    const size_t ops_bytes = fp32(b) ? sizeof(BatchOps<float>) : sizeof(BatchOps<double>);
     if (hipMalloc(&b->ctl_d, cap * sizeof(ScenCtl)) != hipSuccess ||
         hipHostMalloc(&b->ctl_pin, cap * sizeof(ScenCtl), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&b->args_d, (size_t)cap * 18 * sizeof(SweepArgsT<float>)) != hipSuccess ||
-        hipMalloc(&b->run_d, (size_t)cap * 18 * sizeof(int *)) != hipSuccess)
+        hipMalloc(&b->args_d, (size_t)cap * 18 * args_bytes) != hipSuccess ||
+        hipMalloc(&b->run_d, (size_t)cap * 18 * sizeof(int *)) != hipSuccess ||
+        hipMalloc(&b->ops_d, (size_t)cap * ops_bytes) != hipSuccess ||
+        hipMalloc(&b->loff_d, (size_t)NLG * (cap + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->lgdim_d, (size_t)NLG * cap * sizeof(int4)) != hipSuccess)
         return MAGIC_ENOMEM;
     b->cap = cap;
     b->graph_ok = false;
@@ -681,7 +728,7 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
 {
     if (!out || nscen < 0 || (nscen > 0 && (!dims || !geom))) return MAGIC_EARG;
     *out = nullptr;
-    if (form != MAGIC_FORM_FP32P) return MAGIC_EUNSUPPORTED;
+    if (form != MAGIC_FORM_FP32P && form != MAGIC_FORM_FP64) return MAGIC_EUNSUPPORTED;
     if (mbc < 2 || mbc > 8 || meqn < 5 || meqn > 10 || maux < 19) return MAGIC_EARG;
     if (!sweep_supported(meqn - 5)) return MAGIC_EUNSUPPORTED;
     magic_batch *b = new magic_batch();
@@ -689,23 +736,13 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
     b->mbc = mbc;
     b->maux = maux;
     b->form = form;
-    if (const char *e = getenv("MAGIC_BATCH_BRANCHES")) {
-        const int v = atoi(e);
-        if (v >= 1 && v <= magic_batch::NSIDE) b->nside = v;
-    }
     int st = MAGIC_OK;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
-    b->cs = b->stream;
-    for (int i = 0; i < b->nside && st == MAGIC_OK; i++)
-        if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) != hipSuccess) st = MAGIC_EHIP;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess)
+        st = MAGIC_EHIP;
     if (st == MAGIC_OK && (hipEventCreateWithFlags(&b->sw_fork, hipEventDisableTiming) != hipSuccess ||
                            hipEventCreateWithFlags(&b->sw_join, hipEventDisableTiming) != hipSuccess))
         st = MAGIC_EHIP;
-    for (int k = 0; k < 2 && st == MAGIC_OK; k++) {
-        if (hipEventCreateWithFlags(&b->fork_ev[k], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
-        for (int i = 0; i < b->nside && st == MAGIC_OK; i++)
-            if (hipEventCreateWithFlags(&b->join_ev[k][i], hipEventDisableTiming) != hipSuccess) st = MAGIC_EHIP;
-    }
     if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
     if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
     if (st == MAGIC_OK) st = hipMalloc(&b->one_d, sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
@@ -732,20 +769,11 @@ extern "C" int magic_batch_destroy(magic_batch *b)
     for (auto c : b->sc)
         if (c) magic_destroy(c);
     hipFree(b->ctl_d); hipFree(b->args_d); hipFree(b->run_d); hipFree(b->items_d); hipFree(b->counters_d);
-    hipFree(b->one_d);
+    hipFree(b->one_d); hipFree(b->ops_d); hipFree(b->loff_d); hipFree(b->lgdim_d);
     if (b->ctl_pin) hipHostFree(b->ctl_pin);
     if (b->sw_fork) hipEventDestroy(b->sw_fork);
     if (b->sw_join) hipEventDestroy(b->sw_join);
-    for (int k = 0; k < 2; k++) {
-        if (b->fork_ev[k]) hipEventDestroy(b->fork_ev[k]);
-        for (int i = 0; i < magic_batch::NSIDE; i++)
-            if (b->join_ev[k][i]) hipEventDestroy(b->join_ev[k][i]);
-    }
-    for (int i = 0; i < magic_batch::NSIDE; i++)
-        if (b->side[i]) {
-            hipStreamSynchronize(b->side[i]);
-            hipStreamDestroy(b->side[i]);
-        }
+    if (b->side) hipStreamDestroy(b->side);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return MAGIC_OK;
@@ -771,6 +799,8 @@ extern "C" int magic_batch_start(magic_batch *b, int s, double t, double tend, d
     const magic_params &p = c->prm;
     if (p.omega1 != 0.0 || !(p.forcemth == 0.0 || p.forcemth == 1.0 || p.forcemth == 3.0))
         return MAGIC_EUNSUPPORTED;
+    // the FP64 form's batched source pass reads the background as columns
+    if (!fp32(b) && !c->aux_layered) return MAGIC_EUNSUPPORTED;
     ScenCtl &h = b->ctl[s];
     if (memcmp(&b->mthbc[6 * s], mthbc, 6 * sizeof(int)) != 0) b->graph_ok = false;
     memcpy(&b->mthbc[6 * s], mthbc, 6 * sizeof(int));
@@ -801,7 +831,8 @@ extern "C" int magic_batch_start(magic_batch *b, int s, double t, double tend, d
     h.twidth = p.twidth;
     int a, bb;
     src_caps(c, dt > dtv[1] ? dt : dtv[1], &a, &bb);
-    if (a != h.ndtr_cap || bb != h.ndtrc_cap) b->graph_ok = false;
+    // the captured substep loops cover every count up to the batch's largest
+    if (a > b->ndtr_cap || bb > b->ndtrc_cap) b->graph_ok = false;
     h.ndtr_cap = a;
     h.ndtrc_cap = bb;
     b->dirty = true;
@@ -893,17 +924,18 @@ extern "C" int magic_batch_get_tile_shapes(magic_batch *b, int shapes[3])
 // per shape and round, all directions); q is restored from the backup
 // afterwards.  The candidates' Courant numbers and reports land in the
 // control blocks' reduction fields, which the next attempt's prepare clears.
+template <typename T, bool PERT>
 static int tune_rounds(magic_batch *b, int reps, TuneClock &clk)
 {
     for (int r = 0; r <= reps; r++) {   // repetition 0 warms up and is not timed
         for (int sh = 0; sh < TILE_NSHAPE; sh++) {
             for (int k = 0; k < 3; k++) b->shape[k] = sh;
-            TRY(list_grids(b));
-            TRY(build_lists(b, true));
+            TRY((list_grids<T, PERT>(b)));
+            TRY(build_lists<T>(b, true));
             for (int dir = 0; dir < 3; dir++) {
                 TRY(clk.mark(dir, sh, 0, b->stream));
                 HIPCHECK(hipMemsetAsync(b->counters_d, 0, 18 * sizeof(int), b->stream));
-                TRY(launch_dir(b, dir));
+                TRY((launch_dir<T, PERT>(b, dir)));
                 TRY(clk.mark(dir, sh, 1, b->stream));
             }
         }
@@ -925,13 +957,13 @@ extern "C" int magic_batch_tune(magic_batch *b, int reps, int shapes_out[3], flo
     for (int s = 0; s < b->n; s++) {
         magic_ctx *c = b->sc[s];
         if (c)
-            HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, (size_t)c->P * c->meqn * sizeof(float),
+            HIPCHECK(hipMemcpyAsync(state_qold(c), state_q(c), (size_t)c->P * c->meqn * elt(b),
                                     hipMemcpyDeviceToDevice, b->stream));
     }
-    int st = tune_rounds(b, reps, clk);
+    int st = fp32(b) ? tune_rounds<float, true>(b, reps, clk) : tune_rounds<double, false>(b, reps, clk);
     for (int s = 0; s < b->n; s++) {   // also after a failed round
         magic_ctx *c = b->sc[s];
-        if (c && hipMemcpyAsync(c->q32, c->qold32, (size_t)c->P * c->meqn * sizeof(float), hipMemcpyDeviceToDevice,
+        if (c && hipMemcpyAsync(state_q(c), state_qold(c), (size_t)c->P * c->meqn * elt(b), hipMemcpyDeviceToDevice,
                                 b->stream) != hipSuccess)
             st = MAGIC_EHIP;
     }

@@ -61,13 +61,14 @@ __device__ __forceinline__ double sa_heating(double precipw, double zcell)
     return heat;
 }
 
+template <class G>
 __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
                        const double *aux2, const double *aux3, const double *aux5,
-                       const double *zratio);
+                       const double *zratio, G gate);
 template <typename T, class G>
 __global__ void k_clamp(T *q, long long P, int nc, G gate);
-template <bool COL>
-__global__ void k_src_convert(Dims d, double *q, const Aux a);
+template <bool COL, class G>
+__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate);
 template <typename T, class G>
 __global__ void k_zghost_uvw(Dims d, T *q, G gate);
 template <typename T, bool COLC, class G>
@@ -82,15 +83,16 @@ template <typename T, class G>
 __global__ void k_conduct_zspill(Dims d, T *u, int S, G gate);
 template <typename T, bool COLC, class G>
 __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate);
-template <bool COL>
-__global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f);
+template <bool COL, class G>
+__global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f, G gate);
 __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, int nc);
 
 // ---- FP32 perturbation form (bc_src32.hip) ----
 // Background columns at rest, indexed by k + mbc - 1: rho_bar, E_bar,
-// gamma (aux19), R (aux18).
+// gamma (aux19), R (aux18), and rho_bar, E_bar in FP64.
 struct Col32 {
     const float *rho, *E, *gam, *R;
+    const double *rho64, *E64;
 };
 // bc3 z-face factors (rho_bar_ghost/rho_bar_boundary)**0.5 per side and layer
 struct Zr32 {

@@ -8,6 +8,7 @@
 // ((k+mbc-1)*NY + (j+mbc-1))*NX + (i+mbc-1) of every plane.
 #include "magic_dev.h"
 #include "stencil_body.h"
+#include "batch_ops.h"
 
 namespace magic {
 
@@ -16,12 +17,12 @@ namespace magic {
 // z faces: bc3_mpi.f:544-809.  Extrapolation and wall keep the perturbation
 // from the background scaled by sc = (rho_bar_ghost/rho_bar_1)**0.5, which
 // the host precomputes (zratio[side][kbc-1][plane index of the (i,j) column]).
-__global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
-                       const double *aux2, const double *aux3, const double *aux5,
-                       const double *zratio)
+template <class G>
+__device__ __forceinline__ void b_bc_z(const uint3 BI, Dims d, double *q, int nc, int lo, int hi, const double *aux1, const double *aux2, const double *aux3, const double *aux5, const double *zratio, const G &gate)
 {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
+    if (!gate.on()) return;
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)BI.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
     const long long col = (long long)(j + d.mbc - 1) * d.NX + (i + d.mbc - 1);
     const long long NXY = (long long)d.NX * d.NY;
@@ -57,13 +58,21 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
         }
     }
 }
+template <class G>
+__global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *aux1,
+                       const double *aux2, const double *aux3, const double *aux5,
+                       const double *zratio, G gate)
+{
+    b_bc_z(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi, aux1, aux2, aux3, aux5, zratio, gate);
+}
 
 // ---------------------------------------------------------------- src3 ---
 // conserved -> (rho, u', v', w', T') on every cell: src3.f90:242-304.
 // COL: the background comes as columns (index kk), else as planes.
-template <bool COL>
-__global__ void k_src_convert(Dims d, double *q, const Aux a)
+template <bool COL, class G>
+__device__ __forceinline__ void b_src_convert(const uint3 BI, Dims d, double *q, const Aux a, const G &gate)
 {
+    if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
     const long long x = COL ? (long long)kk : c;
@@ -79,12 +88,19 @@ __global__ void k_src_convert(Dims d, double *q, const Aux a)
     q[2 * P + c] = q3 / q1 - a3 / a1;
     q[3 * P + c] = q4 / q1 - a4 / a1;
 }
+template <bool COL, class G>
+__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate)
+{
+    b_src_convert<COL>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, a, gate);
+}
 
 // src3.f90:380-418 (reassembly), 430-445 / 514-607 (forcing), 616-626
 // (tracer clamp), on the interior.
-template <bool COL>
-__global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f)
+template <bool COL, class G>
+__device__ __forceinline__ void b_src_finish(const uint3 BI, Dims d, double *q, int nc, const Aux a, Forcing f, const G &gate)
 {
+    if (!gate.on()) return;
+    gate.forcing(f.t, f.dt, f.tgauss);
     INTERIOR_IJK
     const long long P = d.P;
     const long long x = COL ? (long long)(k + d.mbc - 1) : c;
@@ -127,6 +143,11 @@ __global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f)
     q[4 * P + c] = q5;
     for (int m = 5; m < nc; m++) q[m * P + c] = fmaxD(0.0, fminD(q[m * P + c], 1.0));
 }
+template <bool COL, class G>
+__global__ void k_src_finish(Dims d, double *q, int nc, const Aux a, Forcing f, G gate)
+{
+    b_src_finish<COL>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, a, f, gate);
+}
 
 // ---------------------------------------------------------- layout I/O ---
 // reference AoS q(meqn, i, j, k) <-> component planes
@@ -144,15 +165,50 @@ __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, 
     for (int m = 0; m < nc; m++) aos[c * nc + m] = planes[m * P + c];
 }
 
+// The FP64 form's own bodies for the batched work-list kernels
+// (batch_ops.h); its scenarios keep background columns
+struct Form64 {
+    __device__ static void bc_z(uint3 bi, const BatchOps<double> &o, const StepGate &g)
+    {
+        b_bc_z(bi, o.d, o.q, o.nc, lohi(o.bc[4]), lohi(o.bc[5]), o.aux1, o.aux2, o.aux3, o.aux5, o.zratio, g);
+    }
+    __device__ static void convert(uint3 bi, const BatchOps<double> &o, const StepGate &g)
+    {
+        b_src_convert<true>(bi, o.d, o.q, o.ax, g);
+    }
+    __device__ static void finish(uint3 bi, const BatchOps<double> &o, const StepGate &g)
+    {
+        b_src_finish<true>(bi, o.d, o.q, o.nc, o.ax, o.f, g);
+    }
+    __device__ static const double *nu(const BatchOps<double> &o) { return o.nu64; }
+    __device__ static const double *kap(const BatchOps<double> &o) { return o.kap64; }
+};
+
+template <>
+const void *batch_op_fn<double>(int op)
+{
+    switch (op) {
+#define MAGIC_OP(X) case X: return reinterpret_cast<const void *>(&k_batch_op<double, X, Form64>);
+    MAGIC_OP(OP_BCX) MAGIC_OP(OP_BCY) MAGIC_OP(OP_BCZ) MAGIC_OP(OP_CLAMP) MAGIC_OP(OP_BACKUP) MAGIC_OP(OP_RESTORE)
+    MAGIC_OP(OP_CONVERT) MAGIC_OP(OP_ZGHOST_UVW) MAGIC_OP(OP_DIFF_CROSS) MAGIC_OP(OP_DIFF_XYZ) MAGIC_OP(OP_COPY_Q5)
+    MAGIC_OP(OP_PASS) MAGIC_OP(OP_CONDUCT_ZGHOST) MAGIC_OP(OP_ZSPILL) MAGIC_OP(OP_COPY_INTERIOR) MAGIC_OP(OP_FINISH)
+    MAGIC_OP(OP_MERGE)
+#undef MAGIC_OP
+    default: return nullptr;
+    }
+}
+
 // FP64 instantiations of the shared kernels
 template __global__ void k_bc_x<double, NoGate>(Dims, double *, int, int, int, NoGate);
 template __global__ void k_bc_y<double, NoGate>(Dims, double *, int, int, int, NoGate);
 template __global__ void k_clamp<double, NoGate>(double *, long long, int, NoGate);
 template __global__ void k_zghost_uvw<double, NoGate>(Dims, double *, NoGate);
-template __global__ void k_src_convert<false>(Dims, double *, const Aux);
-template __global__ void k_src_convert<true>(Dims, double *, const Aux);
-template __global__ void k_src_finish<false>(Dims, double *, int, const Aux, Forcing);
-template __global__ void k_src_finish<true>(Dims, double *, int, const Aux, Forcing);
+template __global__ void k_bc_z<NoGate>(Dims, double *, int, int, int, const double *, const double *, const double *,
+                                        const double *, const double *, NoGate);
+template __global__ void k_src_convert<false, NoGate>(Dims, double *, const Aux, NoGate);
+template __global__ void k_src_convert<true, NoGate>(Dims, double *, const Aux, NoGate);
+template __global__ void k_src_finish<false, NoGate>(Dims, double *, int, const Aux, Forcing, NoGate);
+template __global__ void k_src_finish<true, NoGate>(Dims, double *, int, const Aux, Forcing, NoGate);
 template __global__ void k_diff_xyz<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,
                                                          double, int, NoGate);
 template __global__ void k_diff_cross<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,

@@ -12,6 +12,7 @@
 #include "batch.h"
 #include "magic_dev.h"
 #include "stencil_body.h"
+#include "batch_ops.h"
 
 namespace magic {
 
@@ -36,11 +37,11 @@ template __global__ void k_conduct_zspill<float, NoGate>(Dims, float *, int, NoG
 // the vertical momentum; with a background at rest every one of them is
 // q'_ghost = q'_boundary * sc.  sc depends on the layer only.
 template <class G>
-__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate)
+__device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int nc, int lo, int hi, Zr32 zr, const G &gate)
 {
     if (!gate.on()) return;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)BI.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
     for (int side = 0; side < 2; side++) {
         const int mth = side == 0 ? lo : hi;
@@ -70,25 +71,41 @@ __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G ga
         }
     }
 }
+template <class G>
+__global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate)
+{
+    b32_bc_z(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi, zr, gate);
+}
 
 // conserved perturbation -> (rho', u, v, w, T') on every cell
 // (src3.f90:242-304 with u_bar = 0):
 //   T' = T - T_bar = (p' - rho' (gamma-1) E_bar / rho_bar) / (rho R),
 //   p' = (gamma-1) (E' - |m|^2 / (2 rho)),  rho = rho_bar + rho'.
+//
+// The arithmetic of the conversion and of the reassembly below is FP64 (the
+// stored values are FP32): T' is a small difference of the pressure and
+// density perturbations' background-weighted parts, and E' a sum of such
+// parts, whose FP32 rounding the flow carries into the thin upper layers.
 template <class G>
-__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate)
+__device__ __forceinline__ void b32_src_convert(const uint3 BI, Dims d, float *q, Col32 col, const G &gate)
 {
     if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
-    const float rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0f, R = col.R[kk];
-    const float rp = q[c], rho = rb + rp;
-    const float m2 = q[P + c], m3 = q[2 * P + c], m4 = q[3 * P + c], ep = q[4 * P + c];
-    const float pp = g1 * (ep - 0.5f * (m2 * m2 + m3 * m3 + m4 * m4) / rho);
-    q[4 * P + c] = (pp - rp * (g1 * Eb / rb)) / (rho * R);
-    q[P + c] = m2 / rho;
-    q[2 * P + c] = m3 / rho;
-    q[3 * P + c] = m4 / rho;
+    const double rb = col.rho64[kk], Eb = col.E64[kk], g1 = (double)col.gam[kk] - 1.0, R = col.R[kk];
+    const double rp = q[c], rho = rb + rp;
+    const double m2 = q[P + c], m3 = q[2 * P + c], m4 = q[3 * P + c], ep = q[4 * P + c];
+    const double irho = 1.0 / rho;
+    const double pp = g1 * (ep - 0.5 * (m2 * m2 + m3 * m3 + m4 * m4) * irho);
+    q[4 * P + c] = (float)((pp - rp * (g1 * Eb / rb)) * irho / R);
+    q[P + c] = (float)(m2 * irho);
+    q[2 * P + c] = (float)(m3 * irho);
+    q[3 * P + c] = (float)(m4 * irho);
+}
+template <class G>
+__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate)
+{
+    b32_src_convert(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, col, gate);
 }
 
 // Reassembly, plume forcing and tracer clamp on the interior
@@ -97,18 +114,18 @@ __global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate)
 // the heating adding dt*heat to T' first.  Positions and the forcing are
 // evaluated in FP64.
 template <class G>
-__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G gate)
+__device__ __forceinline__ void b32_src_finish(const uint3 BI, Dims d, float *q, int nc, Col32 col, Forcing f, const G &gate)
 {
     if (!gate.on()) return;
     gate.forcing(f.t, f.dt, f.tgauss);
     INTERIOR_IJK
     const long long P = d.P;
     const int kk = k + d.mbc - 1;
-    const float rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0f, R = col.R[kk];
-    const float rp = q[c], rho = rb + rp;
-    const float m2 = q[P + c] * rho, m3 = q[2 * P + c] * rho;
-    float m4 = q[3 * P + c] * rho;
-    float tp = q[4 * P + c];
+    const double rb = col.rho64[kk], Eb = col.E64[kk], g1 = (double)col.gam[kk] - 1.0, R = col.R[kk];
+    const double rp = q[c], rho = rb + rp;
+    const double m2 = q[P + c] * rho, m3 = q[2 * P + c] * rho;
+    double m4 = q[3 * P + c] * rho;
+    double tp = q[4 * P + c];
     const double xcell = f.xlower + (i - 0.5) * f.dx - f.t * f.vsrcx;
     const double ycell = f.ylower + (j - 0.5) * f.dy;
     const double zcell = f.zlower + (k - 0.5) * f.dz;
@@ -119,58 +136,71 @@ __global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G
                            exp(-.5 * ((f.t - f.tcenter) * (f.t - f.tcenter) / (f.twidth * f.twidth)));
         const double ffz = sin(f.omega * (f.t - f.tcenter) - f.propx * (xcell - f.xpos) -
                                f.propz * (ycell - f.ypos));
-        m4 = m4 + (float)(f.dt * f.amplitude * gau * ffz) * rho;
+        m4 = m4 + (f.dt * f.amplitude * gau * ffz) * rho;
     } else if (f.forcemth == 3 && !(zcell > 15000.0)) {
         const double g2 = exp(-.5 * ((xcell - f.xpos) * (xcell - f.xpos) / (f.xwidth * f.xwidth))) *
                           exp(-.5 * ((ycell - f.ypos) * (ycell - f.ypos) / (f.ywidth * f.ywidth))) *
                           f.tgauss;
         const double precipw = f.amplitude * g2;
-        if (precipw > 0.1) tp = tp + (float)(f.dt * sa_heating(precipw, zcell));
+        if (precipw > 0.1) tp = tp + f.dt * sa_heating(precipw, zcell);
     }
-    q[P + c] = m2;
-    q[2 * P + c] = m3;
-    q[3 * P + c] = m4;
-    q[4 * P + c] = Eb * (rp / rb) + tp * rho * R / g1 + 0.5f * (m2 * m2 + m3 * m3 + m4 * m4) / rho;
+    q[P + c] = (float)m2;
+    q[2 * P + c] = (float)m3;
+    q[3 * P + c] = (float)m4;
+    q[4 * P + c] = (float)(Eb * (rp / rb) + tp * rho * R / g1 + 0.5 * (m2 * m2 + m3 * m3 + m4 * m4) / rho);
     for (int m = 5; m < nc; m++) q[m * P + c] = fmaxD(0.0f, fminD(q[m * P + c], 1.0f));
 }
-
-// plain copy of n elements (copyq3 and the working copies of the source pass)
-template <typename T, class G>
-__global__ void k_copy(T *dst, const T *src, long long n, G gate)
+template <class G>
+__global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G gate)
 {
-    if (!gate.on()) return;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (long long)gridDim.x * blockDim.x)
-        dst[e] = src[e];
+    b32_src_finish(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, col, f, gate);
 }
 
-// instantiations: one grid (NoGate) and the batched driver (StepGate)
+
+// The FP32 form's own bodies for the batched work-list kernels (batch_ops.h)
+struct Form32 {
+    __device__ static void bc_z(uint3 bi, const BatchOps<float> &o, const StepGate &g)
+    {
+        b32_bc_z(bi, o.d, o.q, o.nc, lohi(o.bc[4]), lohi(o.bc[5]), o.zr, g);
+    }
+    __device__ static void convert(uint3 bi, const BatchOps<float> &o, const StepGate &g)
+    {
+        b32_src_convert(bi, o.d, o.q, o.col, g);
+    }
+    __device__ static void finish(uint3 bi, const BatchOps<float> &o, const StepGate &g)
+    {
+        b32_src_finish(bi, o.d, o.q, o.nc, o.col, o.f, g);
+    }
+    __device__ static const float *nu(const BatchOps<float> &o) { return o.nu32; }
+    __device__ static const float *kap(const BatchOps<float> &o) { return o.kap32; }
+};
+
+template <>
+const void *batch_op_fn<float>(int op)
+{
+    switch (op) {
+#define MAGIC_OP(X) case X: return reinterpret_cast<const void *>(&k_batch_op<float, X, Form32>);
+    MAGIC_OP(OP_BCX) MAGIC_OP(OP_BCY) MAGIC_OP(OP_BCZ) MAGIC_OP(OP_CLAMP) MAGIC_OP(OP_BACKUP) MAGIC_OP(OP_RESTORE)
+    MAGIC_OP(OP_CONVERT) MAGIC_OP(OP_ZGHOST_UVW) MAGIC_OP(OP_DIFF_CROSS) MAGIC_OP(OP_DIFF_XYZ) MAGIC_OP(OP_COPY_Q5)
+    MAGIC_OP(OP_PASS) MAGIC_OP(OP_CONDUCT_ZGHOST) MAGIC_OP(OP_ZSPILL) MAGIC_OP(OP_COPY_INTERIOR) MAGIC_OP(OP_FINISH)
+    MAGIC_OP(OP_MERGE)
+#undef MAGIC_OP
+    default: return nullptr;
+    }
+}
+
+// instantiations of the single-grid kernels (the batched driver's are
+// k_batch_op, above)
 template __global__ void k32_bc_z<NoGate>(Dims, float *, int, int, int, Zr32, NoGate);
 template __global__ void k32_src_convert<NoGate>(Dims, float *, Col32, NoGate);
 template __global__ void k32_src_finish<NoGate>(Dims, float *, int, Col32, Forcing, NoGate);
-template __global__ void k32_bc_z<StepGate>(Dims, float *, int, int, int, Zr32, StepGate);
-template __global__ void k32_src_convert<StepGate>(Dims, float *, Col32, StepGate);
-template __global__ void k32_src_finish<StepGate>(Dims, float *, int, Col32, Forcing, StepGate);
-template __global__ void k_copy<float, StepGate>(float *, const float *, long long, StepGate);
-template __global__ void k_bc_x<float, StepGate>(Dims, float *, int, int, int, StepGate);
-template __global__ void k_bc_y<float, StepGate>(Dims, float *, int, int, int, StepGate);
-template __global__ void k_clamp<float, StepGate>(float *, long long, int, StepGate);
-template __global__ void k_zghost_uvw<float, StepGate>(Dims, float *, StepGate);
-template __global__ void k_diff_xyz<float, true, StepGate>(Dims, float *, const float *, const float *, float, float,
-                                                          float, int, StepGate);
-template __global__ void k_diff_cross<float, true, StepGate>(Dims, float *, const float *, const float *, float,
-                                                            float, float, int, StepGate);
-template __global__ void k_copy_interior<float, StepGate>(Dims, float *, const float *, int, StepGate);
-template __global__ void k_pass<float, true, StepGate>(Dims, float *, const float *, const float *, const float *,
-                                                      float, int, StepGate);
-template __global__ void k_conduct_zghost<float, StepGate>(Dims, float *, StepGate);
-template __global__ void k_conduct_zspill<float, StepGate>(Dims, float *, int, StepGate);
 
 // reference AoS q(meqn, i, j, k) (FP64, full state) <-> perturbation planes:
 // the background is subtracted / added in FP64, then rounded.
 __global__ void k32_aos_to_pert(Dims d, float *q, const double *aos, int nc, const double *bgrho,
                                 const double *bgE)
 {
+    const uint3 BI = {blockIdx.x, blockIdx.y, blockIdx.z};
     PADDED_IJK
     for (int m = 0; m < nc; m++) {
         const double bg = m == 0 ? bgrho[kk] : (m == 4 ? bgE[kk] : 0.0);
@@ -181,6 +211,7 @@ __global__ void k32_aos_to_pert(Dims d, float *q, const double *aos, int nc, con
 __global__ void k32_pert_to_aos(Dims d, double *aos, const float *q, int nc, const double *bgrho,
                                 const double *bgE)
 {
+    const uint3 BI = {blockIdx.x, blockIdx.y, blockIdx.z};
     PADDED_IJK
     for (int m = 0; m < nc; m++) {
         const double bg = m == 0 ? bgrho[kk] : (m == 4 ? bgE[kk] : 0.0);

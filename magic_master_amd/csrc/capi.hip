@@ -397,7 +397,7 @@ static int bc3_launch(magic_ctx *c, const int *mthbc)
     if (mthbc[4] >= 0 || mthbc[5] >= 0) {
         k_bc_z<<<dim3(blocks(d.NX, 64), d.NY), 64, 0, c->stream>>>(
             d, c->q, c->meqn, lohi(mthbc[4]), lohi(mthbc[5]), A + AX_RHO * c->P, A + AX_MX * c->P,
-            A + AX_MY * c->P, A + AX_E * c->P, c->zratio);
+            A + AX_MY * c->P, A + AX_E * c->P, c->zratio, NoGate{});
         TRY(launch_check());
     }
     return MAGIC_OK;
@@ -962,7 +962,7 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const dim3 gface(blocks(d.NX, 64), d.NY);
     const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
     const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
-                       c->col32 + C32_R * NZ};
+                       c->col32 + C32_R * NZ, c->bg64, c->bg64 + NZ};
     {
         Span sp(c, TC_SRC, 1);
         k32_src_convert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(d, c->q32, col, NoGate{});
@@ -1044,7 +1044,7 @@ static int src3_passes64(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const dim3 gface(blocks(d.NX, 64), d.NY);
     {
         Span sp(c, TC_SRC, 1);
-        k_src_convert<COL><<<padded_cells(d), cell_block(), 0, c->stream>>>(d, c->q, ax);
+        k_src_convert<COL><<<padded_cells(d), cell_block(), 0, c->stream>>>(d, c->q, ax, NoGate{});
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -1104,7 +1104,7 @@ static int src3_passes64(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     }
     {
         Span sp(c, TC_SRC, 1);
-        k_src_finish<COL><<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->meqn, ax, f);
+        k_src_finish<COL><<<gint, cell_block(), 0, c->stream>>>(d, c->q, c->meqn, ax, f, NoGate{});
         TRY(launch_check());
     }
     return MAGIC_OK;

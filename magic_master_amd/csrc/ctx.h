@@ -189,14 +189,14 @@ static inline const void *tile_fn(int sh)
     }
 }
 
-template <int NTR>
+template <typename T, bool PERT, int NTR>
 static inline const void *tile_list_fn(int sh)
 {
     switch (sh) {
-    case 0: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 0>);
-    case 1: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 1>);
-    case 2: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 2>);
-    case 3: return reinterpret_cast<const void *>(&tile_sweep_list<float, true, NTR, 3>);
+    case 0: return reinterpret_cast<const void *>(&tile_sweep_list<T, PERT, NTR, 0>);
+    case 1: return reinterpret_cast<const void *>(&tile_sweep_list<T, PERT, NTR, 1>);
+    case 2: return reinterpret_cast<const void *>(&tile_sweep_list<T, PERT, NTR, 2>);
+    case 3: return reinterpret_cast<const void *>(&tile_sweep_list<T, PERT, NTR, 3>);
     default: return nullptr;
     }
 }

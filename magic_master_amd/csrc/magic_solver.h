@@ -54,6 +54,42 @@ __device__ __forceinline__ T sqrtT(T x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float sqrtT<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// The FP32 form solves the normal Riemann problem in FP64 (the sweep's
+// part 1): its jumps and wave strengths subtract background-sized products
+// (enthalpy times velocity, enthalpy times the density jump), whose FP32
+// rounding the flow carries up into the thin upper layers; in FP32 the
+// form's temperature error there grew to 3e-4 of the field peak in 300
+// steps, in FP64 to 3e-5-6e-5.  Its reciprocals and square roots need not
+// be correctly rounded there: the hardware's FP64 approximations refined by
+// one Newton step (relative error near 1e-15, far inside that form's error
+// bar).  RF selects them (true: the FP32 form) over the correctly rounded
+// operations of the FP64 form.
+template <bool RF>
+__device__ __forceinline__ double rcp1(double x)
+{
+    if constexpr (RF) {
+        const double r = __builtin_amdgcn_rcp(x);
+        return fma(r, fma(-x, r, 1.0), r);
+    } else {
+        return 1.0 / x;
+    }
+}
+template <bool RF>
+__device__ __forceinline__ double sqrt1(double x)
+{
+    if constexpr (RF) {
+        // Goldschmidt step from the reciprocal square root; 0 and negative
+        // (or NaN) arguments take the plain hardware square root
+        const double y = __builtin_amdgcn_rsq(x);
+        const double s0 = x * y, h = 0.5 * y;
+        const double r = fma(-s0, h, 0.5);
+        const double s1 = fma(s0, r, s0);
+        return x > 0.0 ? s1 : __builtin_amdgcn_sqrt(x);
+    } else {
+        return sqrt(x);
+    }
+}
+
 // flang's MIN/MAX: a < b ? a : b, a > b ? a : b
 template <typename T>
 __device__ __forceinline__ T fminF(T a, T b) { return a < b ? a : b; }

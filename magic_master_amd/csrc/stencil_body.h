@@ -24,11 +24,11 @@ __device__ __forceinline__ long long cid(const Dims &d, int i, int j, int k)
 
 // x faces: bc3_mpi.f:97-312.  One thread per (j, k) of the padded face.
 template <typename T, class G>
-__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi, G gate)
+__device__ __forceinline__ void b_bc_x(const uint3 BI, Dims d, T *q, int nc, int lo, int hi, const G &gate)
 {
     if (!gate.on()) return;
-    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int k = (int)blockIdx.y + 1 - d.mbc;
+    const int j = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int k = (int)BI.y + 1 - d.mbc;
     if (j > d.my + d.mbc || k > d.mz + d.mbc) return;
     const bool inner = j >= 1 && j <= d.my && k >= 1 && k <= d.mz;
     for (int m = 0; m < nc; m++) {
@@ -43,14 +43,19 @@ __global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi, G gate)
         }
     }
 }
+template <typename T, class G>
+__global__ void k_bc_x(Dims d, T *q, int nc, int lo, int hi, G gate)
+{
+    b_bc_x(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi, gate);
+}
 
 // y faces: bc3_mpi.f:320-536.  One thread per (i, k).
 template <typename T, class G>
-__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi, G gate)
+__device__ __forceinline__ void b_bc_y(const uint3 BI, Dims d, T *q, int nc, int lo, int hi, const G &gate)
 {
     if (!gate.on()) return;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int k = (int)blockIdx.y + 1 - d.mbc;
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int k = (int)BI.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || k > d.mz + d.mbc) return;
     const bool inner = k >= 1 && k <= d.mz;
     for (int m = 0; m < nc; m++) {
@@ -65,27 +70,37 @@ __global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi, G gate)
         }
     }
 }
+template <typename T, class G>
+__global__ void k_bc_y(Dims d, T *q, int nc, int lo, int hi, G gate)
+{
+    b_bc_y(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi, gate);
+}
 
 // b4step3.f:35-45: clamp tracers to [0, 1] on every cell.
 template <typename T, class G>
-__global__ void k_clamp(T *q, long long P, int nc, G gate)
+__device__ __forceinline__ void b_clamp(const uint3 BI, T *q, long long P, int nc, const G &gate)
 {
     if (!gate.on()) return;
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long c = (long long)BI.x * blockDim.x + threadIdx.x;
     if (c >= P) return;
     for (int m = 5; m < nc; m++) {
         const T v = q[m * P + c];
         q[m * P + c] = fmaxD(T(0.0), fminD(v, T(1.0)));
     }
 }
+template <typename T, class G>
+__global__ void k_clamp(T *q, long long P, int nc, G gate)
+{
+    b_clamp(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, q, P, nc, gate);
+}
 
 // diffuse.f:48-54: z ghosts of q(2:4) from the first/last interior layer.
 template <typename T, class G>
-__global__ void k_zghost_uvw(Dims d, T *q, G gate)
+__device__ __forceinline__ void b_zghost_uvw(const uint3 BI, Dims d, T *q, const G &gate)
 {
     if (!gate.on()) return;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)BI.y + 1 - d.mbc;
     if (i > d.mx + d.mbc || j > d.my + d.mbc) return;
     for (int m = 1; m <= 3; m++) {
         T *p = q + (long long)m * d.P;
@@ -96,22 +111,27 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
         }
     }
 }
+template <typename T, class G>
+__global__ void k_zghost_uvw(Dims d, T *q, G gate)
+{
+    b_zghost_uvw(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, gate);
+}
 
 // Interior thread mapping: x fastest.
 #define INTERIOR_IJK                                                          \
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;           \
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;           \
-    const int k = (int)blockIdx.z + 1;                                        \
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1;                \
+    const int j = (int)(BI.y * blockDim.y + threadIdx.y) + 1;                \
+    const int k = (int)BI.z + 1;                                             \
     if (i > d.mx || j > d.my) return;                                         \
     const long long c = cid(d, i, j, k);
 
 // All-cells thread mapping over the padded grid: x fastest; kk is the
 // padded layer index (k + mbc - 1).
 #define PADDED_IJK                                                            \
-    const int ip = (int)(blockIdx.x * blockDim.x + threadIdx.x);              \
-    const int jp = (int)(blockIdx.y * blockDim.y + threadIdx.y);              \
+    const int ip = (int)(BI.x * blockDim.x + threadIdx.x);                   \
+    const int jp = (int)(BI.y * blockDim.y + threadIdx.y);                   \
     if (ip >= d.NX || jp >= d.NY) return;                                     \
-    const int kk = (int)blockIdx.z;                                           \
+    const int kk = (int)BI.z;                                                \
     const long long c = ((long long)kk * d.NY + jp) * d.NX + ip;
 
 // diffuse.f:70-118: cross-derivative terms of q(2:4) (after the z-ghost
@@ -123,8 +143,7 @@ __global__ void k_zghost_uvw(Dims d, T *q, G gate)
 // is read from memory once per workgroup.  Launch: block (64, 4), grid
 // (ceil(mx/64), ceil(my/4), ceil(mz/kb)).
 template <typename T, bool COLC, class G>
-__global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz,
-                                                    int kb, G gate)
+__device__ __forceinline__ void b_diff_cross(const uint3 BI, Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz, int kb, const G &gate)
 {
     if (!gate.on()) return;
     dxy = gate.f(dxy, 0);
@@ -133,8 +152,8 @@ __global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, co
     constexpr int TX = 66, TL = TX * 6;     // (64 + 2) x (4 + 2) cells per layer
     __shared__ T s[3][3][TL];               // [layer k mod 3][component][cell]
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, t = ty * 64 + tx;
-    const int i0 = (int)blockIdx.x * 64 + 1, j0 = (int)blockIdx.y * 4 + 1;
-    const int k0 = (int)blockIdx.z * kb + 1;
+    const int i0 = (int)BI.x * 64 + 1, j0 = (int)BI.y * 4 + 1;
+    const int k0 = (int)BI.z * kb + 1;
     const int k1 = k0 + kb - 1 < d.mz ? k0 + kb - 1 : d.mz;
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i <= d.mx && j <= d.my;
@@ -184,6 +203,12 @@ __global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, co
         __syncthreads();
     }
 }
+template <typename T, bool COLC, class G>
+__global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, const T *coef, T dxy, T dxz, T dyz,
+                                                    int kb, G gate)
+{
+    b_diff_cross<T, COLC>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, u, q, coef, dxy, dxz, dyz, kb, gate);
+}
 
 // diffuse.f:153-208 for one velocity component: the x, y and z passes of
 // one cycle of the component loop, fused.  u holds the cross-term result on
@@ -200,8 +225,7 @@ __global__ __launch_bounds__(256) void k_diff_cross(Dims d, T *u, const T *q, co
 // (the layer below its first is recomputed).  Launch: block (64, 4), grid
 // (ceil(mx/64), ceil(my/4), ceil(mz/kb)).
 template <typename T, bool COLC, class G>
-__global__ __launch_bounds__(256) void k_diff_xyz(Dims d, T *q, const T *u, const T *coef, T fx, T fy, T fz, int kb,
-                                                  G gate)
+__device__ __forceinline__ void b_diff_xyz(const uint3 BI, Dims d, T *q, const T *u, const T *coef, T fx, T fy, T fz, int kb, const G &gate)
 {
     if (!gate.on()) return;
     fx = gate.f(fx, 0);
@@ -209,10 +233,10 @@ __global__ __launch_bounds__(256) void k_diff_xyz(Dims d, T *q, const T *u, cons
     fz = gate.f(fz, 2);
     __shared__ T sx[6][64];
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
-    const int i = (int)blockIdx.x * 64 + tx + 1;
-    const int j0 = (int)blockIdx.y * 4 + 1;
+    const int i = (int)BI.x * 64 + tx + 1;
+    const int j0 = (int)BI.y * 4 + 1;
     const int j = j0 + ty;
-    const int k0 = (int)blockIdx.z * kb + 1;
+    const int k0 = (int)BI.z * kb + 1;
     const int k1 = k0 + kb - 1 < d.mz ? k0 + kb - 1 : d.mz;
     const bool own = i <= d.mx && j <= d.my;
     const long long sy = d.NX, sz = (long long)d.NX * d.NY;
@@ -257,14 +281,25 @@ __global__ __launch_bounds__(256) void k_diff_xyz(Dims d, T *q, const T *u, cons
         y0 = yp;
     }
 }
+template <typename T, bool COLC, class G>
+__global__ __launch_bounds__(256) void k_diff_xyz(Dims d, T *q, const T *u, const T *coef, T fx, T fy, T fz, int kb,
+                                                  G gate)
+{
+    b_diff_xyz<T, COLC>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, u, coef, fx, fy, fz, kb, gate);
+}
 
 // interior copy of planes src -> dst (n planes with stride P)
 template <typename T, class G>
-__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes, G gate)
+__device__ __forceinline__ void b_copy_interior(const uint3 BI, Dims d, T *dst, const T *src, int nplanes, const G &gate)
 {
     if (!gate.on()) return;
     INTERIOR_IJK
     for (int m = 0; m < nplanes; m++) dst[m * d.P + c] = src[m * d.P + c];
+}
+template <typename T, class G>
+__global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes, G gate)
+{
+    b_copy_interior(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, dst, src, nplanes, gate);
 }
 
 // One explicit directional diffusion pass (diffuse.f:153-208, conduct.f:262-323):
@@ -275,7 +310,7 @@ __global__ void k_copy_interior(Dims d, T *dst, const T *src, int nplanes, G gat
 // second difference keeps the reference's operand order: (up-mid-mid+down)
 // along x and y, (down-mid-mid+up) along z.
 template <typename T, bool COLC, class G>
-__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate)
+__device__ __forceinline__ void b_pass(const uint3 BI, Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, const G &gate)
 {
     if (!gate.on()) return;
     f = gate.f(f, 0);
@@ -291,6 +326,11 @@ __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *co
     const T qt = axis == 2 ? (am - aq - aq + ap) : (ap - aq - aq + am);
     out[c] = aq + f * coef[COLC ? (long long)(k + d.mbc - 1) : c] * qt;
 }
+template <typename T, bool COLC, class G>
+__global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *coef, T f, int axis, G gate)
+{
+    b_pass<T, COLC>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, out, src, ghost, coef, f, axis, gate);
+}
 
 // conduct.f:86-91: z ghosts of u, for j in [1-mbc, my+mbc] and i in
 // [1-mbc, mz+mbc] -- the reference bounds i by mz, not mx.  When mx > mz the
@@ -301,11 +341,11 @@ __global__ void k_pass(Dims d, T *out, const T *src, const T *ghost, const T *co
 // in-row fill writes anyway), so this kernel fills whole rows and
 // k_conduct_zspill adds the writes that cross into the next layer.
 template <typename T, class G>
-__global__ void k_conduct_zghost(Dims d, T *u, G gate)
+__device__ __forceinline__ void b_conduct_zghost(const uint3 BI, Dims d, T *u, const G &gate)
 {
     if (!gate.on()) return;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
-    const int j = (int)blockIdx.y + 1 - d.mbc;
+    const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
+    const int j = (int)BI.y + 1 - d.mbc;
     const int ilast = (d.mz < d.mx ? d.mz : d.mx) + d.mbc;
     if (i > ilast || j > d.my + d.mbc) return;
     const T lo = u[cid(d, i, j, 1)], hi = u[cid(d, i, j, d.mz)];
@@ -313,6 +353,11 @@ __global__ void k_conduct_zghost(Dims d, T *u, G gate)
         u[cid(d, i, j, 1 - kbc)] = lo;
         u[cid(d, i, j, d.mz + kbc)] = hi;
     }
+}
+template <typename T, class G>
+__global__ void k_conduct_zghost(Dims d, T *u, G gate)
+{
+    b_conduct_zghost(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, u, gate);
 }
 
 // The layer-crossing writes of conduct.f:86-91 when mz > mx.  The rows at
@@ -325,10 +370,10 @@ __global__ void k_conduct_zghost(Dims d, T *u, G gate)
 // the last layer leave the reference's array; they are dropped here.
 // Layers are Fortran k indices; r counts from cell (1-mbc, 1-mbc).
 template <typename T, class G>
-__global__ void k_conduct_zspill(Dims d, T *u, int S, G gate)
+__device__ __forceinline__ void b_conduct_zspill(const uint3 BI, Dims d, T *u, int S, const G &gate)
 {
     if (!gate.on()) return;
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int r = (int)(BI.x * blockDim.x + threadIdx.x);
     if (r >= S) return;
     const long long NXY = (long long)d.NX * d.NY;
     auto L = [&](int k) { return (long long)(k + d.mbc - 1) * NXY + r; };
@@ -337,6 +382,26 @@ __global__ void k_conduct_zspill(Dims d, T *u, int S, G gate)
         u[L(2 - kbc)] = lo;
         if (d.mz + 1 + kbc <= d.mz + d.mbc) u[L(d.mz + 1 + kbc)] = hi;
     }
+}
+template <typename T, class G>
+__global__ void k_conduct_zspill(Dims d, T *u, int S, G gate)
+{
+    b_conduct_zspill(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, u, S, gate);
+}
+
+// plain copy of n elements (copyq3 and the working copies of the source pass)
+template <typename T, class G>
+__device__ __forceinline__ void b_copy(const uint3 BI, unsigned nblk, T *dst, const T *src, long long n, const G &gate)
+{
+    if (!gate.on()) return;
+    for (long long e = (long long)BI.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)nblk * blockDim.x)
+        dst[e] = src[e];
+}
+template <typename T, class G>
+__global__ void k_copy(T *dst, const T *src, long long n, G gate)
+{
+    b_copy(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, gridDim.x, dst, src, n, gate);
 }
 
 }  // namespace magic

@@ -66,6 +66,15 @@ constexpr bool tile_flush_atomic() { return sizeof(T) == 8; }
 __device__ __forceinline__ void add_noret(gptr<double> p, double v) { (void)__builtin_amdgcn_global_atomic_fadd_f64(p, v); }
 __device__ __forceinline__ void add_noret(gptr<float> p, float v) { (void)__builtin_amdgcn_global_atomic_fadd_f32(p, v); }
 
+// The Roe state the limiters use: the FP64 form's own; the FP32 form's FP64
+// copy from its solve.
+template <typename T>
+__device__ __forceinline__ const RoeT<double> &limiter_roe(const RoeT<T> &R, const RoeT<double> &RD)
+{
+    if constexpr (sizeof(T) == 8) return R;
+    else return RD;
+}
+
 template <typename AT>
 __device__ __forceinline__ void report_t(const AT &A, int code, int i, int pg, int ph)
 {
@@ -91,7 +100,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     constexpr T beta = T(0.666666666666666);
     static_assert(TA * CH == NT && TA == tile_ta(SH), "tile is 256 lanes");
 
-    __shared__ T s_del[5][NT];
+    __shared__ double s_del[5][NT];   // FP64 in both forms: the limiters' upwind wave strengths
     __shared__ T s_s2[NT];
     __shared__ T s_trw[NTR > 0 ? NTR : 1][NT];
     __shared__ T s_cfo[NT];
@@ -171,9 +180,14 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         bool f1 = false, f2 = false, f5 = false;
 #pragma unroll
         for (int m = 0; m < NC; m++) { fw1[m] = fw2[m] = fw3[m] = amdq[m] = apdq[m] = T(0.0); }
-        if (act1) {
-            T qa[NC], qb[NC], vgc = T(0.0);
-            T ga, gb;
+        // The solve runs in FP64 in both forms (T1 = double): the FP64 form
+        // writes its results in place; the FP32 form solves into FP64
+        // arrays and rounds the results (see rcp1 in magic_solver.h).
+        auto riemann = [&](auto &RR, auto &F1, auto &F2, auto &F3, auto &AM, auto &AP, auto &W) {
+            using T1 = double;
+            constexpr bool RF = sizeof(T) == 4;
+            T1 qa[NC], qb[NC], vgc = T1(0.0);
+            T1 ga, gb;
             if constexpr (PREFETCH) {
 #pragma unroll
                 for (int m = 0; m < NC; m++) {
@@ -200,7 +214,7 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
             // perturbation form: q holds q - background; the background is a
             // column at rest (rho_bar(k), E_bar(k), gamma(k)), so the full
             // states differ from q only in rho and E
-            T rpa = 0, rpb = 0, ppa = 0, ppb = 0;
+            T1 rpa = 0, rpb = 0, ppa = 0, ppb = 0;
             int kib = 0;
             if constexpr (PERT) {
                 const int ka = A.sweep == 3 ? I - 1 : b, kb = A.sweep == 3 ? I : b;
@@ -210,138 +224,163 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 gb = cgam[kib];
                 rpa = qa[0];
                 rpb = qb[0];
-                const T epa = qa[4], epb = qb[4];
+                const T1 epa = qa[4], epb = qb[4];
                 qa[0] = crho[kia] + rpa;
                 qb[0] = crho[kib] + rpb;
                 qa[4] = cE[kia] + epa;
                 qb[4] = cE[kib] + epb;
                 // p - p_bar = (gamma-1) (E' - kinetic energy), p_bar = (gamma-1) E_bar
-                ppa = (ga - T(1.0)) * (epa - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
-                ppb = (gb - T(1.0)) * (epb - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
+                ppa = (ga - T1(1.0)) * (epa - T1(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / qa[0]);
+                ppb = (gb - T1(1.0)) * (epb - T1(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / qb[0]);
             }
-            if (!(qa[0] > T(0.0)) || !(qb[0] > T(0.0))) report_t(A, 1, I, pg, ph);
-            const T rhsqrtl = sqrtT(qa[0]), rhsqrtr = sqrtT(qb[0]);
-            const T rhsq2 = rhsqrtl + rhsqrtr;
+            if (!(qa[0] > T1(0.0)) || !(qb[0] > T1(0.0))) report_t(A, 1, I, pg, ph);
+            const T1 rhsqrtl = sqrt1<RF>(qa[0]), rhsqrtr = sqrt1<RF>(qb[0]);
+            const T1 rhsq2 = rhsqrtl + rhsqrtr;
             // reciprocals taken once where rpn3euX.f:159-216 divides by the
             // same value several times (the products differ from the
             // quotients by at most one rounding)
-            const T il = rcpT(rhsqrtl), ir = rcpT(rhsqrtr), i2 = rcpT(rhsq2);
-            const T iqa = rcpT(qa[0]), iqb = rcpT(qb[0]);
-            const T gamma = (rhsqrtr * gb + rhsqrtl * ga) * i2;
-            const T gamma1 = gamma - T(1.0);
-            const T pl = pressure_nc(ga, qa[4], qa[1], qa[2], qa[3], iqa);
-            const T pr = pressure_nc(gb, qb[4], qb[1], qb[2], qb[3], iqb);
-            R.u = (qa[1] * il + qb[1] * ir) * i2;
-            R.v = (qa[2] * il + qb[2] * ir) * i2;
-            R.w = (qa[3] * il + qb[3] * ir) * i2;
-            R.enth = ((qa[4] + pl) * il + (qb[4] + pr) * ir) * i2;
-            R.u2v2w2 = R.u * R.u + R.v * R.v + R.w * R.w;
-            const T a2 = gamma1 * (R.enth - T(0.5) * R.u2v2w2);
-            if (!(a2 > T(0.0))) report_t(A, 2, I, pg, ph);
-            R.a = sqrtT(a2);
-            R.i2a = rcpT(T(2.0) * R.a);
-            R.g1a2 = gamma1 / a2;
-            R.euv = R.enth - R.u2v2w2;
-            const T ur = qb[1] * iqb, ul = qa[1] * iqa;
-            const T d1 = qb[1] - qa[1];
-            T d2;
+            const T1 il = rcp1<RF>(rhsqrtl), ir = rcp1<RF>(rhsqrtr), i2 = rcp1<RF>(rhsq2);
+            const T1 iqa = rcp1<RF>(qa[0]), iqb = rcp1<RF>(qb[0]);
+            const T1 gamma = (rhsqrtr * gb + rhsqrtl * ga) * i2;
+            const T1 gamma1 = gamma - T1(1.0);
+            const T1 pl = pressure_nc(ga, qa[4], qa[1], qa[2], qa[3], iqa);
+            const T1 pr = pressure_nc(gb, qb[4], qb[1], qb[2], qb[3], iqb);
+            RR.u = (qa[1] * il + qb[1] * ir) * i2;
+            RR.v = (qa[2] * il + qb[2] * ir) * i2;
+            RR.w = (qa[3] * il + qb[3] * ir) * i2;
+            RR.enth = ((qa[4] + pl) * il + (qb[4] + pr) * ir) * i2;
+            RR.u2v2w2 = RR.u * RR.u + RR.v * RR.v + RR.w * RR.w;
+            const T1 a2 = gamma1 * (RR.enth - T1(0.5) * RR.u2v2w2);
+            if (!(a2 > T1(0.0))) report_t(A, 2, I, pg, ph);
+            RR.a = sqrt1<RF>(a2);
+            RR.i2a = rcp1<RF>(T1(2.0) * RR.a);
+            RR.g1a2 = gamma1 / a2;
+            RR.euv = RR.enth - RR.u2v2w2;
+            const T1 ur = qb[1] * iqb, ul = qa[1] * iqa;
+            const T1 d1 = qb[1] - qa[1];
+            T1 d2;
             if constexpr (PERT) d2 = (qb[1] * ur + ppb) - (qa[1] * ul + ppa);   // background part below
             else d2 = flux_jump_nc(qb[1], ur, pr, qa[1], ul, pl);
-            const T d3 = ur * qb[2] - ul * qa[2];
-            const T d4 = ur * qb[3] - ul * qa[3];
-            T d5 = (qb[4] + pr) * ur - (qa[4] + pl) * ul;
+            const T1 d3 = ur * qb[2] - ul * qa[2];
+            const T1 d4 = ur * qb[3] - ul * qa[3];
+            T1 d5 = (qb[4] + pr) * ur - (qa[4] + pl) * ul;
             if (A.gravity) {
                 if constexpr (PERT) {
                     // the background's own jump p_bar_b - p_bar_a - .5 (rho_bar_a +
                     // rho_bar_b) aux17 is the column residual (zero to FP64 rounding)
-                    const T vg = cvg[kib];
-                    d2 = d2 - T(0.5) * (rpb + rpa) * vg + cres[kib];
-                    d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
+                    const T1 vg = cvg[kib];
+                    d2 = d2 - T1(0.5) * (rpb + rpa) * vg + cres[kib];
+                    d5 = d5 - T1(0.5) * (qb[1] + qa[1]) * vg;
                 } else {
-                    const T vg = vgc;
+                    const T1 vg = vgc;
                     d2 = gravity_jump_nc(d2, qa[0], qb[0], vg);
-                    d5 = d5 - T(0.5) * (qb[1] + qa[1]) * vg;
+                    d5 = d5 - T1(0.5) * (qb[1] + qa[1]) * vg;
                 }
             }
-            const T b4 = R.g1a2 * (R.euv * d1 + R.u * d2 + R.v * d3 + R.w * d4 - d5);
-            const T b2 = d3 - R.v * d1;
-            const T b3 = d4 - R.w * d1;
-            const T b5 = (d2 + (R.a - R.u) * d1 - R.a * b4) * R.i2a;
-            const T b1 = d1 - b4 - b5;
-            fw1[0] = b1; fw1[1] = b1 * (R.u - R.a); fw1[2] = b1 * R.v; fw1[3] = b1 * R.w;
-            fw1[4] = b1 * (R.enth - R.u * R.a);
-            s1 = R.u - R.a;
-            fw2[0] = b4; fw2[1] = b4 * R.u; fw2[2] = b2 + b4 * R.v; fw2[3] = b3 + b4 * R.w;
-            fw2[4] = b2 * R.v + b3 * R.w + b4 * T(0.5) * R.u2v2w2;
-            s2 = R.u;
-            fw3[0] = b5; fw3[1] = b5 * (R.u + R.a); fw3[2] = b5 * R.v; fw3[3] = b5 * R.w;
-            fw3[4] = b5 * (R.enth + R.u * R.a);
-            s3 = R.u + R.a;
+            const T1 b4 = RR.g1a2 * (RR.euv * d1 + RR.u * d2 + RR.v * d3 + RR.w * d4 - d5);
+            const T1 b2 = d3 - RR.v * d1;
+            const T1 b3 = d4 - RR.w * d1;
+            const T1 b5 = (d2 + (RR.a - RR.u) * d1 - RR.a * b4) * RR.i2a;
+            const T1 b1 = d1 - b4 - b5;
+            F1[0] = b1; F1[1] = b1 * (RR.u - RR.a); F1[2] = b1 * RR.v; F1[3] = b1 * RR.w;
+            F1[4] = b1 * (RR.enth - RR.u * RR.a);
+            s1 = RR.u - RR.a;
+            F2[0] = b4; F2[1] = b4 * RR.u; F2[2] = b2 + b4 * RR.v; F2[3] = b3 + b4 * RR.w;
+            F2[4] = b2 * RR.v + b3 * RR.w + b4 * T1(0.5) * RR.u2v2w2;
+            s2 = RR.u;
+            F3[0] = b5; F3[1] = b5 * (RR.u + RR.a); F3[2] = b5 * RR.v; F3[3] = b5 * RR.w;
+            F3[4] = b5 * (RR.enth + RR.u * RR.a);
+            s3 = RR.u + RR.a;
 #pragma unroll
             for (int m = 5; m < NC; m++)
-                fw2[m] = (m - 4 <= A.nspecadv) ? s2 * (qb[m] - qa[m]) : ur * qb[m] - ul * qa[m];
+                F2[m] = (m - 4 <= A.nspecadv) ? s2 * (qb[m] - qa[m]) : ur * qb[m] - ul * qa[m];
             // entropy fix and amdq: rpn3euX.f:316-398
             {
-                const T gmm = ga, gm1 = ga - T(1.0);
-                const T pim1 = gm1 * (qa[4] - T(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
-                const T cim1 = sqrtT(gmm * pim1 * iqa);
-                const T s0 = qa[1] * iqa - cim1;
-                if (!(s0 >= T(0.0) && s1 > T(0.0))) {
-                    const T is1 = rcpT(s1);
-                    const T rho1 = qa[0] + fw1[0] * is1;
-                    const T rhou1 = qa[1] + fw1[1] * is1;
-                    const T rhov1 = qa[2] + fw1[2] * is1;
-                    const T rhow1 = qa[3] + fw1[3] * is1;
-                    const T en1 = qa[4] + fw1[4] * is1;
-                    const T ir1 = rcpT(rho1);
-                    const T p1 = gm1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) * ir1);
-                    const T cc1 = sqrtT(gmm * p1 * ir1);
-                    const T sa = rhou1 * ir1 - cc1;
-                    T sfract;
-                    if (s0 < T(0.0) && sa > T(0.0)) sfract = s0 * (sa - s1) / (sa - s0);
-                    else if (s1 < T(0.0)) sfract = s1;
-                    else sfract = T(0.0);
+                const T1 gmm = ga, gm1 = ga - T1(1.0);
+                const T1 pim1 = gm1 * (qa[4] - T1(0.5) * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * iqa);
+                const T1 cim1 = sqrt1<RF>(gmm * pim1 * iqa);
+                const T1 s0 = qa[1] * iqa - cim1;
+                if (!(s0 >= T1(0.0) && s1 > T1(0.0))) {
+                    const T1 is1 = rcp1<RF>(s1);
+                    const T1 rho1 = qa[0] + F1[0] * is1;
+                    const T1 rhou1 = qa[1] + F1[1] * is1;
+                    const T1 rhov1 = qa[2] + F1[2] * is1;
+                    const T1 rhow1 = qa[3] + F1[3] * is1;
+                    const T1 en1 = qa[4] + F1[4] * is1;
+                    const T1 ir1 = rcp1<RF>(rho1);
+                    const T1 p1 = gm1 * (en1 - T1(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) * ir1);
+                    const T1 cc1 = sqrt1<RF>(gmm * p1 * ir1);
+                    const T1 sa = rhou1 * ir1 - cc1;
+                    T1 sfract;
+                    if (s0 < T1(0.0) && sa > T1(0.0)) sfract = s0 * (sa - s1) / (sa - s0);
+                    else if (s1 < T1(0.0)) sfract = s1;
+                    else sfract = T1(0.0);
 #pragma unroll
-                    for (int m = 0; m < NC; m++) amdq[m] = sfract * fw1[m] * is1;
-                    if (s2 < T(0.0)) {
+                    for (int m = 0; m < NC; m++) AM[m] = sfract * F1[m] * is1;
+                    if (s2 < T1(0.0)) {
 #pragma unroll
-                        for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + fw2[m];
-                        const T gm = gb, g1 = gb - T(1.0);
-                        const T pi_ = g1 * (qb[4] - T(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
-                        const T ci = sqrtT(gm * pi_ * iqb);
-                        const T s3r = qb[1] * iqb + ci;
-                        const T is3 = rcpT(s3);
-                        const T rho2 = qb[0] - fw3[0] * is3;
-                        const T rhou2 = qb[1] - fw3[1] * is3;
-                        const T rhov2 = qb[2] - fw3[2] * is3;
-                        const T rhow2 = qb[3] - fw3[3] * is3;
-                        const T en2 = qb[4] - fw3[4] * is3;
-                        const T ir2 = rcpT(rho2);
-                        const T p2 = g1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) * ir2);
-                        const T cc2 = sqrtT(gm * p2 * ir2);
-                        const T sb = rhou2 * ir2 + cc2;
+                        for (int m = 0; m < NC; m++) AM[m] = AM[m] + F2[m];
+                        const T1 gm = gb, g1 = gb - T1(1.0);
+                        const T1 pi_ = g1 * (qb[4] - T1(0.5) * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * iqb);
+                        const T1 ci = sqrt1<RF>(gm * pi_ * iqb);
+                        const T1 s3r = qb[1] * iqb + ci;
+                        const T1 is3 = rcp1<RF>(s3);
+                        const T1 rho2 = qb[0] - F3[0] * is3;
+                        const T1 rhou2 = qb[1] - F3[1] * is3;
+                        const T1 rhov2 = qb[2] - F3[2] * is3;
+                        const T1 rhow2 = qb[3] - F3[3] * is3;
+                        const T1 en2 = qb[4] - F3[4] * is3;
+                        const T1 ir2 = rcp1<RF>(rho2);
+                        const T1 p2 = g1 * (en2 - T1(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) * ir2);
+                        const T1 cc2 = sqrt1<RF>(gm * p2 * ir2);
+                        const T1 sb = rhou2 * ir2 + cc2;
                         bool add = true;
-                        if (sb < T(0.0) && s3r > T(0.0)) sfract = sb * (s3r - s3) / (s3r - sb);
-                        else if (s3 < T(0.0)) sfract = s3;
+                        if (sb < T1(0.0) && s3r > T1(0.0)) sfract = sb * (s3r - s3) / (s3r - sb);
+                        else if (s3 < T1(0.0)) sfract = s3;
                         else add = false;
                         if (add) {
 #pragma unroll
-                            for (int m = 0; m < NC; m++) amdq[m] = amdq[m] + sfract * fw3[m] * is3;
+                            for (int m = 0; m < NC; m++) AM[m] = AM[m] + sfract * F3[m] * is3;
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int m = 0; m < NC; m++) apdq[m] = ((T(0.0) + fw1[m]) + fw2[m]) + fw3[m] - amdq[m];
+            for (int m = 0; m < NC; m++) AP[m] = ((T1(0.0) + F1[m]) + F2[m]) + F3[m] - AM[m];
             c1 = fabs(s1 * dtdn); c2 = fabs(s2 * dtdn); c3 = fabs(s3 * dtdn);
-            const T b2r = fw2[2] - b4 * R.v;
-            f1 = b1 != T(0.0); f2 = b2r != T(0.0); f5 = b5 != T(0.0);
+            const T1 b2r = F2[2] - b4 * RR.v;
+            f1 = b1 != T1(0.0); f2 = b2r != T1(0.0); f5 = b5 != T1(0.0);
+            W[0] = b1; W[1] = b2r; W[2] = F2[3] - b4 * RR.w; W[3] = b4; W[4] = b5;
             s_del[0][t] = d1; s_del[1][t] = d2; s_del[2][t] = d3; s_del[3][t] = d4; s_del[4][t] = d5;
             s_s2[t] = s2;
 #pragma unroll
-            for (int k = 0; k < NTR; k++) s_trw[k][t] = fw2[5 + k];
+            for (int k = 0; k < NTR; k++) s_trw[k][t] = F2[5 + k];
             s_flag[t] = (act2 && (f1 || f2 || f5)) ? 1 : 0;
             s_cfo[t] = f5 ? c3 : (f2 ? c2 : c1);
+        };
+        // FP32 form: the solve's FP64 Roe state and wave strengths (b1 .. b5),
+        // kept for the limiters' Lax-Liu ratios, whose upwind wave strengths
+        // cancel as the solve's do
+        RoeT<double> RD = {0, 0, 0, 0, 1, 0, 0, 0, 0};
+        double WB[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (act1) {
+            if constexpr (sizeof(T) == 8) {
+                riemann(R, fw1, fw2, fw3, amdq, apdq, WB);
+            } else {
+                RoeT<double> &RR = RD;
+                // zero as the FP32 arrays above: the solve leaves the tracer
+                // entries of waves 1 and 3 at zero
+                double F1[NC], F2[NC], F3[NC], AM[NC], AP[NC];
+#pragma unroll
+                for (int m = 0; m < NC; m++) F1[m] = F2[m] = F3[m] = AM[m] = AP[m] = 0.0;
+                riemann(RR, F1, F2, F3, AM, AP, WB);
+#pragma unroll
+                for (int m = 0; m < NC; m++) {
+                    fw1[m] = (T)F1[m]; fw2[m] = (T)F2[m]; fw3[m] = (T)F3[m];
+                    amdq[m] = (T)AM[m]; apdq[m] = (T)AP[m];
+                }
+                R = {(T)RR.u, (T)RR.v, (T)RR.w, (T)RR.enth, (T)RR.a, (T)RR.g1a2, (T)RR.euv, (T)RR.u2v2w2, (T)RR.i2a};
+            }
         } else {
             s_flag[t] = 0;
         }
@@ -367,24 +406,36 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                     fw2[5 + k] = wl * fc;
                 }
             }
-            const T b1 = fw1[0], b4 = fw2[0];
-            const T b2 = fw2[2] - b4 * R.v, b3 = fw2[3] - b4 * R.w, b5 = fw3[0];
+            // wave strengths and ratios in FP64 (TL); the limiter functions
+            // in the form's type
+            using TL = double;
+            const RoeT<double> &RL = limiter_roe(R, RD);
+            TL b1, b2, b3, b4, b5;
+            if constexpr (sizeof(T) == 8) {
+                b1 = fw1[0];
+                b4 = fw2[0];
+                b2 = fw2[2] - b4 * R.v;
+                b3 = fw2[3] - b4 * R.w;
+                b5 = fw3[0];
+            } else {
+                b1 = WB[0]; b2 = WB[1]; b3 = WB[2]; b4 = WB[3]; b5 = WB[4];
+            }
             T wl1 = T(1.0), wl2 = T(1.0), wl3 = T(1.0), wl4 = T(1.0), wl5 = T(1.0);
             auto del = [&](int k, int up) { return s_del[k][t + up * nb]; };
-            if (b1 != T(0.0)) {
+            if (b1 != TL(0.0)) {
                 const int up = (int)(-copysign(T(1.0), s1));
-                const T b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                              R.w * del(3, up) - del(4, up));
-                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) * R.i2a;
-                const T b1Id = del(0, up) - b4Id - b5Id;
-                wl1 = suppow(b1Id / b1, c1, c1);
+                const TL b4Id = RL.g1a2 * (RL.euv * del(0, up) + RL.u * del(1, up) + RL.v * del(2, up) +
+                                           RL.w * del(3, up) - del(4, up));
+                const TL b5Id = (del(1, up) + (RL.a - RL.u) * del(0, up) - RL.a * b4Id) * RL.i2a;
+                const TL b1Id = del(0, up) - b4Id - b5Id;
+                wl1 = suppow((T)(b1Id / b1), c1, c1);
             }
-            if (b2 != T(0.0)) {
+            if (b2 != TL(0.0)) {
                 const int up = (int)(-copysign(T(1.0), s2));
-                const T b2Id = del(2, up) - R.v * del(0, up);
-                wl2 = cflsupbee(T(1.0), beta, b2Id / b2, c2, c2);
+                const TL b2Id = del(2, up) - RL.v * del(0, up);
+                wl2 = cflsupbee(T(1.0), beta, (T)(b2Id / b2), c2, c2);
             }
-            if (b3 != T(0.0) || b4 != T(0.0)) {
+            if (b3 != TL(0.0) || b4 != TL(0.0)) {
                 T cflup;
                 if (f2) cflup = c2;
                 else if (f1) cflup = c1;
@@ -394,34 +445,34 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                         if (s_flag[t - (c - j) * nb]) { cflup = s_cfo[t - (c - j) * nb]; break; }
                 }
                 const int up = (int)(-copysign(T(1.0), s2));
-                if (b3 != T(0.0)) {
-                    const T b3Id = del(3, up) - R.w * del(0, up);
-                    wl3 = cflsupbee(T(1.0), beta, b3Id / b3, c2, cflup);
+                if (b3 != TL(0.0)) {
+                    const TL b3Id = del(3, up) - RL.w * del(0, up);
+                    wl3 = cflsupbee(T(1.0), beta, (T)(b3Id / b3), c2, cflup);
                 }
-                if (b4 != T(0.0)) {
-                    const T b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                                  R.w * del(3, up) - del(4, up));
-                    wl4 = cflsupbee(T(1.0), beta, b4Id / b4, c2, cflup);
+                if (b4 != TL(0.0)) {
+                    const TL b4Id = RL.g1a2 * (RL.euv * del(0, up) + RL.u * del(1, up) + RL.v * del(2, up) +
+                                               RL.w * del(3, up) - del(4, up));
+                    wl4 = cflsupbee(T(1.0), beta, (T)(b4Id / b4), c2, cflup);
                 }
             }
-            if (b5 != T(0.0)) {
+            if (b5 != TL(0.0)) {
                 const int up = (int)(-copysign(T(1.0), s3));
-                const T b4Id = R.g1a2 * (R.euv * del(0, up) + R.u * del(1, up) + R.v * del(2, up) +
-                                              R.w * del(3, up) - del(4, up));
-                const T b5Id = (del(1, up) + (R.a - R.u) * del(0, up) - R.a * b4Id) * R.i2a;
-                wl5 = suppow(b5Id / b5, c3, c3);
+                const TL b4Id = RL.g1a2 * (RL.euv * del(0, up) + RL.u * del(1, up) + RL.v * del(2, up) +
+                                           RL.w * del(3, up) - del(4, up));
+                const TL b5Id = (del(1, up) + (RL.a - RL.u) * del(0, up) - RL.a * b4Id) * RL.i2a;
+                wl5 = suppow((T)(b5Id / b5), c3, c3);
             }
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 fw1[m] = fw1[m] * wl1;
                 fw3[m] = fw3[m] * wl5;
             }
-            const T b2l = wl2 * b2, b3l = wl3 * b3, b4l = wl4 * b4;
-            fw2[0] = b4l;
-            fw2[1] = b4l * R.u;
-            fw2[2] = b2l + b4l * R.v;
-            fw2[3] = b3l + b4l * R.w;
-            fw2[4] = b2l * R.v + b3l * R.w + b4l * T(0.5) * R.u2v2w2;
+            const TL b2l = wl2 * b2, b3l = wl3 * b3, b4l = wl4 * b4;
+            fw2[0] = (T)b4l;
+            fw2[1] = (T)(b4l * RL.u);
+            fw2[2] = (T)(b2l + b4l * RL.v);
+            fw2[3] = (T)(b3l + b4l * RL.w);
+            fw2[4] = (T)(b2l * RL.v + b3l * RL.w + b4l * TL(0.5) * RL.u2v2w2);
             const T k1 = T(0.5) * copysign(T(1.0), s1) * (T(1.0) - fabs(s1) * dtdn);
             const T k2 = T(0.5) * copysign(T(1.0), s2) * (T(1.0) - fabs(s2) * dtdn);
             const T k3 = T(0.5) * copysign(T(1.0), s3) * (T(1.0) - fabs(s3) * dtdn);
@@ -708,17 +759,15 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 
 
 // Waves per SIMD the register allocation aims at: two for the FP32 form
-// with tile-axis-major lanes and no tracer, and with one tracer for the
-// work-list form and the one-grid form of shape 1 (left to themselves they
-// miss 256 registers by one and halve the occupancy: 50 % slower on the
-// full-domain FP32 step), one otherwise.  The one-grid kernel of shape 0
-// with one tracer reaches two waves without being asked, and asking changes
-// its schedule for the worse (6 % on the full-domain FP32 step).
+// with tile-axis-major lanes and at most one tracer (left to themselves
+// these kernels miss 256 registers by a few, since the Riemann solve runs in
+// FP64, and halve the occupancy: 50 % slower on the full-domain FP32 step),
+// one otherwise.
 template <typename T, int NTR, int SH, bool LIST>
 constexpr int tile_waves()
 {
     if (sizeof(T) != 4 || TileShape<SH>::PM) return 1;
-    return NTR == 0 || (NTR == 1 && (LIST || SH == 1)) ? 2 : 1;
+    return NTR <= 1 ? 2 : 1;
 }
 
 template <typename T, bool PERT, int NTR, int SH>

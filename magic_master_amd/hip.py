@@ -316,19 +316,21 @@ class MagicHIP:
 
 
 class MagicBatch:
-    """Many grids of the FP32 perturbation form advanced together with the
-    step control on the device (magic_batch_*).  Each scenario takes its
-    parameters, a background column (aux(1:maux, k) for every k) and q
-    (None: the fluid at rest on the background, qinit without noise)."""
+    """Many grids of one form (FORM_FP32P, the default, or FORM_FP64)
+    advanced together with the step control on the device (magic_batch_*).
+    Each scenario takes its parameters, a background column (aux(1:maux, k)
+    for every k) and q (None: the fluid at rest on the background, qinit
+    without noise)."""
 
-    def __init__(self, grids, params, aux_columns, qs=None, meqn=None, mbc=None, maux=None):
+    def __init__(self, grids, params, aux_columns, qs=None, meqn=None, mbc=None, maux=None, form=None):
         self.L = load_library()
         g0 = grids[0] if grids else None
         meqn = g0.meqn if g0 else meqn
         mbc = g0.mbc if g0 else mbc
         maux = g0.maux if g0 else maux
+        self.form = FORM_FP32P if form is None else form
         h = C.c_void_p()
-        self._check(self.L.magic_batch_create(0, None, meqn, mbc, maux, None, FORM_FP32P, C.byref(h)),
+        self._check(self.L.magic_batch_create(0, None, meqn, mbc, maux, None, self.form, C.byref(h)),
                     "magic_batch_create")
         self.h = h
         self.meqn, self.mbc, self.maux = meqn, mbc, maux

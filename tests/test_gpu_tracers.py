@@ -8,8 +8,9 @@ form), i.e. every sweep instantiation the library carries.
   (first attempt rejected) with every dt to 1e-12 and the state ≤ 1e-9.
 * FP32 form against the oracle after 10 claw3 steps: w, T', rho' within 1e-4
   of their peaks, tracers within 1e-4 (their range is [0, 1]).
-* Batched driver with 0 and 2 tracers: each scenario equals the same grid
-  stepped alone (sweeps in turn, the batch's default), bit for bit.
+* Batched driver with 0 and 2 tracers, FP32 and FP64 forms: each scenario
+  equals the same grid stepped alone in the same form (sweeps in turn, the
+  batch's default), bit for bit.
 """
 import numpy as np
 import pytest
@@ -108,16 +109,19 @@ def test_fp32_tracer_counts(meqn, nspec, oracle):
         assert np.abs(qg[it][..., 5:] - qo[it][..., 5:]).max() <= 1e-4
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp64"])
 @pytest.mark.parametrize("meqn,nspec", [(5, 0), (7, 2)])
-def test_batch_tracer_counts(meqn, nspec):
-    from magic_master_amd.hip import FORM_FP32P, MagicBatch, MagicHIP
+def test_batch_tracer_counts(meqn, nspec, form):
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicBatch, MagicHIP
+    fm = FORM_FP32P if form == "fp32" else FORM_FP64
     scen = []
     for k, (dims, mthbc) in enumerate([((12, 10, 8), [2, 2, 2, 2, 3, 1]), ((9, 11, 7), [1, 1, 1, 1, 3, 1])]):
         c, p, g, aux, q = perturbed(*dims, mthbc, seed=20 + k, vel=0.05, meqn=meqn, nspecadv=nspec)
         p.tcenter = 1e9          # forcing off: the batch and the single grid then agree bit for bit
         scen.append((c, p, g, aux, aux[:, 0, 0, :].copy(), q))
-    print(f"[tracers batch] meqn {meqn}", flush=True)
-    b = MagicBatch([s[2] for s in scen], [s[1] for s in scen], [s[4] for s in scen], [s[5] for s in scen])
+    print(f"[tracers batch] meqn {meqn} {form}", flush=True)
+    b = MagicBatch([s[2] for s in scen], [s[1] for s in scen], [s[4] for s in scen], [s[5] for s in scen],
+                   form=fm)
     try:
         for k, (c, p, g, aux, col, q) in enumerate(scen):
             b.start(k, 0.0, 1e300, 1.0, [1.0, c.dtv[1], 0, 0, 0], c.cflv, 4, c.mthbc)
@@ -127,7 +131,7 @@ def test_batch_tracer_counts(meqn, nspec):
     finally:
         b.close()
     for k, (c, p, g, aux, col, q) in enumerate(scen):
-        d = MagicHIP(g, p, aux, q, form=FORM_FP32P)
+        d = MagicHIP(g, p, aux, q, form=fm)
         d.set_sweep_overlap(False)   # the batch's default order of additions
         try:
             dtv = [1.0, c.dtv[1], 0, 0, 0]

@@ -40,9 +40,28 @@ struct ScenCtl {
     int err[6];
 };
 
+// Pointer into memory that no kernel writes while the reading kernel runs
+// (so that its uniform loads may be scalar loads).
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+// *p copied word by word (a struct's copy constructor takes a generic
+// reference, which does not bind to such memory)
+template <typename T>
+__device__ __forceinline__ T cload(cptr<T> p)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    T v;
+    unsigned *d = reinterpret_cast<unsigned *>(&v);
+    const cptr<unsigned> w = (cptr<unsigned>)p;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) d[i] = w[i];
+    return v;
+}
+
 // Gate of the shared kernels for one scenario of a batch.
 struct StepGate {
-    const ScenCtl *c;
+    cptr<ScenCtl> c;
     int mode, nn, kk, fslot;
     __device__ __forceinline__ bool on() const
     {

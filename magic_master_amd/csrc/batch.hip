@@ -51,6 +51,7 @@ struct magic_batch {
     int4 *lgdim_d = nullptr;             //                  cap grid shapes each
     long long ltotal[NLG] = {0};         // blocks of each list
     int ndtr_cap = 0, ndtrc_cap = 0;     // substep loops of the captured attempt
+    int opgrid[NOP] = {0};               // persistent grid of each kernel class
     // Sweep overlap, as for a single grid (off by default): the y-sweep and
     // the last three colour launches of the z-sweep add into each
     // scenario's dq (ctx dq[0]) on a second stream, the rest into q on the
@@ -268,28 +269,30 @@ static dim3 list_block(int lg)
     }
 }
 
-// One launch of kernel class op over block list lg (several when the list
-// exceeds what one dispatch may hold)
-template <typename T>
-static int launch_op(magic_batch *b, int op, int lg, GateSpec gs, int arg = 0)
+// The block list each kernel class walks
+static int op_list(int op)
 {
-    const long long total = b->ltotal[lg];
+    static const int lg[NOP] = {LG_FX,  LG_FY,  LG_BCZ, LG_ALL, LG_CPY, LG_CPY, LG_PAD, LG_FZ, LG_DIF,
+                                LG_DIF, LG_CP1, LG_INT, LG_FZ,  LG_SPL, LG_INT, LG_INT, LG_MRG};
+    return lg[op];
+}
+
+// One launch of kernel class op over its block list: a persistent grid of
+// at most as many workgroups as fit on the chip at once
+template <typename T>
+static int launch_op(magic_batch *b, int op, GateSpec gs, int arg = 0)
+{
+    const int lg = op_list(op);
+    const int total = (int)b->ltotal[lg];
     if (total <= 0) return MAGIC_OK;
     const void *fn = batch_op_fn<T>(op);
-    if (!fn) return MAGIC_EARG;
+    if (!fn || b->opgrid[op] < 1) return MAGIC_EARG;
     const BatchOps<T> *ops = (const BatchOps<T> *)b->ops_d;
     const int *off = b->loff_d + (size_t)lg * (b->cap + 1);
     const int4 *gd = b->lgdim_d + (size_t)lg * b->cap;
     int n = b->n;
-    const dim3 blk = list_block(lg);
-    // a dispatch holds fewer than 2^32 work-items
-    const long long chunk = (1ll << 31) / (blk.x * blk.y * blk.z);
-    for (long long base = 0; base < total; base += chunk) {
-        const unsigned nb = (unsigned)std::min(chunk, total - base);
-        int ib = (int)base;
-        void *kargs[] = {&ops, &off, &gd, &n, &gs, &arg, &ib};
-        HIPCHECK(hipLaunchKernel(fn, dim3(nb), blk, kargs, 0, b->stream));
-    }
+    void *kargs[] = {&ops, &off, &gd, &n, &gs, &arg};
+    HIPCHECK(hipLaunchKernel(fn, dim3(std::min(total, b->opgrid[op])), list_block(lg), kargs, 0, b->stream));
     return launch_check();
 }
 
@@ -301,31 +304,31 @@ static GateSpec gs(int mode, int nn = 0, int kk = 0, int fslot = -1) { return Ga
 template <typename T>
 static int launch_src(magic_batch *b)
 {
-    TRY(launch_op<T>(b, OP_CONVERT, LG_PAD, gs(GATE_ACCEPTED)));
+    TRY(launch_op<T>(b, OP_CONVERT, gs(GATE_ACCEPTED)));
     for (int nn = 1; nn <= b->ndtr_cap; nn++) {
         const GateSpec gd = gs(GATE_DIFF, nn);
         // bc3 with the z faces skipped (src3.f90:333-336)
-        TRY(launch_op<T>(b, OP_BCX, LG_FX, gd));
-        TRY(launch_op<T>(b, OP_BCY, LG_FY, gd));
-        TRY(launch_op<T>(b, OP_ZGHOST_UVW, LG_FZ, gd));
-        TRY(launch_op<T>(b, OP_DIFF_CROSS, LG_DIF, gs(GATE_DIFF, nn, 0, FAC_CROSS)));
+        TRY(launch_op<T>(b, OP_BCX, gd));
+        TRY(launch_op<T>(b, OP_BCY, gd));
+        TRY(launch_op<T>(b, OP_ZGHOST_UVW, gd));
+        TRY(launch_op<T>(b, OP_DIFF_CROSS, gs(GATE_DIFF, nn, 0, FAC_CROSS)));
         for (int cc = 1; cc <= 3; cc++)
-            TRY(launch_op<T>(b, OP_DIFF_XYZ, LG_DIF, gs(GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)), cc));
-        TRY(launch_op<T>(b, OP_COPY_Q5, LG_CP1, gd));
+            TRY(launch_op<T>(b, OP_DIFF_XYZ, gs(GATE_DIFF, nn, 0, FAC_DIFF + 3 * (cc - 1)), cc));
+        TRY(launch_op<T>(b, OP_COPY_Q5, gd));
         for (int kk = 1; kk <= b->ndtrc_cap; kk++) {
             const GateSpec gk = gs(GATE_COND, nn, kk);
-            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND, nn, kk, FAC_COND + 0), PASS_QT_UT * 4 + 0));
-            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND, nn, kk, FAC_COND + 1), PASS_UT_QT * 4 + 1));
-            TRY(launch_op<T>(b, OP_CONDUCT_ZGHOST, LG_FZ, gk));
-            TRY(launch_op<T>(b, OP_ZSPILL, LG_SPL, gk));
+            TRY(launch_op<T>(b, OP_PASS, gs(GATE_COND, nn, kk, FAC_COND + 0), PASS_QT_UT * 4 + 0));
+            TRY(launch_op<T>(b, OP_PASS, gs(GATE_COND, nn, kk, FAC_COND + 1), PASS_UT_QT * 4 + 1));
+            TRY(launch_op<T>(b, OP_CONDUCT_ZGHOST, gk));
+            TRY(launch_op<T>(b, OP_ZSPILL, gk));
             if (kk < b->ndtrc_cap) {   // at kk = ndtrc_cap >= ndtrc these never run
-                TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND_MORE, nn, kk, FAC_COND + 2), PASS_QT_UT * 4 + 2));
-                TRY(launch_op<T>(b, OP_COPY_INTERIOR, LG_INT, gs(GATE_COND_MORE, nn, kk)));
+                TRY(launch_op<T>(b, OP_PASS, gs(GATE_COND_MORE, nn, kk, FAC_COND + 2), PASS_QT_UT * 4 + 2));
+                TRY(launch_op<T>(b, OP_COPY_INTERIOR, gs(GATE_COND_MORE, nn, kk)));
             }
-            TRY(launch_op<T>(b, OP_PASS, LG_INT, gs(GATE_COND_LAST, nn, kk, FAC_COND + 2), PASS_Q5_UT * 4 + 2));
+            TRY(launch_op<T>(b, OP_PASS, gs(GATE_COND_LAST, nn, kk, FAC_COND + 2), PASS_Q5_UT * 4 + 2));
         }
     }
-    return launch_op<T>(b, OP_FINISH, LG_INT, gs(GATE_ACCEPTED));
+    return launch_op<T>(b, OP_FINISH, gs(GATE_ACCEPTED));
 }
 
 // one attempt of every scenario, as launched into the capture
@@ -336,16 +339,16 @@ static int record_attempt(magic_batch *b)
     k_batch_prepare<T><<<nb, 256, 0, b->stream>>>(b->ctl_d, b->n, (SweepArgsT<T> *)b->args_d, b->counters_d);
     TRY(launch_check());
     const GateSpec run = gs(GATE_RUN);
-    TRY(launch_op<T>(b, OP_BCX, LG_FX, run));
-    TRY(launch_op<T>(b, OP_BCY, LG_FY, run));
-    TRY(launch_op<T>(b, OP_BCZ, LG_BCZ, run));
-    if (b->meqn > 5) TRY(launch_op<T>(b, OP_CLAMP, LG_ALL, run));
-    TRY(launch_op<T>(b, OP_BACKUP, LG_CPY, run));
+    TRY(launch_op<T>(b, OP_BCX, run));
+    TRY(launch_op<T>(b, OP_BCY, run));
+    TRY(launch_op<T>(b, OP_BCZ, run));
+    if (b->meqn > 5) TRY(launch_op<T>(b, OP_CLAMP, run));
+    TRY(launch_op<T>(b, OP_BACKUP, run));
     TRY((launch_sweeps<T, PERT>(b)));
     k_batch_decide<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     TRY(launch_check());
-    if (b->overlap) TRY(launch_op<T>(b, OP_MERGE, LG_MRG, gs(GATE_SWEPT)));
-    TRY(launch_op<T>(b, OP_RESTORE, LG_CPY, gs(GATE_REJECTED)));
+    if (b->overlap) TRY(launch_op<T>(b, OP_MERGE, gs(GATE_SWEPT)));
+    TRY(launch_op<T>(b, OP_RESTORE, gs(GATE_REJECTED)));
     TRY(launch_src<T>(b));
     k_batch_finalize<<<blocks(b->n, 256), 256, 0, b->stream>>>(b->ctl_d, b->n);
     return launch_check();
@@ -493,6 +496,13 @@ static void form_ops(const magic_ctx *c, BatchOps<double> &o)
     o.zratio = c->zratio;
 }
 
+// Layers per workgroup of a scenario's diffusion passes.  A workgroup
+// recomputes the x and y passes of the two layers around its own; one grid
+// takes few layers (diff_kb) to have enough workgroups, a batch has enough
+// from its scenarios and takes at least 16.  The result does not depend on
+// the layer count.
+static int batch_kb(const Dims &d) { return std::max(diff_kb(d), std::min(16, d.mz)); }
+
 // Grid of list lg for one scenario (x, y, z blocks; x = 0: no blocks)
 static int4 list_grid(const magic_batch *b, const magic_ctx *c, const int *bc, int lg)
 {
@@ -508,7 +518,7 @@ static int4 list_grid(const magic_batch *b, const magic_ctx *c, const int *bc, i
     case LG_CPY: return g1(std::min<long long>(8192, blocks(P * b->meqn, 256)));
     case LG_CP1: return g1(std::min<long long>(8192, blocks(P, 256)));
     case LG_PAD: { const dim3 g = padded_cells(d); return make_int4(g.x, g.y, g.z, 0); }
-    case LG_DIF: { const dim3 g = diff_grid(d); return make_int4(g.x, g.y, g.z, 0); }
+    case LG_DIF: return make_int4(blocks(d.mx, 64), blocks(d.my, 4), blocks(d.mz, batch_kb(d)), 0);
     case LG_INT: { const dim3 g = interior_grid(d); return make_int4(g.x, g.y, g.z, 0); }
     case LG_SPL: return g1(c->mz > c->mx ? blocks(c->mz - c->mx, 64) : 0);
     case LG_MRG: {
@@ -547,15 +557,15 @@ static int build_ops(magic_batch *b)
         magic_ctx *c = b->sc[s];
         BatchOps<T> &o = ops[s];
         memset(&o, 0, sizeof(o));
+        o.ctl = b->ctl_d + s;   // an empty slot's control block is all zeros: every gate off
         if (!c) continue;
         o.d = c->d;
         form_ops(c, o);
         o.dq = (T *)c->dq[0];
         o.nc = c->meqn;
-        o.kb = diff_kb(c->d);
+        o.kb = batch_kb(c->d);
         o.spill = c->mz > c->mx ? c->mz - c->mx : 0;
         memcpy(o.bc, &b->mthbc[6 * s], sizeof(o.bc));
-        o.ctl = b->ctl_d + s;
         const magic_params &p = c->prm;
         Forcing &f = o.f;
         f.forcemth = (int)p.forcemth;
@@ -576,7 +586,8 @@ static int build_ops(magic_batch *b)
     return MAGIC_OK;
 }
 
-// persistent grid of each direction: as many workgroups as fit on the chip
+// persistent grid of each sweep direction and of each kernel class of the
+// work-list launches: as many workgroups as fit on the chip at once
 template <typename T, bool PERT>
 static int list_grids(magic_batch *b)
 {
@@ -589,6 +600,14 @@ static int list_grids(magic_batch *b)
         int occ = 0;
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
         b->grid[dir] = ncu * (occ > 0 ? occ : 1);
+    }
+    for (int op = 0; op < NOP; op++) {
+        const void *fn = batch_op_fn<T>(op);
+        if (!fn) return MAGIC_EUNSUPPORTED;
+        const dim3 blk = list_block(op_list(op));
+        int occ = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, (int)(blk.x * blk.y * blk.z), 0));
+        b->opgrid[op] = ncu * (occ > 0 ? occ : 1);
     }
     return MAGIC_OK;
 }
@@ -692,6 +711,10 @@ extern "C" int magic_batch_add(magic_batch *b, const int dims[3], const double g
     c->own_stream = false;
     int s = 0;
     while (s < b->n && b->sc[s]) s++;
+    if (s == BATCH_MAX_SLOTS) {
+        magic_destroy(c);
+        return MAGIC_EUNSUPPORTED;
+    }
     if (s == b->n) {
         const int st = reserve(b, b->n + 1);
         if (st != MAGIC_OK) {

@@ -1,11 +1,13 @@
 // Boundary, pre-step, copy and source-pass kernels of the batched driver as
 // work-list launches: one launch of a kernel class covers every scenario of
-// a batch.  The launch's blocks are the scenarios' own grids laid end to end
-// (scenario s owns blocks off[s] .. off[s+1]-1, a grid of gdim[s] blocks), so
-// a block finds its scenario by a binary search over off and runs the
-// single-grid kernel's body (stencil_body.h, bc_src.hip, bc_src32.hip) on that
-// scenario's operands, gated by its step-control block (batch.h).  The number
-// of launches per attempt does not depend on the number of scenarios.
+// a batch.  The list's blocks are the scenarios' own grids laid end to end
+// (scenario s owns blocks off[s] .. off[s+1]-1, a grid of gdim[s] blocks); a
+// persistent grid walks the list, finds each block's scenario by a binary
+// search over off and runs the single-grid kernel's body (stencil_body.h,
+// bc_src.hip, bc_src32.hip) on that scenario's operands, gated by its
+// step-control block (batch.h).  The number of launches per attempt does not
+// depend on the number of scenarios, and a scenario whose gate is off (not
+// running, or past its own substep counts) costs next to nothing.
 //
 // The kernel template is defined here and instantiated in bc_src32.hip (FP32
 // form) and bc_src.hip (FP64 form), each under its form's floating-point
@@ -60,21 +62,20 @@ struct GateSpec {
     int mode, nn, kk, fslot;
 };
 
-// The scenario of list block b and the block's index within that
-// scenario's grid: the largest s with off[s] <= b (empty slots, whose off
-// equals the next slot's, are passed over).
-__device__ __forceinline__ int list_scenario(const int *__restrict__ off, const int4 *__restrict__ gdim, int n,
-                                             int b, uint3 &bi)
+// Slots a batch may hold (the work-list kernels keep one offset per slot in
+// LDS)
+constexpr int BATCH_MAX_SLOTS = 2048;
+
+// The slot of active block a: the largest s with pre[s] <= a (slots with
+// no active blocks, whose pre equals the next slot's, are passed over)
+__device__ __forceinline__ int active_find(const int *pre, int n, int a)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= b) lo = mid;
+        if (pre[mid] <= a) lo = mid;
         else hi = mid - 1;
     }
-    const int l = b - off[lo];
-    const int4 g = gdim[lo];
-    bi = make_uint3((unsigned)(l % g.x), (unsigned)((l / g.x) % g.y), (unsigned)(l / (g.x * g.y)));
     return lo;
 }
 
@@ -105,57 +106,125 @@ __device__ __forceinline__ void b_merge(const uint3 BI, unsigned nblk, T *__rest
     }
 }
 
-// One launch of kernel class OP over list blocks base .. base+gridDim.x-1
-// of the scenarios' list (off, gdim); arg: the velocity component of
-// OP_DIFF_XYZ (1-3), PASS_* * 4 + axis of OP_PASS.  The form's own bodies are
-// reached through the FORM functor (FP32: bc_src32.hip, FP64: bc_src.hip).
+// One launch of kernel class OP over the blocks of the scenarios' list
+// (off, gdim) whose gates are on.  Each workgroup of the persistent grid
+// first evaluates every slot's gate and lays the active slots' blocks end to
+// end in LDS (pre: active blocks before each slot), then takes active
+// blocks blockIdx.x, blockIdx.x + gridDim.x, ...; a launch whose gates are
+// all off ends after that scan.  arg: the velocity component of
+// OP_DIFF_XYZ (1-3), PASS_* * 4 + axis of OP_PASS.  The form's own bodies
+// are reached through the FORM functor (FP32: bc_src32.hip, FP64:
+// bc_src.hip).  n <= BATCH_MAX_SLOTS (the host checks).
 template <typename T, int OP, class FORM>
 __global__ __launch_bounds__(256) void k_batch_op(const BatchOps<T> *__restrict__ S, const int *__restrict__ off,
-                                                  const int4 *__restrict__ gdim, int n, GateSpec gs, int arg,
-                                                  int base)
+                                                  const int4 *__restrict__ gdim, int n, GateSpec gs, int arg)
 {
-    uint3 bi;
-    const int s = list_scenario(off, gdim, n, (int)blockIdx.x + base, bi);
-    const BatchOps<T> &o = S[s];
-    const StepGate g = {o.ctl, gs.mode, gs.nn, gs.kk, gs.fslot};
-    const long long P = o.d.P;
-    if constexpr (OP == OP_BCX) {
-        b_bc_x(bi, o.d, o.q, o.nc, lohi(o.bc[0]), lohi(o.bc[1]), g);
-    } else if constexpr (OP == OP_BCY) {
-        b_bc_y(bi, o.d, o.q, o.nc, lohi(o.bc[2]), lohi(o.bc[3]), g);
-    } else if constexpr (OP == OP_BCZ) {
-        FORM::bc_z(bi, o, g);
-    } else if constexpr (OP == OP_CLAMP) {
-        b_clamp(bi, o.q, P, o.nc, g);
-    } else if constexpr (OP == OP_BACKUP) {
-        b_copy(bi, (unsigned)gdim[s].x, o.qold, o.q, P * o.nc, g);
-    } else if constexpr (OP == OP_RESTORE) {
-        b_copy(bi, (unsigned)gdim[s].x, o.q, o.qold, P * o.nc, g);
-    } else if constexpr (OP == OP_CONVERT) {
-        FORM::convert(bi, o, g);
-    } else if constexpr (OP == OP_ZGHOST_UVW) {
-        b_zghost_uvw(bi, o.d, o.q, g);
-    } else if constexpr (OP == OP_DIFF_CROSS) {
-        b_diff_cross<T, true>(bi, o.d, o.u, o.q, FORM::nu(o), T(0.0), T(0.0), T(0.0), o.kb, g);
-    } else if constexpr (OP == OP_DIFF_XYZ) {
-        b_diff_xyz<T, true>(bi, o.d, o.q + arg * P, o.u + (arg - 1) * P, FORM::nu(o), T(0.0), T(0.0), T(0.0), o.kb, g);
-    } else if constexpr (OP == OP_COPY_Q5) {
-        b_copy(bi, (unsigned)gdim[s].x, o.ut, o.q + 4 * P, P, g);
-    } else if constexpr (OP == OP_PASS) {
-        const int v = arg >> 2, axis = arg & 3;
-        T *out = v == PASS_QT_UT ? o.qt : v == PASS_UT_QT ? o.ut : o.q + 4 * P;
-        const T *src = v == PASS_UT_QT ? o.qt : o.ut;
-        b_pass<T, true>(bi, o.d, out, src, o.ut, FORM::kap(o), T(0.0), axis, g);
-    } else if constexpr (OP == OP_CONDUCT_ZGHOST) {
-        b_conduct_zghost(bi, o.d, o.ut, g);
-    } else if constexpr (OP == OP_ZSPILL) {
-        b_conduct_zspill(bi, o.d, o.ut, o.spill, g);
-    } else if constexpr (OP == OP_COPY_INTERIOR) {
-        b_copy_interior(bi, o.d, o.ut, o.qt, 1, g);
-    } else if constexpr (OP == OP_FINISH) {
-        FORM::finish(bi, o, g);
-    } else if constexpr (OP == OP_MERGE) {
-        b_merge(bi, (unsigned)gdim[s].x, o.q, o.dq, P * o.nc, g);
+    // No kernel of an attempt writes the operand blocks or the lists, and
+    // only prepare, decide and finalize write the control blocks: read
+    // through the constant address space, their values stay in scalar
+    // registers although the bodies store to q between the reads.
+    const cptr<BatchOps<T>> CS = (cptr<BatchOps<T>>)S;
+    const cptr<int> coff = (cptr<int>)off;
+    const cptr<int> cgdim = (cptr<int>)gdim;   // 4 ints per scenario
+    __shared__ int pre[BATCH_MAX_SLOTS + 1];
+    __shared__ int wsum[4];
+    {
+        // thread t owns slots t*per .. t*per+per-1: their active block
+        // counts go to pre, then their exclusive prefix replaces them
+        const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        const int nt = (int)(blockDim.x * blockDim.y);   // 64 or 256 lanes
+        const int per = (n + nt - 1) / nt;
+        int sum = 0;
+        for (int i = 0; i < per; i++) {
+            const int sl = t * per + i;
+            if (sl >= n) break;
+            // a slot with no blocks here (an empty slot among them) is not
+            // looked at further
+            int c = coff[sl + 1] - coff[sl];
+            if (c > 0) {
+                const StepGate g = {(cptr<ScenCtl>)CS[sl].ctl, gs.mode, gs.nn, gs.kk, gs.fslot};
+                if (!g.on()) c = 0;
+            }
+            pre[sl] = c;
+            sum += c;
+        }
+        const int lane = t & 63, wave = t >> 6;
+        int v = sum;   // inclusive scan over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(v, d, 64);
+            if (lane >= d) v += y;
+        }
+        if (lane == 63) wsum[wave] = v;
+        __syncthreads();
+        int acc = v - sum;
+        for (int w = 0; w < wave; w++) acc += wsum[w];
+        for (int i = 0; i < per; i++) {
+            const int sl = t * per + i;
+            if (sl >= n) break;
+            const int c = pre[sl];
+            pre[sl] = acc;
+            acc += c;
+        }
+        if (t == nt - 1) pre[n] = acc;
+        __syncthreads();
+    }
+    const int total = pre[n];
+    const int G = (int)gridDim.x;
+    int s = 0, lo = 0, hi = 0;   // the current slot and its active blocks [lo, hi)
+    for (int b = (int)blockIdx.x; b < total; b += G) {
+        if (b < lo || b >= hi) {
+            s = __builtin_amdgcn_readfirstlane(active_find(pre, n, b));
+            lo = __builtin_amdgcn_readfirstlane(pre[s]);
+            hi = __builtin_amdgcn_readfirstlane(pre[s + 1]);
+        }
+        const BatchOps<T> o = cload(CS + s);
+        const StepGate g = {(cptr<ScenCtl>)o.ctl, gs.mode, gs.nn, gs.kk, gs.fslot};
+        const int l = b - lo;
+        const int gx = cgdim[4 * s], gy = cgdim[4 * s + 1];
+        const uint3 bi = make_uint3((unsigned)(l % gx), (unsigned)((l / gx) % gy), (unsigned)(l / (gx * gy)));
+        const long long P = o.d.P;
+        if constexpr (OP == OP_BCX) {
+            b_bc_x(bi, o.d, o.q, o.nc, lohi(o.bc[0]), lohi(o.bc[1]), g);
+        } else if constexpr (OP == OP_BCY) {
+            b_bc_y(bi, o.d, o.q, o.nc, lohi(o.bc[2]), lohi(o.bc[3]), g);
+        } else if constexpr (OP == OP_BCZ) {
+            FORM::bc_z(bi, o, CS + s, g);
+        } else if constexpr (OP == OP_CLAMP) {
+            b_clamp(bi, o.q, P, o.nc, g);
+        } else if constexpr (OP == OP_BACKUP) {
+            b_copy(bi, (unsigned)gx, o.qold, o.q, P * o.nc, g);
+        } else if constexpr (OP == OP_RESTORE) {
+            b_copy(bi, (unsigned)gx, o.q, o.qold, P * o.nc, g);
+        } else if constexpr (OP == OP_CONVERT) {
+            FORM::convert(bi, o, g);
+        } else if constexpr (OP == OP_ZGHOST_UVW) {
+            b_zghost_uvw(bi, o.d, o.q, g);
+        } else if constexpr (OP == OP_DIFF_CROSS) {
+            b_diff_cross<T, true>(bi, o.d, o.u, o.q, FORM::nu(o), T(0.0), T(0.0), T(0.0), o.kb, g);
+            __syncthreads();   // the next block's writers of the tile arrays wait for this one's readers
+        } else if constexpr (OP == OP_DIFF_XYZ) {
+            b_diff_xyz<T, true>(bi, o.d, o.q + arg * P, o.u + (arg - 1) * P, FORM::nu(o), T(0.0), T(0.0), T(0.0), o.kb,
+                                g);
+            __syncthreads();
+        } else if constexpr (OP == OP_COPY_Q5) {
+            b_copy(bi, (unsigned)gx, o.ut, o.q + 4 * P, P, g);
+        } else if constexpr (OP == OP_PASS) {
+            const int v = arg >> 2, axis = arg & 3;
+            T *out = v == PASS_QT_UT ? o.qt : v == PASS_UT_QT ? o.ut : o.q + 4 * P;
+            const T *src = v == PASS_UT_QT ? o.qt : o.ut;
+            b_pass<T, true>(bi, o.d, out, src, o.ut, FORM::kap(o), T(0.0), axis, g);
+        } else if constexpr (OP == OP_CONDUCT_ZGHOST) {
+            b_conduct_zghost(bi, o.d, o.ut, g);
+        } else if constexpr (OP == OP_ZSPILL) {
+            b_conduct_zspill(bi, o.d, o.ut, o.spill, g);
+        } else if constexpr (OP == OP_COPY_INTERIOR) {
+            b_copy_interior(bi, o.d, o.ut, o.qt, 1, g);
+        } else if constexpr (OP == OP_FINISH) {
+            FORM::finish(bi, o, g);
+        } else if constexpr (OP == OP_MERGE) {
+            b_merge(bi, (unsigned)gx, o.q, o.dq, P * o.nc, g);
+        }
     }
 }
 

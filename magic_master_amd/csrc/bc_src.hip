@@ -168,7 +168,7 @@ __global__ void k_planes_to_aos(double *aos, const double *planes, long long P, 
 // The FP64 form's own bodies for the batched work-list kernels
 // (batch_ops.h); its scenarios keep background columns
 struct Form64 {
-    __device__ static void bc_z(uint3 bi, const BatchOps<double> &o, const StepGate &g)
+    __device__ static void bc_z(uint3 bi, const BatchOps<double> &o, cptr<BatchOps<double>>, const StepGate &g)
     {
         b_bc_z(bi, o.d, o.q, o.nc, lohi(o.bc[4]), lohi(o.bc[5]), o.aux1, o.aux2, o.aux3, o.aux5, o.zratio, g);
     }

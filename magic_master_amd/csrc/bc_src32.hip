@@ -36,8 +36,11 @@ template __global__ void k_conduct_zspill<float, NoGate>(Dims, float *, int, NoG
 // for rho, the horizontal momenta and E, and q_ghost = q_boundary * sc for
 // the vertical momentum; with a background at rest every one of them is
 // q'_ghost = q'_boundary * sc.  sc depends on the layer only.
-template <class G>
-__device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int nc, int lo, int hi, Zr32 zr, const G &gate)
+// zf(side, kbc - 1): the factor sc of ghost layer kbc on side 0 (bottom) or 1
+// (top), Zr32's s[side][kbc - 1]
+template <class G, class ZF>
+__device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int nc, int lo, int hi, ZF zf,
+                                         const G &gate)
 {
     if (!gate.on()) return;
     const int i = (int)(BI.x * blockDim.x + threadIdx.x) + 1 - d.mbc;
@@ -51,7 +54,7 @@ __device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int n
             for (int kbc = 1; kbc <= d.mbc; kbc++) {
                 const int kg = side == 0 ? 1 - kbc : d.mz + kbc;
                 const long long cg = cid(d, i, j, kg);
-                const float sc = zr.s[side][kbc - 1];
+                const float sc = zf(side, kbc - 1);
                 for (int m = 0; m < 5; m++) q[m * d.P + cg] = q[m * d.P + cb] * sc;
                 for (int m = 5; m < nc; m++) q[m * d.P + cg] = q[m * d.P + cb];
             }
@@ -74,7 +77,8 @@ __device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int n
 template <class G>
 __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate)
 {
-    b32_bc_z(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi, zr, gate);
+    b32_bc_z(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, nc, lo, hi,
+             [&](int side, int k) { return zr.s[side][k]; }, gate);
 }
 
 // conserved perturbation -> (rho', u, v, w, T') on every cell
@@ -159,9 +163,11 @@ __global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G
 
 // The FP32 form's own bodies for the batched work-list kernels (batch_ops.h)
 struct Form32 {
-    __device__ static void bc_z(uint3 bi, const BatchOps<float> &o, const StepGate &g)
+    // po: the operand block in memory (the factors are indexed by layer)
+    __device__ static void bc_z(uint3 bi, const BatchOps<float> &o, cptr<BatchOps<float>> po, const StepGate &g)
     {
-        b32_bc_z(bi, o.d, o.q, o.nc, lohi(o.bc[4]), lohi(o.bc[5]), o.zr, g);
+        b32_bc_z(bi, o.d, o.q, o.nc, lohi(o.bc[4]), lohi(o.bc[5]),
+                 [=](int side, int k) { return po->zr.s[side][k]; }, g);
     }
     __device__ static void convert(uint3 bi, const BatchOps<float> &o, const StepGate &g)
     {

@@ -70,6 +70,33 @@ def measured_traffic(kernel, shapes):
     return None, None
 
 
+def run_active(steps, warmup, form, shapes):
+    """The full-domain workload with every part of the step at work: the
+    plume forcing at its peak over the timed window (tcenter = 0, so the
+    heating is above threshold in the plume's cells) and a 1e-3 density
+    perturbation in every cell (qinit's noise), so every wave family is
+    nonzero at every interface and every limiter branch runs.  Same tile
+    shapes as the main run; steps accepted steps timed on the host clock."""
+    from magic_master_amd import problem as P
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
+
+    c, p, g, aux, _ = P.config_full()
+    p.tcenter = 0.0
+    q = P.qinit(g, aux, noise=1e-3, seed=7, tracer_blob=dict(z0=10e3, width=2e3, amp=1.0))
+    dev = MagicHIP(g, p, aux, q, form=FORM_FP32P if form == "fp32" else FORM_FP64)
+    try:
+        dev.set_tile_shapes(shapes)
+        t, dtv, _ = dev.run_steps(c, warmup, t0=0.0)
+        t_start = t   # claw3 returns after its last step has finished on the device
+        tw0 = time.perf_counter()
+        t, dtv, lg = dev.run_steps(c, steps, t0=t, dt0=dtv[0])
+        wall = time.perf_counter() - tw0
+    finally:
+        dev.close()
+    return dict(value=g.ncell * steps / wall / 1e6, wall=wall, attempts=len(lg.dt),
+                window_s=[round(t_start, 3), round(t, 3)])
+
+
 def run_gpu(workload, steps, warmup, form="fp64", tune=True, shapes=None):
     from magic_master_amd import problem as P
     from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
@@ -208,17 +235,41 @@ def _cpu_slab(budget_s):
     return g.ncell * steps, el, steps
 
 
+def _host_cpu():
+    """(CPU model, logical CPUs, whether it runs x86-64-v4 code) of this host."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    return model, os.cpu_count() or 1, v4
+
+
+def _ref_speed_build():
+    """Whether the timing build of the reference (x86-64-v4) is there and this
+    host runs it."""
+    from oracle.oracle import Reference
+    return Reference.available(speed=True) and _host_cpu()[2]
+
+
 def _cpu_slab_reference(budget_s):
     """One host core: the reference's own Fortran step (claw3 with step3,
     flux3, rpn3euX/rpt3euX/rptt3euX, bc3_mpi, src3 as compiled from its
-    sources into oracle/_ref, flang -O2, one MPI rank) on the same slab;
-    its console lines go to /dev/null."""
+    sources into oracle/_ref, one MPI rank; the timing build, -O3 with FMA
+    and AVX-512, where this host runs it, else the -O2 parity build) on the
+    same slab; its console lines go to /dev/null."""
     from magic_master_amd import problem as P
     from oracle.oracle import Reference
 
     null = os.open(os.devnull, os.O_WRONLY)
     os.dup2(null, 1)
-    R = Reference.get()
+    R = Reference.get(speed=_ref_speed_build())
     c, p, g, aux, q = P.config_full(mx=250, my=16, mz=300)
     R.setup(p, c.mthbc)
     dtv = list(c.dtv)
@@ -240,10 +291,13 @@ def _cpu_slab_reference(budget_s):
 def cpu_baseline(budget_s=20.0, workers=None, kind="port"):
     """The same step on the host's cores: one process per core (at most 16,
     the host share of one GPU), each stepping its own 250 x 16 x 300 slab of
-    the full-domain workload with the same column and physics; the
-    throughput is the sum over the processes.  kind "port": the C port
-    (oracle); "reference": the reference's Fortran compiled into oracle/_ref.
-    Runs before the process touches the GPU (the workers are forked)."""
+    the full-domain workload with the same column and physics; the slabs
+    exchange no halos (each is stepped as a domain of its own, the
+    reference's bc3 on every face), so the sum over the processes is an
+    upper bound on what the decomposed domain would reach.  kind "port":
+    the C port (oracle); "reference": the reference's Fortran compiled into
+    oracle/_ref.  Runs before the process touches the GPU (the workers are
+    forked)."""
     import multiprocessing as mp
     n = workers or max(1, min(16, os.cpu_count() or 1))
     fn = _cpu_slab if kind == "port" else _cpu_slab_reference
@@ -252,12 +306,17 @@ def cpu_baseline(budget_s=20.0, workers=None, kind="port"):
     value = sum(cu / el for cu, el, _ in res) / 1e6
     steps = sum(st for _, _, st in res)
     el = max(el for _, el, _ in res)
-    what = ("C port of the step (oracle/magic_oracle.c, gcc -O2)" if kind == "port" else
-            "the reference's Fortran step (claw3/step3/flux3/rpn3euX/src3 compiled from its sources into "
-            "oracle/_ref, flang -O2, one MPI rank per process)")
+    model, ncpu, _ = _host_cpu()
+    if kind == "port":
+        what = "C port of the step (oracle/magic_oracle.c, gcc -O2)"
+    else:
+        flags = "flang -O3 -march=x86-64-v4 -ffp-contract=fast" if _ref_speed_build() else "flang -O2"
+        what = ("the reference's Fortran step (claw3/step3/flux3/rpn3euX/src3 compiled from its sources into "
+                f"oracle/_ref, {flags}, one MPI rank per process)")
     return dict(value=value, unit="Mcell-updates/s", cores=n, kind=kind,
-                sample=f"{what}, {n} processes on one core each, each on a 250x16x300 slab of the full-domain "
-                       f"workload: {steps} accepted steps in all within {el:.1f} s")
+                sample=f"{what}, {n} processes on one core each of a {ncpu}-CPU host ({model}; 16 CPUs are "
+                       f"one GPU's host share), each on a 250x16x300 slab of the full-domain workload, the "
+                       f"slabs exchanging no halos: {steps} accepted steps in all within {el:.1f} s")
 
 
 def copy_calibration():
@@ -392,6 +451,16 @@ def main():
                                      "frac": round(value * 1e6 * FLOP_PER_CELL_UPDATE / 1e12
                                                    / VECTOR_PEAK_TFLOPS[a.form], 4)}},
     }
+    if a.workload == "full":
+        act = run_active(K, a.warmup, a.form, r["shapes"])
+        out["plume_active"] = {
+            "value": round(act["value"], 3), "unit": "Mcell-updates/s", "steps": K,
+            "ms_per_step": round(act["wall"] / K * 1e3, 4), "attempts": act["attempts"],
+            "model_time_window_s": act["window_s"],
+            "what": ("the same workload with the plume forcing at its peak in the timed window (tcenter = 0) "
+                     "and a 1e-3 density perturbation in every cell: every wave family nonzero and every "
+                     "limiter branch taken (value is timed on the fluid at rest with the forcing below "
+                     "threshold)")}
     ens_shapes = None
     if not a.no_ensemble:
         e = run_ensemble(a.ensemble_steps, 2, tune=not a.no_tune)

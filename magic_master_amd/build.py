@@ -130,6 +130,8 @@ def build_reference(ref_root: str = "/root/reference"):
     if not os.path.isdir(ref_root):
         return None
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), f"REF={ref_root}"])
+    # the timing build of the same sources (the CPU baseline's reference kind)
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref"), f"REF={ref_root}", "speed"])
     return os.path.join(ROOT, "oracle", "_ref", "libmagic_ref.so")
 
 

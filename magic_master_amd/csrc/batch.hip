@@ -41,7 +41,7 @@ struct magic_batch {
     const int **run_d = nullptr;         // per argument block: &ctl_d[s].run
     int4 *items_d = nullptr;
     int item_off[19] = {0};              // work list of direction d, colour k at [item_off[6d+k], ..)
-    int *counters_d = nullptr;           // 18 work counters
+    int *counters_d = nullptr;           // 18 pairs of work counters (tile_sweep_list)
     int shape[3] = {0, 0, 0};            // tile shape of the x-, y- and z-sweeps
     int grid[3] = {0, 0, 0};             // persistent grid size of each direction's kernel
     int *one_d = nullptr;                // a device 1: the run flag of every tile while tuning
@@ -82,7 +82,7 @@ template <typename T>
 __global__ void k_batch_prepare(ScenCtl *ctl, int n, SweepArgsT<T> *args, int *counters)
 {
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (s < 18) counters[s] = 0;
+    if (s < 36) counters[s] = 0;   // the sweeps' counter pairs (the kernel also resets them)
     if (s >= n) return;
     ScenCtl &c = ctl[s];
     c.accepted = 0;
@@ -233,7 +233,7 @@ static int launch_dir(magic_batch *b, int dir, hipStream_t st = nullptr, int k0 
         const SweepArgsT<T> *args = (const SweepArgsT<T> *)b->args_d;
         const int4 *items = b->items_d + lo;
         const int *const *run = b->run_d;
-        int *cnt = b->counters_d + 6 * dir + k;
+        int *cnt = b->counters_d + 2 * (6 * dir + k);
         void *kargs[] = {&args, &items, &n, &run, &cnt};
         HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, st));
         TRY(launch_check());
@@ -767,7 +767,7 @@ extern "C" int magic_batch_create(int nscen, const int *dims, int meqn, int mbc,
                            hipEventCreateWithFlags(&b->sw_join, hipEventDisableTiming) != hipSuccess))
         st = MAGIC_EHIP;
     if (st == MAGIC_OK) st = reserve(b, nscen > 0 ? nscen : 1);
-    if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 18 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
+    if (st == MAGIC_OK) st = hipMalloc(&b->counters_d, 36 * sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
     if (st == MAGIC_OK) st = hipMalloc(&b->one_d, sizeof(int)) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
     if (st == MAGIC_OK) {
         const int one = 1;
@@ -957,7 +957,7 @@ static int tune_rounds(magic_batch *b, int reps, TuneClock &clk)
             TRY(build_lists<T>(b, true));
             for (int dir = 0; dir < 3; dir++) {
                 TRY(clk.mark(dir, sh, 0, b->stream));
-                HIPCHECK(hipMemsetAsync(b->counters_d, 0, 18 * sizeof(int), b->stream));
+                HIPCHECK(hipMemsetAsync(b->counters_d, 0, 36 * sizeof(int), b->stream));
                 TRY((launch_dir<T, PERT>(b, dir)));
                 TRY(clk.mark(dir, sh, 1, b->stream));
             }

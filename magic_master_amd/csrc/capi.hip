@@ -123,6 +123,9 @@ extern "C" int magic_destroy(magic_ctx *c)
     hipFree(c->u); hipFree(c->qt); hipFree(c->ut); hipFree(c->dscal); hipFree(c->sink);
     hipFree(c->q32); hipFree(c->qold32); hipFree(c->u32); hipFree(c->qt32); hipFree(c->ut32);
     hipFree(c->col32); hipFree(c->bg64); hipFree(c->col64); hipFree(c->sums_d);
+    hipFree(c->largs); hipFree(c->lrun); hipFree(c->lone); hipFree(c->lcnt);
+    for (auto &d : c->litems)
+        for (int4 *p : d) hipFree(p);
     if (c->side) hipStreamSynchronize(c->side);
     hipFree(c->dq[0]);
     hipFree(c->dq[1]);
@@ -487,26 +490,100 @@ static int launch_sweep(magic_ctx *c, SweepArgs a, int *nl)
     return MAGIC_OK;
 }
 
-// Colour launches k0 .. k1-1 (k = 3 ca + cb) of one direction.
+// Argument blocks of colours k0 .. k1-1 of direction dir (slot 6 dir + k),
+// written on the stream of their launches, so that a later call may change
+// them while these launches are still queued
+template <typename T>
+__global__ void k_put_args(SweepArgsT<T> *slots, SweepArgsT<T> a, int dir, int k0, int k1)
+{
+    const int k = k0 + (int)threadIdx.x;
+    if (k >= k1) return;
+    SweepArgsT<T> s = a;
+    s.cA = k / 3;
+    s.cB = k % 3;
+    slots[dir * 6 + k] = s;
+}
+
+// The work-list buffers of the context (first tiled sweep)
+template <typename T>
+static int list_setup(magic_ctx *c)
+{
+    if (c->largs) return MAGIC_OK;
+    if (hipMalloc(&c->largs, 18 * sizeof(SweepArgsT<T>)) != hipSuccess ||
+        hipMalloc(&c->lrun, 18 * sizeof(int *)) != hipSuccess || hipMalloc(&c->lone, sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->lcnt, 36 * sizeof(int)) != hipSuccess) {
+        hipFree(c->largs); hipFree(c->lrun); hipFree(c->lone); hipFree(c->lcnt);
+        c->largs = nullptr; c->lrun = nullptr; c->lone = nullptr; c->lcnt = nullptr;
+        return MAGIC_ENOMEM;
+    }
+    const int one = 1;
+    const int *run[18];
+    for (auto &r : run) r = c->lone;
+    const int zero[36] = {};
+    HIPCHECK(hipMemcpy(c->lone, &one, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->lrun, run, sizeof(run), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->lcnt, zero, sizeof(zero), hipMemcpyHostToDevice));
+    return MAGIC_OK;
+}
+
+// The tiles of direction dir in shape sh, colour by colour (built once)
+static int list_items(magic_ctx *c, int dir, int sh, int nA, int nB, int a_is_g)
+{
+    if (c->litems[dir][sh]) return MAGIC_OK;
+    const int TA = tile_ta(sh);
+    const int ntiles = (nA + 2 + TA - 1) / TA;
+    std::vector<int4> items;
+    int *off = c->litem_off[dir][sh];
+    for (int ca = 0; ca < 2; ca++)
+        for (int cb = 0; cb < 3; cb++) {
+            const int k = 3 * ca + cb;
+            off[k] = (int)items.size();
+            const int gx = (ntiles - ca + 1) / 2, gy = (nB + 1 - cb) / 3 + 1;
+            for (int by = 0; by < gy; by++)
+                for (int bx = 0; bx < gx; bx++) items.push_back(make_int4(dir * 6 + k, a_is_g, bx, by));
+        }
+    off[6] = (int)items.size();
+    int4 *d = nullptr;
+    HIPCHECK(hipMalloc(&d, (items.size() ? items.size() : 1) * sizeof(int4)));
+    if (hipMemcpy(d, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return MAGIC_EHIP;
+    }
+    c->litems[dir][sh] = d;
+    return MAGIC_OK;
+}
+
+// Colour launches k0 .. k1-1 (k = 3 ca + cb) of one direction: the
+// persistent work-list kernel over this grid's tiles of each colour.
 template <typename T, bool PERT, int NTR>
 static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s, int k0, int k1)
 {
-    const void *fn = tile_fn<T, PERT, NTR>(sh);
+    const void *fn = tile_list_fn<T, PERT, NTR>(sh);
     if (!fn) return MAGIC_EARG;
-    const int TA = tile_ta(sh);
-    const int ntiles = (a.nA + 2 + TA - 1) / TA;
-    for (int ca = 0; ca < 2; ca++)
-        for (int cb = 0; cb < 3; cb++) {
-            if (3 * ca + cb < k0 || 3 * ca + cb >= k1) continue;
-            a.cA = ca;
-            a.cB = cb;
-            const unsigned gx = (unsigned)((ntiles - ca + 1) / 2);
-            const unsigned gy = (unsigned)((a.nB + 1 - cb) / 3 + 1);
-            if (gx == 0) continue;
-            void *kargs[] = {&a, &a_is_g};
-            HIPCHECK(hipLaunchKernel(fn, dim3(gx, gy), dim3(256), kargs, 0, s));
-            ++*nl;
-        }
+    const int dir = a.sweep - 1;
+    TRY(list_setup<T>(c));
+    TRY(list_items(c, dir, sh, a.nA, a.nB, a_is_g));
+    if (!c->lgrid[sh]) {
+        int dev = 0, ncu = 0, occ = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
+        c->lgrid[sh] = ncu * (occ > 0 ? occ : 1);
+    }
+    k_put_args<T><<<1, 64, 0, s>>>((SweepArgsT<T> *)c->largs, a, dir, k0, k1);
+    TRY(launch_check());
+    const int *off = c->litem_off[dir][sh];
+    for (int k = k0; k < k1; k++) {
+        int n = off[k + 1] - off[k];
+        if (n <= 0) continue;
+        const SweepArgsT<T> *args = (const SweepArgsT<T> *)c->largs;
+        const int4 *items = c->litems[dir][sh] + off[k];
+        const int *const *run = c->lrun;
+        int *cnt = c->lcnt + 2 * (dir * 6 + k);
+        void *kargs[] = {&args, &items, &n, &run, &cnt};
+        HIPCHECK(hipLaunchKernel(fn, dim3(std::min(c->lgrid[sh], n)), dim3(256), kargs, 0, s));
+        ++*nl;
+    }
     return MAGIC_OK;
 }
 

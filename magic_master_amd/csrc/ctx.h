@@ -78,6 +78,18 @@ struct magic_ctx {
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps
+    // The sweeps' work lists (set up on the first tiled sweep): 18 argument
+    // blocks (direction x colour, written by a kernel before each
+    // direction's launches), run flags all pointing at a device 1, 18 pairs
+    // of work counters, and per direction and shape the tiles of each
+    // colour (items of colour k at [litem_off[dir][sh][k], ..[k + 1]))
+    void *largs = nullptr;
+    const int **lrun = nullptr;
+    int *lone = nullptr;
+    int *lcnt = nullptr;
+    int4 *litems[3][TILE_NSHAPE] = {};
+    int litem_off[3][TILE_NSHAPE][7] = {};
+    int lgrid[TILE_NSHAPE] = {};           // persistent grid of each shape's kernel (0: not yet known)
     unsigned long long *hscal = nullptr;   // pinned host words: [0, 8) mirror dscal, [HSCAL_SUMS, +10) global sums
     void *sums_d = nullptr;                // global sums: partials and results (first magic_global_sums)
     magic_params prm;
@@ -175,20 +187,8 @@ static inline dim3 diff_grid(const Dims &d) { return dim3(blocks(d.mx, 64), bloc
 static inline bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
 
 // ------------------------------------------------------- tile shapes ----
-// The tiled sweep kernel of shape sh (0 .. TILE_NSHAPE-1); one grid and the
-// persistent work-list form.
-template <typename T, bool PERT, int NTR>
-static inline const void *tile_fn(int sh)
-{
-    switch (sh) {
-    case 0: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 0>);
-    case 1: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 1>);
-    case 2: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 2>);
-    case 3: return reinterpret_cast<const void *>(&tile_sweep<T, PERT, NTR, 3>);
-    default: return nullptr;
-    }
-}
-
+// The tiled sweep kernel of shape sh (0 .. TILE_NSHAPE-1): the persistent
+// work-list kernel that one grid and a batch of grids both launch.
 template <typename T, bool PERT, int NTR>
 static inline const void *tile_list_fn(int sh)
 {
@@ -200,7 +200,7 @@ static inline const void *tile_list_fn(int sh)
     default: return nullptr;
     }
 }
-static_assert(TILE_NSHAPE == 4, "tile_fn and tile_list_fn list every shape");
+static_assert(TILE_NSHAPE == 4, "tile_list_fn lists every shape");
 
 // Event pairs of the autotuners, one per (direction, shape).  The shapes
 // are timed round-robin (every round times each candidate once), so that a

@@ -45,19 +45,18 @@ using SweepArgs = SweepArgsT<double>;
 template <int NTR>
 __global__ void sweep_kernel(SweepArgs a);
 
-// Tiled variant, one instantiation per tile shape SH (0 .. TILE_NSHAPE-1):
+// Tiled sweep, one instantiation per tile shape SH (0 .. TILE_NSHAPE-1):
 // TA = tile_ta(SH) pencils along the tile axis a per workgroup, 256/TA
 // interfaces per chunk along the pencil.  a_is_g says whether the tile axis
 // is the sweep's g direction; sA/sB/nA/nB then describe the tile axis a and
 // the coloured axis b.  (T, PERT) = (double, false): FP64 form; (float,
-// true): FP32 perturbation form.
+// true): FP32 perturbation form.  A persistent kernel over a work list of
+// tiles of one or several grids (items: {argument block, a_is_g, bx, by};
+// run[argument block] points at the grid's "runs in this attempt" flag;
+// next: the work counter and the count of finished workgroups, both 0 at
+// launch and set back to 0 by the kernel).
 constexpr int TILE_NSHAPE = 4;
 __host__ __device__ constexpr int tile_ta(int sh) { return sh == 0 ? 16 : sh == 3 ? 4 : 8; }
-template <typename T, bool PERT, int NTR, int SH>
-__global__ void tile_sweep(SweepArgsT<T> a, int a_is_g);
-// Persistent variant over a work list of tiles of several grids (items:
-// {argument block, a_is_g, bx, by}; run[argument block] points at the
-// grid's "runs in this attempt" flag; next is the work counter, 0 at launch).
 template <typename T, bool PERT, int NTR, int SH>
 __global__ void tile_sweep_list(const SweepArgsT<T> *args, const int4 *items, int nitems, const int *const *run,
                                 int *next);

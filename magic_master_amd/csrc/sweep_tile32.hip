@@ -12,7 +12,6 @@
 namespace magic {
 
 #define MAGIC_TILE32(NTR)                                                                                  \
-    template __global__ void tile_sweep<float, true, NTR, MAGIC_TILE_SHAPE>(SweepArgsT<float>, int);      \
     template __global__ void tile_sweep_list<float, true, NTR, MAGIC_TILE_SHAPE>(                        \
         const SweepArgsT<float> *, const int4 *, int, const int *const *, int *);
 MAGIC_TILE32(0)

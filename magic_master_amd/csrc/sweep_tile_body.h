@@ -763,26 +763,22 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
 // these kernels miss 256 registers by a few, since the Riemann solve runs in
 // FP64, and halve the occupancy: 50 % slower on the full-domain FP32 step),
 // one otherwise.
-template <typename T, int NTR, int SH, bool LIST>
+template <typename T, int NTR, int SH>
 constexpr int tile_waves()
 {
     if (sizeof(T) != 4 || TileShape<SH>::PM) return 1;
     return NTR <= 1 ? 2 : 1;
 }
 
+// The sweep kernel, for one grid and for a batch of grids alike (one
+// compiled body, so both round alike): a persistent grid whose workgroups
+// take tiles from a work list (item = {argument block, a_is_g, bx, by})
+// through the atomic counter next[0] until it runs out; tiles of grids whose
+// control flag (*run) is 0 in this attempt are skipped.  next[1] counts the
+// workgroups that are done; the last one sets both back to 0 for the next
+// launch (every other workgroup has taken its last item by then).
 template <typename T, bool PERT, int NTR, int SH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH, false>())))
-void tile_sweep(SweepArgsT<T> A, int a_is_g)
-{
-    tile_sweep_body<T, PERT, NTR, SH>(A, a_is_g, (int)blockIdx.x, (int)blockIdx.y);
-}
-
-// Persistent form for a batch of grids: the workgroups take tiles from a
-// work list (item = {argument block, a_is_g, bx, by}) through an atomic
-// counter until it runs out; tiles of grids whose control flag (*run) is 0
-// in this attempt are skipped.
-template <typename T, bool PERT, int NTR, int SH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH, true>())))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH>())))
 void tile_sweep_list(const SweepArgsT<T> *__restrict__ args, const int4 *items, int nitems,
                                                         const int *const *run, int *next)
 {
@@ -800,6 +796,14 @@ void tile_sweep_list(const SweepArgsT<T> *__restrict__ args, const int4 *items, 
         if (*run[ai] == 0) continue;
         tile_sweep_body<T, PERT, NTR, SH>(args[ai], __builtin_amdgcn_readfirstlane(w.y),
                                           __builtin_amdgcn_readfirstlane(w.z), __builtin_amdgcn_readfirstlane(w.w));
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();   // this workgroup's last take precedes its count
+        if (atomicAdd(next + 1, 1) == (int)gridDim.x - 1) {
+            __threadfence();
+            atomicExch(next, 0);
+            atomicExch(next + 1, 0);
+        }
     }
 }
 

@@ -24,6 +24,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libmagic_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmagic_ref.so")
+# the same sources built for timing (-O3, FMA, x86-64-v4; oracle/ref/Makefile `speed`)
+REF_SPEED_SO = os.path.join(HERE, "_ref", "libmagic_ref_speed.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -173,13 +175,14 @@ class Reference:
     _singleton = None
 
     @classmethod
-    def available(cls) -> bool:
-        return os.path.exists(REF_SO)
+    def available(cls, speed: bool = False) -> bool:
+        return os.path.exists(REF_SPEED_SO if speed else REF_SO)
 
     @classmethod
-    def get(cls):
+    def get(cls, speed: bool = False):
+        """The parity build, or (speed) the timing build; one per process."""
         if cls._singleton is None:
-            cls._singleton = cls()
+            cls._singleton = cls(REF_SPEED_SO if speed else REF_SO)
         return cls._singleton
 
     def __init__(self, path: str = REF_SO):

@@ -360,11 +360,11 @@ static void sweep_form_args(const magic_ctx *c, SweepArgsT<float> &a)
     const int NZ = c->d.NZ;
     a.qold = c->qold32;
     a.qnew = c->q32;
-    a.crho = c->col32 + C32_RHO * NZ;
-    a.cE = c->col32 + C32_E * NZ;
-    a.cgam = c->col32 + C32_GAM * NZ;
-    a.cvg = c->col32 + C32_VG * NZ;
-    a.cres = c->col32 + C32_RES * NZ;
+    a.crho = c->bg64 + B64_RHO * NZ;
+    a.cE = c->bg64 + B64_E * NZ;
+    a.cgam = c->bg64 + B64_GAM * NZ;
+    a.cvg = c->bg64 + B64_VG * NZ;
+    a.cres = c->bg64 + B64_RES * NZ;
     a.sink = (float *)c->sink;
 }
 
@@ -472,8 +472,7 @@ static void form_ops(const magic_ctx *c, BatchOps<float> &o)
 {
     const int NZ = c->d.NZ;
     o.q = c->q32; o.qold = c->qold32; o.u = c->u32; o.qt = c->qt32; o.ut = c->ut32;
-    o.col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ, c->col32 + C32_R * NZ,
-             c->bg64, c->bg64 + NZ};
+    o.col = col32_of(c);
     o.zr = c->zr32;
     o.nu32 = c->col32 + C32_NU * NZ;
     o.kap32 = c->col32 + C32_KAP * NZ;

@@ -91,12 +91,11 @@ __global__ void k_aos_to_planes(double *planes, const double *aos, long long P, 
 // Background columns at rest, indexed by k + mbc - 1: rho_bar, E_bar,
 // gamma (aux19), R (aux18), and rho_bar, E_bar in FP64.
 struct Col32 {
-    const float *rho, *E, *gam, *R;
-    const double *rho64, *E64;
+    const double *rho, *E, *gam, *R;   // FP64 columns of rho_bar, E_bar, gamma, R
 };
 // bc3 z-face factors (rho_bar_ghost/rho_bar_boundary)**0.5 per side and layer
 struct Zr32 {
-    float s[2][8];
+    double s[2][8];
 };
 template <class G>
 __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate);

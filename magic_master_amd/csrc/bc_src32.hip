@@ -37,7 +37,7 @@ template __global__ void k_conduct_zspill<float, NoGate>(Dims, float *, int, NoG
 // the vertical momentum; with a background at rest every one of them is
 // q'_ghost = q'_boundary * sc.  sc depends on the layer only.
 // zf(side, kbc - 1): the factor sc of ghost layer kbc on side 0 (bottom) or 1
-// (top), Zr32's s[side][kbc - 1]
+// (top), Zr32's s[side][kbc - 1] (FP64)
 template <class G, class ZF>
 __device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int nc, int lo, int hi, ZF zf,
                                          const G &gate)
@@ -54,8 +54,8 @@ __device__ __forceinline__ void b32_bc_z(const uint3 BI, Dims d, float *q, int n
             for (int kbc = 1; kbc <= d.mbc; kbc++) {
                 const int kg = side == 0 ? 1 - kbc : d.mz + kbc;
                 const long long cg = cid(d, i, j, kg);
-                const float sc = zf(side, kbc - 1);
-                for (int m = 0; m < 5; m++) q[m * d.P + cg] = q[m * d.P + cb] * sc;
+                const double sc = zf(side, kbc - 1);   // FP64 factor, one rounding per ghost value
+                for (int m = 0; m < 5; m++) q[m * d.P + cg] = (float)((double)q[m * d.P + cb] * sc);
                 for (int m = 5; m < nc; m++) q[m * d.P + cg] = q[m * d.P + cb];
             }
             if (mth == 3)
@@ -96,7 +96,7 @@ __device__ __forceinline__ void b32_src_convert(const uint3 BI, Dims d, float *q
     if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
-    const double rb = col.rho64[kk], Eb = col.E64[kk], g1 = (double)col.gam[kk] - 1.0, R = col.R[kk];
+    const double rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0, R = col.R[kk];
     const double rp = q[c], rho = rb + rp;
     const double m2 = q[P + c], m3 = q[2 * P + c], m4 = q[3 * P + c], ep = q[4 * P + c];
     const double irho = 1.0 / rho;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void b32_src_finish(const uint3 BI, Dims d, float *q,
     INTERIOR_IJK
     const long long P = d.P;
     const int kk = k + d.mbc - 1;
-    const double rb = col.rho64[kk], Eb = col.E64[kk], g1 = (double)col.gam[kk] - 1.0, R = col.R[kk];
+    const double rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0, R = col.R[kk];
     const double rp = q[c], rho = rb + rp;
     const double m2 = q[P + c] * rho, m3 = q[2 * P + c] * rho;
     double m4 = q[3 * P + c] * rho;

@@ -71,14 +71,14 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
              hipMalloc(&c->qt32, P * sizeof(float)) == hipSuccess &&
              hipMalloc(&c->ut32, P * sizeof(float)) == hipSuccess &&
              hipMalloc(&c->col32, NC32 * NZ * sizeof(float)) == hipSuccess &&
-             hipMalloc(&c->bg64, 2 * NZ * sizeof(double)) == hipSuccess;
+             hipMalloc(&c->bg64, NB64 * NZ * sizeof(double)) == hipSuccess;
         ok = ok && hipMemsetAsync(c->q32, 0, P * meqn * sizeof(float), c->stream) == hipSuccess &&
              hipMemsetAsync(c->qold32, 0, P * meqn * sizeof(float), c->stream) == hipSuccess &&
              hipMemsetAsync(c->u32, 0, 3 * P * sizeof(float), c->stream) == hipSuccess &&
              hipMemsetAsync(c->qt32, 0, P * sizeof(float), c->stream) == hipSuccess &&
              hipMemsetAsync(c->ut32, 0, P * sizeof(float), c->stream) == hipSuccess &&
              hipMemsetAsync(c->col32, 0, NC32 * NZ * sizeof(float), c->stream) == hipSuccess &&
-             hipMemsetAsync(c->bg64, 0, 2 * NZ * sizeof(double), c->stream) == hipSuccess &&
+             hipMemsetAsync(c->bg64, 0, NB64 * NZ * sizeof(double), c->stream) == hipSuccess &&
              hipStreamSynchronize(c->stream) == hipSuccess;
         if (!ok) {
             magic_destroy(c);
@@ -146,20 +146,16 @@ extern "C" int magic_set_params(magic_ctx *c, const magic_params *p)
     return MAGIC_OK;
 }
 
-// FP32 form: the background must be one column at rest.  Keeps float
-// columns of rho_bar, E_bar, gamma, R, aux17, nu, kappa, the background's own
-// z jump of the well-balanced f-wave (p_bar(k) - p_bar(k-1) - .5 (rho_bar(k)
-// + rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
-// evaluates it for a state at rest; zero to FP64 rounding), FP64 rho_bar and
-// E_bar for the conversions, and the bc3 z factors per layer.
+// aux slots (1-based) of the ten background planes, in AX_ order
 static const int AUX_SLOT[NAUXP] = {1, 2, 3, 4, 5, 11, 12, 17, 18, 19};
 
-// FP32 form: the background must be one column at rest.  Keeps float
-// columns of rho_bar, E_bar, gamma, R, aux17, nu, kappa, the background's own
-// z jump of the well-balanced f-wave (p_bar(k) - p_bar(k-1) - .5 (rho_bar(k)
-// + rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
-// evaluates it for a state at rest; zero to FP64 rounding), FP64 rho_bar and
-// E_bar for the conversions, and the bc3 z factors per layer.  get(slot
+// FP32 form: the background must be one column at rest.  Keeps FP64
+// columns of rho_bar, E_bar, gamma, R, aux17 and the background's own z jump
+// of the well-balanced f-wave (p_bar(k) - p_bar(k-1) - .5 (rho_bar(k) +
+// rho_bar(k-1)) aux17(k), p_bar = (gamma-1) E_bar as rpn3euX.f:166-169
+// evaluates it for a state at rest; zero to FP64 rounding), which the
+// form's FP64 Riemann solve and conversions read; float columns of nu and
+// kappa for the diffusion passes; and the bc3 z factors per layer.  get(slot
 // index, cell) returns the background value of a padded cell.
 template <class F>
 static int set_aux32(magic_ctx *c, F get, bool check_columns)
@@ -187,27 +183,25 @@ static int set_aux32(magic_ctx *c, F get, bool check_columns)
     }
     auto cv = [&](int s, int kk) { return col[(size_t)s * NZ + kk]; };
     std::vector<float> h((size_t)NC32 * NZ, 0.0f);
-    std::vector<double> b64((size_t)2 * NZ);
+    std::vector<double> b64((size_t)NB64 * NZ, 0.0);
     for (int kk = 0; kk < NZ; kk++) {
-        h[C32_RHO * NZ + kk] = (float)cv(AX_RHO, kk);
-        h[C32_E * NZ + kk] = (float)cv(AX_E, kk);
-        h[C32_GAM * NZ + kk] = (float)cv(AX_GAM, kk);
-        h[C32_R * NZ + kk] = (float)cv(AX_R, kk);
-        h[C32_VG * NZ + kk] = (float)cv(AX_VG, kk);
         h[C32_NU * NZ + kk] = (float)cv(AX_NU, kk);
         h[C32_KAP * NZ + kk] = (float)cv(AX_KAP, kk);
+        b64[B64_RHO * NZ + kk] = cv(AX_RHO, kk);
+        b64[B64_E * NZ + kk] = cv(AX_E, kk);
+        b64[B64_GAM * NZ + kk] = cv(AX_GAM, kk);
+        b64[B64_R * NZ + kk] = cv(AX_R, kk);
+        b64[B64_VG * NZ + kk] = cv(AX_VG, kk);
         if (kk > 0) {
             const double pb = (cv(AX_GAM, kk) - 1.0) * (cv(AX_E, kk) - 0.5 * 0.0 / cv(AX_RHO, kk));
             const double pa = (cv(AX_GAM, kk - 1) - 1.0) * (cv(AX_E, kk - 1) - 0.5 * 0.0 / cv(AX_RHO, kk - 1));
-            h[C32_RES * NZ + kk] = (float)((pb - pa) - 0.5 * (cv(AX_RHO, kk) + cv(AX_RHO, kk - 1)) * cv(AX_VG, kk));
+            b64[B64_RES * NZ + kk] = (pb - pa) - 0.5 * (cv(AX_RHO, kk) + cv(AX_RHO, kk - 1)) * cv(AX_VG, kk);
         }
-        b64[kk] = cv(AX_RHO, kk);
-        b64[NZ + kk] = cv(AX_E, kk);
     }
     for (int side = 0; side < 2; side++)
         for (int kbc = 1; kbc <= mbc; kbc++) {
             const int kg = side == 0 ? 1 - kbc : mz + kbc, kb = side == 0 ? 1 : mz;
-            c->zr32.s[side][kbc - 1] = (float)std::pow(cv(AX_RHO, kg + mbc - 1) / cv(AX_RHO, kb + mbc - 1), 0.5);
+            c->zr32.s[side][kbc - 1] = std::pow(cv(AX_RHO, kg + mbc - 1) / cv(AX_RHO, kb + mbc - 1), 0.5);
         }
     HIPCHECK(hipMemcpyAsync(c->col32, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->bg64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -307,8 +301,8 @@ extern "C" int magic_set_q(magic_ctx *c, const double *q)
         if (!c->have_aux) return MAGIC_EARG;   // the perturbation needs the background
         const int NZ = c->d.NZ;
         HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        k32_aos_to_pert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn, c->bg64,
-                                                               c->bg64 + NZ);
+        k32_aos_to_pert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn,
+                                                               c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ);
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(c->qold32, c->q32, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -346,8 +340,8 @@ extern "C" int magic_get_q(magic_ctx *c, double *q)
     const size_t n = (size_t)c->P * c->meqn;
     if (c->form == MAGIC_FORM_FP32P) {
         const int NZ = c->d.NZ;
-        k32_pert_to_aos<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn, c->bg64,
-                                                               c->bg64 + NZ);
+        k32_pert_to_aos<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn,
+                                                               c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ);
         TRY(launch_check());
         HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -841,11 +835,11 @@ static SweepArgsT<float> args32(magic_ctx *c)
     a.status = (int *)(c->dscal + 1);
     for (int m = 5; m < c->meqn; m++) a.comp[m] = m;
     const int NZ = c->d.NZ;
-    a.crho = c->col32 + C32_RHO * NZ;
-    a.cE = c->col32 + C32_E * NZ;
-    a.cgam = c->col32 + C32_GAM * NZ;
-    a.cvg = c->col32 + C32_VG * NZ;
-    a.cres = c->col32 + C32_RES * NZ;
+    a.crho = c->bg64 + B64_RHO * NZ;
+    a.cE = c->bg64 + B64_E * NZ;
+    a.cgam = c->bg64 + B64_GAM * NZ;
+    a.cvg = c->bg64 + B64_VG * NZ;
+    a.cres = c->bg64 + B64_RES * NZ;
     a.sink = (float *)c->sink;
     return a;
 }
@@ -1038,8 +1032,7 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const dim3 gint = interior_grid(d);
     const dim3 gface(blocks(d.NX, 64), d.NY);
     const float *nu = c->col32 + C32_NU * NZ, *kap = c->col32 + C32_KAP * NZ;
-    const Col32 col = {c->col32 + C32_RHO * NZ, c->col32 + C32_E * NZ, c->col32 + C32_GAM * NZ,
-                       c->col32 + C32_R * NZ, c->bg64, c->bg64 + NZ};
+    const Col32 col = col32_of(c);
     {
         Span sp(c, TC_SRC, 1);
         k32_src_convert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(d, c->q32, col, NoGate{});

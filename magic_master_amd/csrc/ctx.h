@@ -54,7 +54,7 @@ struct magic_ctx {
     double *q = nullptr, *qold = nullptr, *aux = nullptr, *zratio = nullptr;
     double *stage = nullptr, *u = nullptr, *qt = nullptr, *ut = nullptr;
     // FP32 perturbation form: state planes and scratch, background columns
-    // (float: rho, E, gamma, R, aux17, z residual, nu, kappa; FP64: rho, E)
+    // (float: nu, kappa; FP64: rho, E, gamma, R, aux17, z residual)
     float *q32 = nullptr, *qold32 = nullptr, *u32 = nullptr, *qt32 = nullptr, *ut32 = nullptr;
     float *col32 = nullptr;
     double *bg64 = nullptr;
@@ -259,4 +259,12 @@ static inline bool shapes_valid(const int *sh)
     return true;
 }
 
-enum { C32_RHO = 0, C32_E, C32_GAM, C32_R, C32_VG, C32_RES, C32_NU, C32_KAP, NC32 };
+// FP32 form's background columns (index kk = k + mbc - 1): float columns of
+// the diffusivities, FP64 columns of everything its arithmetic reads
+enum { C32_NU = 0, C32_KAP, NC32 };
+enum { B64_RHO = 0, B64_E, B64_GAM, B64_R, B64_VG, B64_RES, NB64 };
+static inline Col32 col32_of(const magic_ctx *c)
+{
+    const int NZ = c->d.NZ;
+    return Col32{c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ, c->bg64 + B64_GAM * NZ, c->bg64 + B64_R * NZ};
+}

@@ -37,7 +37,7 @@ struct SweepArgsT {
     // perturbation form only: background columns indexed by k + mbc - 1
     // (rho_bar, E_bar, gamma, aux17, and the background's own z jump
     // p_bar(k) - p_bar(k-1) - .5 (rho_bar(k) + rho_bar(k-1)) aux17(k))
-    const float *crho, *cE, *cgam, *cvg, *cres;
+    const double *crho, *cE, *cgam, *cvg, *cres;
     T *sink;                           // >= 256 elements of scratch the tile flush may write
 };
 using SweepArgs = SweepArgsT<double>;

@@ -125,8 +125,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
     const gptr<const T> qold = to_global(A.qold);
     const gptr<T> qnew = to_global(A.qnew);
     const gptr<const double> gam = to_global(A.gam), vgrav = to_global(A.vgrav);
-    const gptr<const float> crho = to_global(A.crho), cE = to_global(A.cE), cgam = to_global(A.cgam),
-                            cvg = to_global(A.cvg), cres = to_global(A.cres);
+    const gptr<const double> crho = to_global(A.crho), cE = to_global(A.cE), cgam = to_global(A.cgam),
+                             cvg = to_global(A.cvg), cres = to_global(A.cres);
     const T k6g = (T(1.0) / T(6.0)) * d * ch;
     const T k6h = (T(1.0) / T(6.0)) * d * cg;
 

@@ -94,6 +94,21 @@ int magic_set_q(magic_ctx *ctx, const double *q);
  * rest on the background), tracers 0.  Needs the aux first. */
 int magic_qinit_rest(magic_ctx *ctx);
 int magic_get_q(magic_ctx *ctx, double *q);
+/* Snapshot of q for output while the context's work goes on (the
+ * reference's out3 frames, out3_mpi_h5sep.f90:66-195, written by the
+ * caller): q in the layout of magic_get_q is converted on the context's
+ * stream and copied to q (host memory; page-locked memory from
+ * magic_host_alloc, else the copy waits for the device) on a copy stream
+ * of the context, so that work queued on the context afterwards (more
+ * steps, a batch's attempts) runs meanwhile.  *done receives a handle that
+ * magic_snapshot_wait waits on and releases; q holds the snapshot once it
+ * returns.  The handle stays valid after the context is destroyed (which
+ * first waits for the copy).  No counterpart in the reference. */
+int magic_snapshot_begin(magic_ctx *ctx, double *q, void **done);
+int magic_snapshot_wait(void *done);
+/* Page-locked host memory for snapshots. */
+int magic_host_alloc(size_t bytes, void **p);
+int magic_host_free(void *p);
 
 /* bc3 (bc3_mpi.f:5-812): fill the ghost layers of q on the device.
  * mthbc(k) = 1 extrapolation, 2 periodic, 3 wall, < 0 skip the face. */

@@ -116,6 +116,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
 extern "C" int magic_destroy(magic_ctx *c)
 {
     if (!c) return MAGIC_OK;
+    if (c->cstream) hipStreamSynchronize(c->cstream);   // a snapshot's copy out of stage
     if (c->stream) hipStreamSynchronize(c->stream);
     drain_timing(c);
     for (auto e : c->pool) hipEventDestroy(e);
@@ -133,6 +134,9 @@ extern "C" int magic_destroy(magic_ctx *c)
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->side) hipStreamDestroy(c->side);
     if (c->hscal) hipHostFree(c->hscal);
+    if (c->snap_conv) hipEventDestroy(c->snap_conv);
+    if (c->snap_done) hipEventDestroy(c->snap_done);
+    if (c->cstream) hipStreamDestroy(c->cstream);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return MAGIC_OK;
@@ -293,10 +297,33 @@ extern "C" int magic_set_aux_column(magic_ctx *c, const double *col)
     return set_aux64(c, get);
 }
 
+// stage may be read by a snapshot's copy still in flight: the stream waits
+// for it before anything writes stage
+static int stage_ready(magic_ctx *c)
+{
+    if (c->snap_pending) HIPCHECK(hipStreamWaitEvent(c->stream, c->snap_done, 0));
+    return MAGIC_OK;
+}
+
+// q in the reference layout q(meqn, i, j, k), FP64, into stage (queued)
+static int stage_q(magic_ctx *c)
+{
+    TRY(stage_ready(c));
+    if (c->form == MAGIC_FORM_FP32P) {
+        const int NZ = c->d.NZ;
+        k32_pert_to_aos<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn,
+                                                               c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ);
+    } else {
+        k_planes_to_aos<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->stage, c->q, c->P, c->meqn);
+    }
+    return launch_check();
+}
+
 extern "C" int magic_set_q(magic_ctx *c, const double *q)
 {
     if (!c || !q) return MAGIC_EARG;
     const size_t n = (size_t)c->P * c->meqn;
+    TRY(stage_ready(c));
     if (c->form == MAGIC_FORM_FP32P) {
         if (!c->have_aux) return MAGIC_EARG;   // the perturbation needs the background
         const int NZ = c->d.NZ;
@@ -338,19 +365,57 @@ extern "C" int magic_get_q(magic_ctx *c, double *q)
 {
     if (!c || !q) return MAGIC_EARG;
     const size_t n = (size_t)c->P * c->meqn;
-    if (c->form == MAGIC_FORM_FP32P) {
-        const int NZ = c->d.NZ;
-        k32_pert_to_aos<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->stage, c->q32, c->meqn,
-                                                               c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ);
-        TRY(launch_check());
-        HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
-        return MAGIC_OK;
-    }
-    k_planes_to_aos<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->stage, c->q, c->P, c->meqn);
-    TRY(launch_check());
+    TRY(stage_q(c));
     HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    return MAGIC_OK;
+}
+
+extern "C" int magic_snapshot_begin(magic_ctx *c, double *q, void **done)
+{
+    if (!c || !q || !done) return MAGIC_EARG;
+    *done = nullptr;
+    if (!c->cstream) {
+        HIPCHECK(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&c->snap_conv, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+    }
+    const size_t n = (size_t)c->P * c->meqn;
+    TRY(stage_q(c));
+    HIPCHECK(hipEventRecord(c->snap_conv, c->stream));
+    HIPCHECK(hipStreamWaitEvent(c->cstream, c->snap_conv, 0));
+    HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->cstream));
+    HIPCHECK(hipEventRecord(c->snap_done, c->cstream));
+    c->snap_pending = true;
+    hipEvent_t ev = nullptr;
+    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (hipEventRecord(ev, c->cstream) != hipSuccess) {
+        hipEventDestroy(ev);
+        return MAGIC_EHIP;
+    }
+    *done = ev;
+    return MAGIC_OK;
+}
+
+extern "C" int magic_snapshot_wait(void *done)
+{
+    if (!done) return MAGIC_EARG;
+    hipEvent_t ev = (hipEvent_t)done;
+    const hipError_t e = hipEventSynchronize(ev);
+    hipEventDestroy(ev);
+    return e == hipSuccess ? MAGIC_OK : MAGIC_EHIP;
+}
+
+extern "C" int magic_host_alloc(size_t bytes, void **p)
+{
+    if (!p || bytes == 0) return MAGIC_EARG;
+    *p = nullptr;
+    return hipHostMalloc(p, bytes, hipHostMallocDefault) == hipSuccess ? MAGIC_OK : MAGIC_ENOMEM;
+}
+
+extern "C" int magic_host_free(void *p)
+{
+    if (p) HIPCHECK(hipHostFree(p));
     return MAGIC_OK;
 }
 

@@ -75,6 +75,14 @@ struct magic_ctx {
     void *dq[2] = {nullptr, nullptr};
     int dq_cur = 0;
     bool dq_clear = false;   // dq[dq_cur] holds zeros (false until set up, and after a failed overlapped step)
+    // Asynchronous snapshots (magic_snapshot_begin): the conversion into
+    // stage runs on the context's stream, the copy out of stage on cstream
+    // (created on first use), so that later work on the stream need not
+    // wait for the copy.  snap_done marks the last such copy; every other
+    // user of stage waits for it first (stage_ready).
+    hipStream_t cstream = nullptr;
+    hipEvent_t snap_conv = nullptr, snap_done = nullptr;
+    bool snap_pending = false;
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps

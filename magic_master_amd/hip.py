@@ -44,12 +44,53 @@ EXPORTS = ["magic_create", "magic_create_form", "magic_destroy", "magic_strerror
            "magic_batch_add", "magic_batch_remove", "magic_set_tile_shapes", "magic_get_tile_shapes", "magic_tune",
            "magic_batch_set_tile_shapes", "magic_batch_get_tile_shapes", "magic_batch_tune",
            "magic_set_sweep_overlap", "magic_batch_set_sweep_overlap", "magic_global_sums",
-           "magic_global_sums_noalloc"]
+           "magic_global_sums_noalloc", "magic_snapshot_begin", "magic_snapshot_wait", "magic_host_alloc",
+           "magic_host_free"]
 
 TILE_NSHAPE = 4   # MAGIC_TILE_NSHAPE
 # (pencils per tile, lane order) of each tile shape
 TILE_SHAPES = {0: (16, "tile-axis-major"), 1: (8, "tile-axis-major"), 2: (8, "pencil-major"),
                3: (4, "pencil-major")}
+
+
+class PinnedBuffer:
+    """Page-locked host memory (magic_host_alloc) of nbytes, with numpy
+    views: the destination of asynchronous snapshots."""
+
+    def __init__(self, nbytes: int):
+        self.L = load_library()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        st = self.L.magic_host_alloc(self.nbytes, C.byref(p))
+        if st != 0:
+            raise MagicError(st, "magic_host_alloc")
+        self.ptr = p
+
+    def array(self, shape) -> np.ndarray:
+        """A float64 view of the first prod(shape) elements (no copy)."""
+        n = int(np.prod(shape))
+        if n * 8 > self.nbytes:
+            raise ValueError(f"{shape} does not fit {self.nbytes} B")
+        buf = (C.c_double * n).from_address(self.ptr.value)
+        return np.ctypeslib.as_array(buf).reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.magic_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def snapshot_wait(handle):
+    """Wait for an asynchronous snapshot (snapshot_begin) and release its handle."""
+    st = load_library().magic_snapshot_wait(handle)
+    if st != 0:
+        raise MagicError(st, "magic_snapshot_wait")
 
 
 class MagicError(RuntimeError):
@@ -100,6 +141,10 @@ def load_library(path: str = LIB_PATH):
     L.magic_set_aux.argtypes = [vp, _dp]
     L.magic_set_q.argtypes = [vp, _dp]
     L.magic_get_q.argtypes = [vp, _dp]
+    L.magic_snapshot_begin.argtypes = [vp, vp, C.POINTER(vp)]
+    L.magic_snapshot_wait.argtypes = [vp]
+    L.magic_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.magic_host_free.argtypes = [vp]
     L.magic_bc3.argtypes = [vp, _ip]
     L.magic_b4step3.argtypes = [vp]
     L.magic_backup.argtypes = [vp]
@@ -209,6 +254,15 @@ class MagicHIP:
             raise ValueError(f"out must be C-contiguous float64 of shape {self.g.shape_q}")
         self._check(self.L.magic_get_q(self.h, _ptr(out)), "magic_get_q")
         return out
+
+    def snapshot_begin(self, buf: "PinnedBuffer"):
+        """Queue a snapshot of q (get_q's layout) into buf and return its
+        handle (snapshot_wait); work queued afterwards runs meanwhile."""
+        if buf.nbytes < 8 * int(np.prod(self.g.shape_q)):
+            raise ValueError("snapshot buffer too small")
+        h = C.c_void_p()
+        self._check(self.L.magic_snapshot_begin(self.h, buf.ptr, C.byref(h)), "magic_snapshot_begin")
+        return h
 
     def global_sums(self) -> np.ndarray:
         """Compensated FP64 sums of every component over the interior cells
@@ -393,6 +447,16 @@ class MagicBatch:
         q = np.empty(self.grids[s].shape_q)
         self._check(self.L.magic_get_q(self.ctx[s], _ptr(q)), "magic_get_q")
         return q
+
+    def snapshot_begin(self, s, buf: "PinnedBuffer"):
+        """Queue a snapshot of scenario s's q (get_q's layout) into buf and
+        return its handle (snapshot_wait); the batch's attempts run on
+        while it is copied."""
+        if buf.nbytes < 8 * int(np.prod(self.grids[s].shape_q)):
+            raise ValueError("snapshot buffer too small")
+        h = C.c_void_p()
+        self._check(self.L.magic_snapshot_begin(self.ctx[s], buf.ptr, C.byref(h)), "magic_snapshot_begin")
+        return h
 
     def global_sums(self, s) -> np.ndarray:
         """Compensated FP64 sums of scenario s's state (MagicHIP.global_sums)."""

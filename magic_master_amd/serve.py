@@ -75,39 +75,94 @@ def footprint_bytes(g: P.Grid) -> int:
     return cells * (8 * g.meqn + 20 + 8 * g.meqn) + 12 * (g.mz + 2 * g.mbc) * 8
 
 
-class _Writer(threading.Thread):
-    """Writes snapshots on the host while the GPU keeps stepping."""
+class _PinnedPool:
+    """Page-locked snapshot buffers (hip.PinnedBuffer), reused, at most
+    cap_bytes in all (one buffer beyond that when none is free, so that a
+    snapshot larger than the cap still runs)."""
 
-    def __init__(self, outdir: Optional[str]):
+    def __init__(self, cap_bytes: float):
+        self.cap = cap_bytes
+        self.free = []
+        self.total = 0
+        self.cv = threading.Condition()
+
+    def get(self, nbytes: int):
+        from .hip import PinnedBuffer
+        with self.cv:
+            while True:
+                for k, b in enumerate(self.free):
+                    if b.nbytes >= nbytes:
+                        return self.free.pop(k)
+                if self.total + nbytes <= self.cap or self.total == 0:
+                    self.total += nbytes
+                    return PinnedBuffer(nbytes)
+                if self.free:          # too small to reuse: make room
+                    b = self.free.pop(0)
+                    self.total -= b.nbytes
+                    b.close()
+                    continue
+                self.cv.wait()         # every buffer is with the writer
+
+    def put(self, b):
+        with self.cv:
+            self.free.append(b)
+            self.cv.notify_all()
+
+    def close(self):
+        with self.cv:
+            for b in self.free:
+                b.close()
+            self.free = []
+
+
+class _Writer(threading.Thread):
+    """Finishes snapshots on the host while the GPU keeps stepping: waits for
+    each snapshot's copy (queued by the serving loop with snapshot_begin),
+    writes it as an out3 frame (hdf5) or drops it (download), keeps a
+    request's final state when asked, and returns the buffer to the pool."""
+
+    def __init__(self, outdir: Optional[str], pool: _PinnedPool):
         super().__init__(daemon=True)
         self.outdir = outdir
+        self.pool = pool
         self.q = queue.Queue()
         self.error = None
 
     def run(self):
         from . import h5
+        from .hip import snapshot_wait
         while True:
             item = self.q.get()
             if item is None:
                 return
-            g, qh, t, frame, rid = item
+            handle, buf, g, t, frame, rid, keep = item
             try:
-                d = os.path.join(self.outdir, f"req{rid:05d}")
-                os.makedirs(d, exist_ok=True)
-                h5.out3(g, qh, t, frame, d)
+                snapshot_wait(handle)
+                qh = buf.array(g.shape_q)
+                if self.outdir:
+                    d = os.path.join(self.outdir, f"req{rid:05d}")
+                    os.makedirs(d, exist_ok=True)
+                    h5.out3(g, qh, t, frame, d)
+                if keep is not None:
+                    keep.final_q = qh.copy()
             except Exception as e:   # reported by Server.run
                 self.error = e
+            finally:
+                self.pool.put(buf)
 
 
 class Server:
     def __init__(self, budget_bytes: float = 250e9, check_every: int = 4, snapshots: str = "hdf5",
                  outdir: Optional[str] = None, keep_final: bool = False, tile_shapes=None,
-                 progress: Optional[float] = None):
+                 progress: Optional[float] = None, pinned_bytes: float = 8e9):
         """snapshots: "hdf5" (frames written under outdir), "download" (copied
-        to the host and dropped) or "none".  tile_shapes: the sweeps' tile
-        shapes (x, y, z) for the whole run (None: the library's default),
-        fixed so that a request's result does not depend on when it ran.
-        progress: print a status line every that many wall-clock seconds."""
+        to the host and dropped) or "none".  Snapshots are copied to
+        page-locked buffers (at most pinned_bytes of them) while the batch
+        keeps stepping, and a writer thread finishes them.  tile_shapes: the
+        sweeps' tile shapes (x, y, z) for the whole run (None: the library's
+        default), fixed so that a request's result does not depend on when
+        it ran.  progress: print a status line every that many wall-clock
+        seconds."""
         if snapshots not in ("hdf5", "download", "none"):
             raise ValueError(snapshots)
         if snapshots == "hdf5" and not outdir:
@@ -119,6 +174,7 @@ class Server:
         self.keep_final = keep_final
         self.tile_shapes = tile_shapes
         self.progress = progress
+        self.pinned_bytes = pinned_bytes
 
     def run(self, requests: List[Request]) -> dict:
         from .hip import MagicBatch
@@ -129,7 +185,9 @@ class Server:
         batch = MagicBatch([], [], [], meqn=g0.meqn, mbc=g0.mbc, maux=g0.maux)
         if self.tile_shapes is not None:
             batch.set_tile_shapes(self.tile_shapes)
-        writer = _Writer(self.outdir) if self.snapshots == "hdf5" else None
+        pool = _PinnedPool(self.pinned_bytes)
+        writer = (_Writer(self.outdir if self.snapshots == "hdf5" else None, pool)
+                  if self.snapshots != "none" else None)
         if writer:
             writer.start()
         out = {r.rid: Outcome(r.rid, r.arrival) for r in reqs}
@@ -191,20 +249,22 @@ class Server:
                     out[r.rid].steps += st["nsteps"]
                     out[r.rid].rejects += st["rejects"]
                     frame += 1
-                    qh = None
-                    if self.snapshots != "none" or (self.keep_final and st["t"] >= r.t_end):
-                        qh = batch.get_q(s)
+                    last = st["t"] >= r.t_end
                     path = None
                     if writer:
-                        writer.q.put((g, qh, st["t"], frame, r.rid))
-                        from . import h5
-                        path = os.path.join(self.outdir, f"req{r.rid:05d}", h5.frame_name(frame))
+                        buf = pool.get(8 * int(np.prod(g.shape_q)))
+                        handle = batch.snapshot_begin(s, buf)
+                        keep = out[r.rid] if (self.keep_final and last) else None
+                        writer.q.put((handle, buf, g, st["t"], frame, r.rid, keep))
+                        if self.snapshots == "hdf5":
+                            from . import h5
+                            path = os.path.join(self.outdir, f"req{r.rid:05d}", h5.frame_name(frame))
+                    elif self.keep_final and last:
+                        out[r.rid].final_q = batch.get_q(s)
                     out[r.rid].snapshots.append((st["t"], path))
                     gs = batch.global_sums(s)   # compensated FP64 sums of the full state
                     out[r.rid].sums.append((st["t"], float(gs[0]), float(gs[4])))
-                    if st["t"] >= r.t_end:
-                        if self.keep_final:
-                            out[r.rid].final_q = qh
+                    if last:
                         out[r.rid].completed = time.perf_counter() - t0
                         batch.remove(s)
                         del active[s]
@@ -220,6 +280,7 @@ class Server:
                 writer.q.put(None)
                 writer.join()
             batch.close()
+            pool.close()
         if writer and writer.error:
             raise writer.error
         oc = [out[r.rid] for r in reqs]

@@ -53,7 +53,7 @@ HYP_BYTES_PER_CELL_FP32 = 48.0
 SRC_BYTES_PER_CELL_FP32 = 48.0
 SWEEP_CLASSES = ["sweep_x", "sweep_y", "sweep_z"]   # timing classes of the three directions
 # kernel of a direction (ntr = meqn - 5 tracers, sh = the direction's tile shape)
-SWEEP_KERNEL = "magic::tile_sweep<{t}, {pert}, {ntr}, {sh}>"
+SWEEP_KERNEL = "magic::tile_sweep_list<{t}, {pert}, {ntr}, {sh}>"
 
 
 def measured_traffic(kernel, shapes):

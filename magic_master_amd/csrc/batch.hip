@@ -234,7 +234,8 @@ static int launch_dir(magic_batch *b, int dir, hipStream_t st = nullptr, int k0 
         const int4 *items = b->items_d + lo;
         const int *const *run = b->run_d;
         int *cnt = b->counters_d + 2 * (6 * dir + k);
-        void *kargs[] = {&args, &items, &n, &run, &cnt};
+        int4 grid = make_int4(0, 0, 0, 0);   // the work list
+        void *kargs[] = {&args, &items, &n, &run, &cnt, &grid};
         HIPCHECK(hipLaunchKernel(fn, dim3(b->grid[dir]), dim3(256), kargs, 0, st));
         TRY(launch_check());
     }

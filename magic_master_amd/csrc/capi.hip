@@ -612,25 +612,33 @@ static int list_items(magic_ctx *c, int dir, int sh, int nA, int nB, int a_is_g)
     return MAGIC_OK;
 }
 
-// Colour launches k0 .. k1-1 (k = 3 ca + cb) of one direction: the
-// persistent work-list kernel over this grid's tiles of each colour.
+// Colour launches k0 .. k1-1 (k = 3 ca + cb) of one direction: the sweep
+// kernel over this grid's tiles of each colour.  The FP64 form runs it as a
+// persistent grid over the work list (925 Mcell-updates/s on the full-domain
+// workload, against 918 for one tile per workgroup), the FP32 form one tile
+// per workgroup (1785 against 1728), which lets the two streams of the sweep
+// overlap share the compute units tile by tile.
 template <typename T, bool PERT, int NTR>
-static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s, int k0, int k1)
+static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *nl, hipStream_t s, int k0, int k1,
+                       bool put)
 {
     const void *fn = tile_list_fn<T, PERT, NTR>(sh);
     if (!fn) return MAGIC_EARG;
     const int dir = a.sweep - 1;
     TRY(list_setup<T>(c));
     TRY(list_items(c, dir, sh, a.nA, a.nB, a_is_g));
-    if (!c->lgrid[sh]) {
+    if (put) {   // else the caller wrote every slot of this step (k_put_step_args)
+        k_put_args<T><<<1, 64, 0, s>>>((SweepArgsT<T> *)c->largs, a, dir, k0, k1);
+        TRY(launch_check());
+    }
+    constexpr bool PER_TILE = sizeof(T) == 4;
+    if (!PER_TILE && !c->lgrid[sh]) {
         int dev = 0, ncu = 0, occ = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
         c->lgrid[sh] = ncu * (occ > 0 ? occ : 1);
     }
-    k_put_args<T><<<1, 64, 0, s>>>((SweepArgsT<T> *)c->largs, a, dir, k0, k1);
-    TRY(launch_check());
     const int *off = c->litem_off[dir][sh];
     for (int k = k0; k < k1; k++) {
         int n = off[k + 1] - off[k];
@@ -639,8 +647,11 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *n
         const int4 *items = c->litems[dir][sh] + off[k];
         const int *const *run = c->lrun;
         int *cnt = c->lcnt + 2 * (dir * 6 + k);
-        void *kargs[] = {&args, &items, &n, &run, &cnt};
-        HIPCHECK(hipLaunchKernel(fn, dim3(std::min(c->lgrid[sh], n)), dim3(256), kargs, 0, s));
+        // one tile per workgroup: tiles (bx, by) of colour k as the list holds them
+        const int TA = tile_ta(sh), ntiles = (a.nA + 2 + TA - 1) / TA, ca = k / 3;
+        int4 grid = PER_TILE ? make_int4((ntiles - ca + 1) / 2, dir * 6 + k, a_is_g, 0) : make_int4(0, 0, 0, 0);
+        void *kargs[] = {&args, &items, &n, &run, &cnt, &grid};
+        HIPCHECK(hipLaunchKernel(fn, dim3(PER_TILE ? n : std::min(c->lgrid[sh], n)), dim3(256), kargs, 0, s));
         ++*nl;
     }
     return MAGIC_OK;
@@ -648,13 +659,13 @@ static int launch_tile(magic_ctx *c, SweepArgsT<T> a, int sh, int a_is_g, int *n
 
 template <typename T, bool PERT>
 static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_g, int *nl, hipStream_t s, int k0,
-                         int k1)
+                         int k1, bool put)
 {
     switch (ntracer(c)) {
-    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s, k0, k1);
-    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s, k0, k1);
-    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s, k0, k1);
-    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s, k0, k1);
+    case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s, k0, k1, put);
+    case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s, k0, k1, put);
+    case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s, k0, k1, put);
+    case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     default: return MAGIC_EUNSUPPORTED;
     }
 }
@@ -709,21 +720,58 @@ static void dir_args(const magic_ctx *c, SweepArgsT<T> &a, int dir, double dt)
 // One directional sweep with the tiled kernel of shape sh (or the pencil
 // kernel of the FP64 form).  The tile axis is g for the x- and z-sweeps and
 // x = h for the y-sweep, so that the y- and z-sweeps tile along rows of x.
+template <typename T>
+static int tile_dir_args(const magic_ctx *c, SweepArgsT<T> &a, int dir, double dt)
+{
+    dir_args(c, a, dir, dt);
+    if (dir == 1) {
+        a.nA = c->mx; a.nB = c->mz;
+        a.sA = 1; a.sB = (long long)c->d.NX * c->d.NY;
+        return 0;
+    }
+    return 1;
+}
+
+// Argument blocks of all 18 launches of one step3 (slot 6 dir + k), from
+// each direction's block; dq: the sweep overlap's buffer, which the y-sweep
+// and the last three z colours add into (nullptr: no overlap).  Launched on
+// the context's stream before any of the step's sweeps, while nothing else
+// runs, so it never waits for a compute unit behind a sweep.
+template <typename T>
+__global__ void k_put_step_args(SweepArgsT<T> *slots, SweepArgsT<T> ax, SweepArgsT<T> ay, SweepArgsT<T> az, T *dq)
+{
+    const int k = (int)threadIdx.x;
+    if (k >= 18) return;
+    const int dir = k / 6, kk = k % 6;
+    SweepArgsT<T> s = dir == 0 ? ax : dir == 1 ? ay : az;
+    s.cA = kk / 3;
+    s.cB = kk % 3;
+    if (dq && (dir == 1 || (dir == 2 && kk >= 3))) s.qnew = dq;
+    slots[k] = s;
+}
+
+template <typename T>
+static int put_step_args(magic_ctx *c, const SweepArgsT<T> &a, double dt, T *dq)
+{
+    TRY(list_setup<T>(c));
+    SweepArgsT<T> ax = a, ay = a, az = a;
+    tile_dir_args(c, ax, 0, dt);
+    tile_dir_args(c, ay, 1, dt);
+    tile_dir_args(c, az, 2, dt);
+    k_put_step_args<T><<<1, 64, 0, c->stream>>>((SweepArgsT<T> *)c->largs, ax, ay, az, dq);
+    return launch_check();
+}
+
 template <typename T, bool PERT>
 static int sweep_dir(magic_ctx *c, SweepArgsT<T> a, int dir, double dt, int sh, bool tile, int *nl,
-                     hipStream_t s = nullptr, int k0 = 0, int k1 = 6)
+                     hipStream_t s = nullptr, int k0 = 0, int k1 = 6, bool put = true)
 {
     if (!s) s = c->stream;
-    dir_args(c, a, dir, dt);
     if (tile) {
-        int a_is_g = 1;
-        if (dir == 1) {
-            a.nA = c->mx; a.nB = c->mz;
-            a.sA = 1; a.sB = (long long)c->d.NX * c->d.NY;
-            a_is_g = 0;
-        }
-        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl, s, k0, k1);
+        const int a_is_g = tile_dir_args(c, a, dir, dt);
+        return tile_dispatch<T, PERT>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     }
+    dir_args(c, a, dir, dt);
     if (s != c->stream || k0 != 0 || k1 != 6) return MAGIC_EUNSUPPORTED;
     if constexpr (!PERT) return sweep_dispatch(c, a, nl);
     return MAGIC_EUNSUPPORTED;
@@ -848,6 +896,7 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
         T *dq = (T *)c->dq[c->dq_cur], *dq_next = (T *)c->dq[c->dq_cur ^ 1];
         SweepArgsT<T> ay = a;
         ay.qnew = dq;
+        TRY(put_step_args<T>(c, a, dt, dq));
         HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
         HIPCHECK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
         // dq_next was added to q by the previous call, before the fork
@@ -855,19 +904,19 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
         TRY(launch_check());
         {
             Span sp(c, TC_SWY, 0, c->side);
-            TRY((sweep_dir<T, PERT>(c, ay, 1, dt, c->shape[1], true, &sp.n, c->side)));
+            TRY((sweep_dir<T, PERT>(c, ay, 1, dt, c->shape[1], true, &sp.n, c->side, 0, 6, false)));
         }
         {
             Span sp(c, TC_SWZ, 0, c->side);
-            TRY((sweep_dir<T, PERT>(c, ay, 2, dt, c->shape[2], true, &sp.n, c->side, 3, 6)));
+            TRY((sweep_dir<T, PERT>(c, ay, 2, dt, c->shape[2], true, &sp.n, c->side, 3, 6, false)));
         }
         {
             Span sp(c, TC_SWX, 0);
-            TRY((sweep_dir<T, PERT>(c, a, 0, dt, c->shape[0], true, &sp.n)));
+            TRY((sweep_dir<T, PERT>(c, a, 0, dt, c->shape[0], true, &sp.n, c->stream, 0, 6, false)));
         }
         {
             Span sp(c, TC_SWZ, 0);
-            TRY((sweep_dir<T, PERT>(c, a, 2, dt, c->shape[2], true, &sp.n, c->stream, 0, 3)));
+            TRY((sweep_dir<T, PERT>(c, a, 2, dt, c->shape[2], true, &sp.n, c->stream, 0, 3, false)));
         }
         HIPCHECK(hipEventRecord(c->ev_join, c->side));
         HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -879,9 +928,10 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
         c->dq_clear = true;
         return MAGIC_OK;
     }
+    if (tile) TRY(put_step_args<T>(c, a, dt, (T *)nullptr));
     for (int dir = 0; dir < 3; dir++) {
         Span sp(c, TC_SWX + dir, 0);
-        TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], tile, &sp.n)));
+        TRY((sweep_dir<T, PERT>(c, a, dir, dt, c->shape[dir], tile, &sp.n, c->stream, 0, 6, !tile)));
     }
     return MAGIC_OK;
 }

@@ -86,11 +86,12 @@ struct magic_ctx {
     unsigned long long *dscal = nullptr;   // [0] cfl bits, [1..] status ints
     void *sink = nullptr;                  // 4 KiB the sweep's tile flush may write (never read back)
     int shape[3] = {0, 0, 0};              // tile shape of the x-, y- and z-sweeps
-    // The sweeps' work lists (set up on the first tiled sweep): 18 argument
-    // blocks (direction x colour, written by a kernel before each
-    // direction's launches), run flags all pointing at a device 1, 18 pairs
-    // of work counters, and per direction and shape the tiles of each
-    // colour (items of colour k at [litem_off[dir][sh][k], ..[k + 1]))
+    // The sweep kernel's operands (set up on the first tiled sweep): 18
+    // argument blocks (direction x colour, written by a kernel at the start
+    // of each step3 or of each tuner launch), run flags all pointing at a
+    // device 1, 18 pairs of work counters, and per direction and shape the
+    // tiles of each colour (items of colour k at [litem_off[dir][sh][k],
+    // ..[k + 1])
     void *largs = nullptr;
     const int **lrun = nullptr;
     int *lone = nullptr;

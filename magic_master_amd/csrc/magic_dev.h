@@ -54,11 +54,12 @@ __global__ void sweep_kernel(SweepArgs a);
 // tiles of one or several grids (items: {argument block, a_is_g, bx, by};
 // run[argument block] points at the grid's "runs in this attempt" flag;
 // next: the work counter and the count of finished workgroups, both 0 at
-// launch and set back to 0 by the kernel).
+// launch and set back to 0 by the kernel; grid: {tiles along a, argument
+// block, a_is_g, 0} for one tile per workgroup, x = 0 for the work list).
 constexpr int TILE_NSHAPE = 4;
 __host__ __device__ constexpr int tile_ta(int sh) { return sh == 0 ? 16 : sh == 3 ? 4 : 8; }
 template <typename T, bool PERT, int NTR, int SH>
 __global__ void tile_sweep_list(const SweepArgsT<T> *args, const int4 *items, int nitems, const int *const *run,
-                                int *next);
+                                int *next, int4 grid);
 
 }  // namespace magic

@@ -16,7 +16,7 @@ namespace magic {
 
 #define MAGIC_TILE64(NTR)                                                                                  \
     template __global__ void tile_sweep_list<double, false, NTR, MAGIC_TILE_SHAPE>(                     \
-        const SweepArgsT<double> *, const int4 *, int, const int *const *, int *);
+        const SweepArgsT<double> *, const int4 *, int, const int *const *, int *, int4);
 MAGIC_TILE64(0)
 MAGIC_TILE64(1)
 MAGIC_TILE64(2)

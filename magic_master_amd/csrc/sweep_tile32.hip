@@ -13,7 +13,7 @@ namespace magic {
 
 #define MAGIC_TILE32(NTR)                                                                                  \
     template __global__ void tile_sweep_list<float, true, NTR, MAGIC_TILE_SHAPE>(                        \
-        const SweepArgsT<float> *, const int4 *, int, const int *const *, int *);
+        const SweepArgsT<float> *, const int4 *, int, const int *const *, int *, int4);
 MAGIC_TILE32(0)
 MAGIC_TILE32(1)
 MAGIC_TILE32(2)

@@ -771,25 +771,38 @@ constexpr int tile_waves()
 }
 
 // The sweep kernel, for one grid and for a batch of grids alike (one
-// compiled body, so both round alike): a persistent grid whose workgroups
-// take tiles from a work list (item = {argument block, a_is_g, bx, by})
-// through the atomic counter next[0] until it runs out; tiles of grids whose
-// control flag (*run) is 0 in this attempt are skipped.  next[1] counts the
-// workgroups that are done; the last one sets both back to 0 for the next
-// launch (every other workgroup has taken its last item by then).
+// compiled body with one call site, so both round alike), in two modes:
+//  * work list (grid.x = 0): a persistent grid whose workgroups take tiles
+//    from a work list (item = {argument block, a_is_g, bx, by}) through the
+//    atomic counter next[0] until it runs out; tiles of grids whose control
+//    flag (*run) is 0 in this attempt are skipped.  next[1] counts the
+//    workgroups that are done; the last one sets both back to 0 for the
+//    next launch (every other workgroup has taken its last item by then);
+//  * one tile per workgroup (grid.x > 0 tiles along a): workgroup b takes
+//    tile {argument block grid.y, a_is_g grid.z, b mod grid.x, b / grid.x},
+//    as the hardware dispatches them, so that the launches of two streams
+//    share the compute units tile by tile (one grid's sweep overlap).
 template <typename T, bool PERT, int NTR, int SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(tile_waves<T, NTR, SH>())))
 void tile_sweep_list(const SweepArgsT<T> *__restrict__ args, const int4 *items, int nitems,
-                                                        const int *const *run, int *next)
+                                                        const int *const *run, int *next, int4 grid)
 {
     __shared__ int s_item;
+    bool first = true;
     for (;;) {
-        __syncthreads();   // the previous tile's readers of s_item and of the tile arrays are done
-        if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
-        __syncthreads();
-        const int it = s_item;
-        if (it >= nitems) break;
-        const int4 w = items[it];
+        int4 w;
+        if (grid.x > 0) {
+            if (!first) break;
+            first = false;
+            w = make_int4(grid.y, grid.z, (int)blockIdx.x % grid.x, (int)blockIdx.x / grid.x);
+        } else {
+            __syncthreads();   // the previous tile's readers of s_item and of the tile arrays are done
+            if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
+            __syncthreads();
+            const int it = s_item;
+            if (it >= nitems) break;
+            w = items[it];
+        }
         // the item is the same for the whole workgroup: scalar, so that the
         // argument block is read with scalar loads
         const int ai = __builtin_amdgcn_readfirstlane(w.x);
@@ -797,7 +810,7 @@ void tile_sweep_list(const SweepArgsT<T> *__restrict__ args, const int4 *items, 
         tile_sweep_body<T, PERT, NTR, SH>(args[ai], __builtin_amdgcn_readfirstlane(w.y),
                                           __builtin_amdgcn_readfirstlane(w.z), __builtin_amdgcn_readfirstlane(w.w));
     }
-    if (threadIdx.x == 0) {
+    if (grid.x == 0 && threadIdx.x == 0) {
         __threadfence();   // this workgroup's last take precedes its count
         if (atomicAdd(next + 1, 1) == (int)gridDim.x - 1) {
             __threadfence();

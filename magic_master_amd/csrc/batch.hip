@@ -214,6 +214,8 @@ static const void *list_fn(const magic_batch *b, int dir)
     case 0: return tile_list_fn<T, PERT, 0>(b->shape[dir]);
     case 1: return tile_list_fn<T, PERT, 1>(b->shape[dir]);
     case 2: return tile_list_fn<T, PERT, 2>(b->shape[dir]);
+    case 3: return tile_list_fn<T, PERT, 3>(b->shape[dir]);
+    case 4: return tile_list_fn<T, PERT, 4>(b->shape[dir]);
     case 5: return tile_list_fn<T, PERT, 5>(b->shape[dir]);
     default: return nullptr;
     }

@@ -5,7 +5,94 @@
 // the bc3 z-boundary density ratios, and the source-pass scratch planes.
 // Everything runs on one HIP stream of default priority; every entry point
 // returns after its work is complete.
+#include <mutex>
+#include <thread>
+
 #include "ctx.h"
+
+// ------------------------------------------------------ host transfers ---
+// q crosses between the caller's host memory (pageable: a Fortran array)
+// and the device through two page-locked bounce chunks shared by every
+// context (one transfer at a time): a chunk's DMA runs while host threads
+// copy the neighbouring chunk between the caller's memory and the other
+// bounce chunk.
+namespace {
+constexpr size_t BOUNCE = size_t(32) << 20;   // bytes per bounce chunk
+constexpr int COPY_THREADS = 8;
+std::mutex bounce_mu;
+char *bounce_buf[2] = {nullptr, nullptr};
+hipEvent_t bounce_ev[2] = {nullptr, nullptr};
+
+int bounce_setup()
+{
+    if (bounce_buf[0]) return MAGIC_OK;
+    for (int k = 0; k < 2; k++) {
+        if (hipHostMalloc((void **)&bounce_buf[k], BOUNCE, hipHostMallocDefault) != hipSuccess) return MAGIC_ENOMEM;
+        HIPCHECK(hipEventCreateWithFlags(&bounce_ev[k], hipEventDisableTiming));
+    }
+    return MAGIC_OK;
+}
+
+// memcpy of n bytes split over COPY_THREADS host threads
+void par_memcpy(char *dst, const char *src, size_t n)
+{
+    if (n < (size_t(1) << 20)) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n + COPY_THREADS - 1) / COPY_THREADS;
+    std::thread th[COPY_THREADS];
+    for (int t = 0; t < COPY_THREADS; t++) {
+        const size_t o = std::min(n, t * part), e = std::min(n, o + part);
+        th[t] = std::thread([=] { memcpy(dst + o, src + o, e - o); });
+    }
+    for (auto &t : th) t.join();
+}
+}  // namespace
+
+// host -> device (dev on the context's stream; returns with the copies queued)
+static int upload(magic_ctx *c, void *dev, const void *host, size_t bytes)
+{
+    std::lock_guard<std::mutex> lock(bounce_mu);
+    TRY(bounce_setup());
+    for (size_t off = 0, i = 0; off < bytes; off += BOUNCE, i++) {
+        const size_t n = std::min(BOUNCE, bytes - off);
+        const int k = (int)(i & 1);
+        HIPCHECK(hipEventSynchronize(bounce_ev[k]));   // the copy that last read this chunk is done
+        par_memcpy(bounce_buf[k], (const char *)host + off, n);
+        HIPCHECK(hipMemcpyAsync((char *)dev + off, bounce_buf[k], n, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipEventRecord(bounce_ev[k], c->stream));
+    }
+    // the bounce chunks are read until these copies run: wait before unlocking
+    for (auto e : bounce_ev) HIPCHECK(hipEventSynchronize(e));
+    return MAGIC_OK;
+}
+
+// device -> host (after the work queued on the context's stream)
+static int download(magic_ctx *c, void *host, const void *dev, size_t bytes)
+{
+    std::lock_guard<std::mutex> lock(bounce_mu);
+    TRY(bounce_setup());
+    size_t prev_off = 0, prev_n = 0;
+    for (size_t off = 0, i = 0; off < bytes; off += BOUNCE, i++) {
+        const size_t n = std::min(BOUNCE, bytes - off);
+        const int k = (int)(i & 1);
+        HIPCHECK(hipMemcpyAsync(bounce_buf[k], (const char *)dev + off, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipEventRecord(bounce_ev[k], c->stream));
+        if (i > 0) {   // the previous chunk, while this one's copy runs
+            HIPCHECK(hipEventSynchronize(bounce_ev[k ^ 1]));
+            par_memcpy((char *)host + prev_off, bounce_buf[k ^ 1], prev_n);
+        }
+        prev_off = off;
+        prev_n = n;
+    }
+    if (prev_n) {
+        const int k = (int)(((bytes + BOUNCE - 1) / BOUNCE - 1) & 1);
+        HIPCHECK(hipEventSynchronize(bounce_ev[k]));
+        par_memcpy((char *)host + prev_off, bounce_buf[k], prev_n);
+    }
+    return MAGIC_OK;
+}
 
 extern "C" const char *magic_strerror(int s)
 {
@@ -327,7 +414,7 @@ extern "C" int magic_set_q(magic_ctx *c, const double *q)
     if (c->form == MAGIC_FORM_FP32P) {
         if (!c->have_aux) return MAGIC_EARG;   // the perturbation needs the background
         const int NZ = c->d.NZ;
-        HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        TRY(upload(c, c->stage, q, n * sizeof(double)));
         k32_aos_to_pert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(c->d, c->q32, c->stage, c->meqn,
                                                                c->bg64 + B64_RHO * NZ, c->bg64 + B64_E * NZ);
         TRY(launch_check());
@@ -335,7 +422,7 @@ extern "C" int magic_set_q(magic_ctx *c, const double *q)
         HIPCHECK(hipStreamSynchronize(c->stream));
         return MAGIC_OK;
     }
-    HIPCHECK(hipMemcpyAsync(c->stage, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(upload(c, c->stage, q, n * sizeof(double)));
     k_aos_to_planes<<<blocks(c->P, 256), 256, 0, c->stream>>>(c->q, c->stage, c->P, c->meqn);
     TRY(launch_check());
     HIPCHECK(hipMemcpyAsync(c->qold, c->q, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -366,9 +453,7 @@ extern "C" int magic_get_q(magic_ctx *c, double *q)
     if (!c || !q) return MAGIC_EARG;
     const size_t n = (size_t)c->P * c->meqn;
     TRY(stage_q(c));
-    HIPCHECK(hipMemcpyAsync(q, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    return MAGIC_OK;
+    return download(c, q, c->stage, n * sizeof(double));
 }
 
 extern "C" int magic_snapshot_begin(magic_ctx *c, double *q, void **done)
@@ -665,6 +750,8 @@ static int tile_dispatch(magic_ctx *c, const SweepArgsT<T> &a, int sh, int a_is_
     case 0: return launch_tile<T, PERT, 0>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     case 1: return launch_tile<T, PERT, 1>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     case 2: return launch_tile<T, PERT, 2>(c, a, sh, a_is_g, nl, s, k0, k1, put);
+    case 3: return launch_tile<T, PERT, 3>(c, a, sh, a_is_g, nl, s, k0, k1, put);
+    case 4: return launch_tile<T, PERT, 4>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     case 5: return launch_tile<T, PERT, 5>(c, a, sh, a_is_g, nl, s, k0, k1, put);
     default: return MAGIC_EUNSUPPORTED;
     }
@@ -682,6 +769,8 @@ static int sweep_dispatch(magic_ctx *c, const SweepArgs &a, int *nl)
     case 0: return launch_sweep<0>(c, a, nl);
     case 1: return launch_sweep<1>(c, a, nl);
     case 2: return launch_sweep<2>(c, a, nl);
+    case 3: return launch_sweep<3>(c, a, nl);
+    case 4: return launch_sweep<4>(c, a, nl);
     case 5: return launch_sweep<5>(c, a, nl);
     default: return MAGIC_EUNSUPPORTED;
     }

@@ -193,7 +193,8 @@ static inline int diff_kb(const Dims &d)
 }
 static inline dim3 diff_grid(const Dims &d) { return dim3(blocks(d.mx, 64), blocks(d.my, 4), blocks(d.mz, diff_kb(d))); }
 
-static inline bool sweep_supported(int ntr) { return ntr == 0 || ntr == 1 || ntr == 2 || ntr == 5; }
+// every tracer count of meqn 5 .. 10 (rpt3euX.f:134-138 splits tracers 6 .. meqn)
+static inline bool sweep_supported(int ntr) { return ntr >= 0 && ntr <= 5; }
 
 // ------------------------------------------------------- tile shapes ----
 // The tiled sweep kernel of shape sh (0 .. TILE_NSHAPE-1): the persistent

@@ -369,6 +369,8 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs A)
 template __global__ void sweep_kernel<0>(SweepArgs);
 template __global__ void sweep_kernel<1>(SweepArgs);
 template __global__ void sweep_kernel<2>(SweepArgs);
+template __global__ void sweep_kernel<3>(SweepArgs);
+template __global__ void sweep_kernel<4>(SweepArgs);
 template __global__ void sweep_kernel<5>(SweepArgs);
 
 }  // namespace magic

@@ -20,6 +20,8 @@ namespace magic {
 MAGIC_TILE64(0)
 MAGIC_TILE64(1)
 MAGIC_TILE64(2)
+MAGIC_TILE64(3)
+MAGIC_TILE64(4)
 MAGIC_TILE64(5)
 #undef MAGIC_TILE64
 

@@ -17,6 +17,8 @@ namespace magic {
 MAGIC_TILE32(0)
 MAGIC_TILE32(1)
 MAGIC_TILE32(2)
+MAGIC_TILE32(3)
+MAGIC_TILE32(4)
 MAGIC_TILE32(5)
 #undef MAGIC_TILE32
 

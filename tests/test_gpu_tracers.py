@@ -1,6 +1,6 @@
-"""Tracer counts other than one: meqn = 5 (no tracer), 7 and 10 (2 and 5
-tracers, some advected with the tracer limiter and the rest in continuity
-form), i.e. every sweep instantiation the library carries.
+"""Tracer counts other than one: meqn = 5 (no tracer), 7, 8, 9 and 10 (2, 3,
+4 and 5 tracers, some advected with the tracer limiter and the rest in
+continuity form), i.e. every sweep instantiation the library carries.
 
 * FP64 form against the C oracle (itself pinned bit for bit to the compiled
   reference for these counts, tests/test_oracle_reference.py): one step3 call
@@ -19,7 +19,7 @@ from oracle.oracle import COracle
 from tests.helpers import perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
-COUNTS = [(5, 0), (7, 1), (10, 3)]
+COUNTS = [(5, 0), (7, 1), (8, 2), (9, 1), (10, 3)]
 GRID = ((12, 10, 8), [2, 2, 2, 2, 3, 1])
 
 

@@ -59,7 +59,7 @@ def test_claw3_bitwise(ci, both):
     assert np.array_equal(qo, qr)
 
 
-@pytest.mark.parametrize("meqn,nspec", [(5, 0), (7, 1), (7, 2), (10, 3)])
+@pytest.mark.parametrize("meqn,nspec", [(5, 0), (7, 1), (7, 2), (8, 2), (9, 1), (10, 3)])
 def test_tracer_counts_bitwise(meqn, nspec, both):
     """No tracer, and 2 and 5 tracers with some advected (tracer limiter,
     rpn3euX.f:449-456) and the rest in continuity form (minmod, :462)."""

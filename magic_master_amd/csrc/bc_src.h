@@ -68,7 +68,7 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
 template <typename T, class G>
 __global__ void k_clamp(T *q, long long P, int nc, G gate);
 template <bool COL, class G>
-__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate);
+__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate, const double *dq, int nc);
 template <typename T, class G>
 __global__ void k_zghost_uvw(Dims d, T *q, G gate);
 template <typename T, bool COLC, class G>
@@ -100,7 +100,7 @@ struct Zr32 {
 template <class G>
 __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G gate);
 template <class G>
-__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate);
+__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate, const float *dq, int nc);
 template <class G>
 __global__ void k32_src_finish(Dims d, float *q, int nc, Col32 col, Forcing f, G gate);
 template <typename T, class G>

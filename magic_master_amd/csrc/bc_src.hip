@@ -69,12 +69,20 @@ __global__ void k_bc_z(Dims d, double *q, int nc, int lo, int hi, const double *
 // ---------------------------------------------------------------- src3 ---
 // conserved -> (rho, u', v', w', T') on every cell: src3.f90:242-304.
 // COL: the background comes as columns (index kk), else as planes.
+// dq (one grid's sweeps on two streams): the second stream's increments,
+// added to every component of the cell first, as a separate q += dq pass
+// would add them (nc components; tracers are only added).
 template <bool COL, class G>
-__device__ __forceinline__ void b_src_convert(const uint3 BI, Dims d, double *q, const Aux a, const G &gate)
+__device__ __forceinline__ void b_src_convert(const uint3 BI, Dims d, double *q, const Aux a, const G &gate,
+                                              const double *dq = nullptr, int nc = 0)
 {
     if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
+    if (dq) {
+        q[c] = q[c] + dq[c];
+        for (int m = 1; m < nc; m++) q[m * P + c] = q[m * P + c] + dq[m * P + c];
+    }
     const long long x = COL ? (long long)kk : c;
     const double gamma1 = a.gam[x] - 1.0;
     const double q1 = q[c], q2 = q[P + c], q3 = q[2 * P + c], q4 = q[3 * P + c];
@@ -89,9 +97,9 @@ __device__ __forceinline__ void b_src_convert(const uint3 BI, Dims d, double *q,
     q[3 * P + c] = q4 / q1 - a4 / a1;
 }
 template <bool COL, class G>
-__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate)
+__global__ void k_src_convert(Dims d, double *q, const Aux a, G gate, const double *dq, int nc)
 {
-    b_src_convert<COL>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, a, gate);
+    b_src_convert<COL>(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, a, gate, dq, nc);
 }
 
 // src3.f90:380-418 (reassembly), 430-445 / 514-607 (forcing), 616-626
@@ -205,8 +213,8 @@ template __global__ void k_clamp<double, NoGate>(double *, long long, int, NoGat
 template __global__ void k_zghost_uvw<double, NoGate>(Dims, double *, NoGate);
 template __global__ void k_bc_z<NoGate>(Dims, double *, int, int, int, const double *, const double *, const double *,
                                         const double *, const double *, NoGate);
-template __global__ void k_src_convert<false, NoGate>(Dims, double *, const Aux, NoGate);
-template __global__ void k_src_convert<true, NoGate>(Dims, double *, const Aux, NoGate);
+template __global__ void k_src_convert<false, NoGate>(Dims, double *, const Aux, NoGate, const double *, int);
+template __global__ void k_src_convert<true, NoGate>(Dims, double *, const Aux, NoGate, const double *, int);
 template __global__ void k_src_finish<false, NoGate>(Dims, double *, int, const Aux, Forcing, NoGate);
 template __global__ void k_src_finish<true, NoGate>(Dims, double *, int, const Aux, Forcing, NoGate);
 template __global__ void k_diff_xyz<double, true, NoGate>(Dims, double *, const double *, const double *, double, double,

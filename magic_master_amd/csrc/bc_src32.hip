@@ -90,12 +90,19 @@ __global__ void k32_bc_z(Dims d, float *q, int nc, int lo, int hi, Zr32 zr, G ga
 // stored values are FP32): T' is a small difference of the pressure and
 // density perturbations' background-weighted parts, and E' a sum of such
 // parts, whose FP32 rounding the flow carries into the thin upper layers.
+// dq: as the FP64 form's conversion (bc_src.hip), the second sweep
+// stream's increments added first, each rounded to FP32 as q += dq rounds
 template <class G>
-__device__ __forceinline__ void b32_src_convert(const uint3 BI, Dims d, float *q, Col32 col, const G &gate)
+__device__ __forceinline__ void b32_src_convert(const uint3 BI, Dims d, float *q, Col32 col, const G &gate,
+                                                const float *dq = nullptr, int nc = 0)
 {
     if (!gate.on()) return;
     PADDED_IJK
     const long long P = d.P;
+    if (dq) {
+        q[c] = q[c] + dq[c];
+        for (int m = 1; m < nc; m++) q[m * P + c] = q[m * P + c] + dq[m * P + c];
+    }
     const double rb = col.rho[kk], Eb = col.E[kk], g1 = col.gam[kk] - 1.0, R = col.R[kk];
     const double rp = q[c], rho = rb + rp;
     const double m2 = q[P + c], m3 = q[2 * P + c], m4 = q[3 * P + c], ep = q[4 * P + c];
@@ -107,9 +114,9 @@ __device__ __forceinline__ void b32_src_convert(const uint3 BI, Dims d, float *q
     q[3 * P + c] = (float)(m4 * irho);
 }
 template <class G>
-__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate)
+__global__ void k32_src_convert(Dims d, float *q, Col32 col, G gate, const float *dq, int nc)
 {
-    b32_src_convert(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, col, gate);
+    b32_src_convert(uint3{blockIdx.x, blockIdx.y, blockIdx.z}, d, q, col, gate, dq, nc);
 }
 
 // Reassembly, plume forcing and tracer clamp on the interior
@@ -198,7 +205,7 @@ const void *batch_op_fn<float>(int op)
 // instantiations of the single-grid kernels (the batched driver's are
 // k_batch_op, above)
 template __global__ void k32_bc_z<NoGate>(Dims, float *, int, int, int, Zr32, NoGate);
-template __global__ void k32_src_convert<NoGate>(Dims, float *, Col32, NoGate);
+template __global__ void k32_src_convert<NoGate>(Dims, float *, Col32, NoGate, const float *, int);
 template __global__ void k32_src_finish<NoGate>(Dims, float *, int, Col32, Forcing, NoGate);
 
 // reference AoS q(meqn, i, j, k) (FP64, full state) <-> perturbation planes:

@@ -974,7 +974,7 @@ static int side_setup(magic_ctx *c)
 // then dq's sum of y and the other z colours), so the result is the same
 // on every run.
 template <typename T, bool PERT>
-static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
+static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt, bool defer_merge = false)
 {
     const bool tile = PERT || use_tile();
     if (tile && overlap_sweeps(c)) {
@@ -1009,12 +1009,16 @@ static int step3_sweeps(magic_ctx *c, const SweepArgsT<T> &a, double dt)
         }
         HIPCHECK(hipEventRecord(c->ev_join, c->side));
         HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        c->dq_cur ^= 1;
+        c->dq_clear = true;
+        if (defer_merge) {
+            c->merge_dq = dq;
+            return MAGIC_OK;
+        }
         Span sp(c, TC_OTHER, 1);
         k_merge<T><<<(unsigned)std::min<long long>(blocks(n / Vec16<T>::N + 1, 256), 8192), 256, 0, c->stream>>>(
             a.qnew, dq, n);
         TRY(launch_check());
-        c->dq_cur ^= 1;
-        c->dq_clear = true;
         return MAGIC_OK;
     }
     if (tile) TRY(put_step_args<T>(c, a, dt, (T *)nullptr));
@@ -1066,12 +1070,13 @@ static SweepArgs args64(magic_ctx *c)
     return a;
 }
 
-static int step3_launch(magic_ctx *c, double dt)
+// defer_merge: leave the overlapped sweeps' q += dq to the caller (merge_dq)
+static int step3_launch(magic_ctx *c, double dt, bool defer_merge = false)
 {
     k_zero_scalars<<<1, 64, 0, c->stream>>>(c->dscal);
     TRY(launch_check());
-    if (c->form == MAGIC_FORM_FP32P) TRY((step3_sweeps<float, true>(c, args32(c), dt)));
-    else TRY((step3_sweeps<double, false>(c, args64(c), dt)));
+    if (c->form == MAGIC_FORM_FP32P) TRY((step3_sweeps<float, true>(c, args32(c), dt, defer_merge)));
+    else TRY((step3_sweeps<double, false>(c, args64(c), dt, defer_merge)));
     HIPCHECK(hipMemcpyAsync(c->hscal, c->dscal, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c->stream));
     return MAGIC_OK;
@@ -1089,6 +1094,24 @@ static int step3_finish(magic_ctx *c, double *cfl)
         return st[0] == 1 ? MAGIC_ERHO : MAGIC_EA2;
     }
     return MAGIC_OK;
+}
+
+// the pending q += dq of a deferred merge, as its own pass
+template <typename T>
+static int merge_launch(magic_ctx *c, T *q)
+{
+    const long long n = (long long)c->meqn * c->P;
+    Span sp(c, TC_OTHER, 1);
+    k_merge<T><<<(unsigned)std::min<long long>(blocks(n / Vec16<T>::N + 1, 256), 8192), 256, 0, c->stream>>>(
+        q, (const T *)c->merge_dq, n);
+    return launch_check();
+}
+static int merge_flush(magic_ctx *c)
+{
+    if (!c->merge_dq) return MAGIC_OK;
+    const int st = c->form == MAGIC_FORM_FP32P ? merge_launch<float>(c, c->q32) : merge_launch<double>(c, c->q);
+    c->merge_dq = nullptr;
+    return st;
 }
 
 extern "C" int magic_step3(magic_ctx *c, double dt, double *cfl)
@@ -1239,7 +1262,10 @@ static int src3_passes32(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const Col32 col = col32_of(c);
     {
         Span sp(c, TC_SRC, 1);
-        k32_src_convert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(d, c->q32, col, NoGate{});
+        // the pending q += dq of the overlapped sweeps, if any, goes with the conversion
+        k32_src_convert<<<padded_cells(c->d), cell_block(), 0, c->stream>>>(d, c->q32, col, NoGate{},
+                                                                             (const float *)c->merge_dq, c->meqn);
+        c->merge_dq = nullptr;
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -1318,7 +1344,10 @@ static int src3_passes64(magic_ctx *c, const int *mthbc, int ndtr, double dtr, i
     const dim3 gface(blocks(d.NX, 64), d.NY);
     {
         Span sp(c, TC_SRC, 1);
-        k_src_convert<COL><<<padded_cells(d), cell_block(), 0, c->stream>>>(d, c->q, ax, NoGate{});
+        // the pending q += dq of the overlapped sweeps, if any, goes with the conversion
+        k_src_convert<COL><<<padded_cells(d), cell_block(), 0, c->stream>>>(d, c->q, ax, NoGate{},
+                                                                            (const double *)c->merge_dq, c->meqn);
+        c->merge_dq = nullptr;
         TRY(launch_check());
     }
     for (int nn = 1; nn <= ndtr; nn++) {
@@ -1453,7 +1482,7 @@ extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv
             if ((st = bc3_launch(c, mthbc))) goto out;
             if ((st = b4step3_launch(c))) goto out;
             if ((st = copy_state(c, true))) goto out;
-            if ((st = step3_launch(c, dt))) goto out;
+            if ((st = step3_launch(c, dt, true))) goto out;
             if ((st = step3_finish(c, &cfl))) goto out;
             t = told + dt;
             if (log_dt && nl < maxlog) {
@@ -1467,6 +1496,7 @@ extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv
                 break;
             }
             t = told;
+            c->merge_dq = nullptr;   // the rejected step's increments are dropped with it
             if ((st = copy_state(c, false))) goto out;
             dt = dtv[1] < dt * cflv[1] / cfl ? dtv[1] : dt * cflv[1] / cfl;
         }
@@ -1482,6 +1512,11 @@ extern "C" int magic_claw3(magic_ctx *c, double tstart, double *tend, double dtv
         if (t >= *tend) break;
     }
 out:
+    {
+        // a step that stopped before its source pass still ends with q += dq
+        const int sm = merge_flush(c);
+        if (st == MAGIC_OK) st = sm;
+    }
     hipStreamSynchronize(c->stream);
     if (c->timing) drain_timing(c);
     if (nlog) *nlog = nl < maxlog ? nl : maxlog;

@@ -75,6 +75,10 @@ struct magic_ctx {
     void *dq[2] = {nullptr, nullptr};
     int dq_cur = 0;
     bool dq_clear = false;   // dq[dq_cur] holds zeros (false until set up, and after a failed overlapped step)
+    // Inside magic_claw3 the q += dq pass after an overlapped step3 is left
+    // to the source pass's conversion (one pass over q instead of two);
+    // merge_dq is the buffer still to be added (null: nothing pending).
+    void *merge_dq = nullptr;
     // Asynchronous snapshots (magic_snapshot_begin): the conversion into
     // stage runs on the context's stream, the copy out of stage on cstream
     // (created on first use), so that later work on the stream need not

@@ -119,6 +119,45 @@ def test_claw3_steps(case, oracle):
     assert rel_err(qg, qo, interior(g)) <= 1e-9
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_claw3_equals_its_operators_bitwise(form):
+    """claw3 adds the second sweep stream's increments to q inside the source
+    pass's conversion (one pass over q); the operators called one by one add
+    them at the end of step3.  The same additions in the same order: claw3's
+    state equals the operator sequence's bit for bit, through a rejected
+    attempt (dtv(1) = 50 s) and three accepted steps."""
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicHIP
+    (mx, my, mz), mthbc = CASES[1]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=7, vel=0.05)
+    fm = FORM_FP64 if form == "fp64" else FORM_FP32P
+    print(f"[claw3 vs operators] {form} grid {mx}x{my}x{mz}", flush=True)
+    a, b = MagicHIP(g, p, aux, q, form=fm), MagicHIP(g, p, aux, q, form=fm)
+    try:
+        dtv = [50.0, 10.0, 0, 0, 0]
+        ta = tb = 0.0
+        attempts = []
+        for _ in range(3):
+            ta, dtv, _, _, log = a.claw3(ta, ta - 1.0, dtv, c.cflv, [100, 0], c.method, mthbc)
+            attempts += list(zip(log.dt, log.accepted))
+            dtv[0] = dtv[4]
+        assert [acc for _, acc in attempts].count(False) >= 1
+        for dt, acc in attempts:
+            b.bc3(mthbc)
+            b.b4step3()
+            b.backup()
+            b.step3(dt)
+            if not acc:
+                b.restore()
+                continue
+            b.src3(tb, dt, mthbc)
+            tb = tb + dt
+        assert ta == tb
+        np.testing.assert_array_equal(a.get_q(), b.get_q())
+    finally:
+        a.close()
+        b.close()
+
+
 def test_src3_background_varying_within_layers(oracle):
     """The FP64 source pass reads the background from planes when it varies
     within a layer (here nu, kappa and the background x momentum vary from

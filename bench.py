@@ -150,18 +150,19 @@ def run_gpu(workload, steps, warmup, form="fp64", tune=True, shapes=None):
                 inc_steps=inc_steps, inc_wall=inc_wall)
 
 
-def run_ensemble(steps, warmup, nscen=64, seed=2024, tune=True):
-    """The batched FP32 ensemble: nscen seeded scenarios (problem.ensemble),
-    all advanced by the batched driver (device-resident step control, one
-    graph replay per attempted step of every scenario).  warmup accepted
-    steps, the batch autotuner's choice of tile shapes (graph capture
-    included), then steps accepted steps of every scenario timed on the host
-    clock around magic_batch_run."""
+def run_ensemble(steps, warmup, nscen=64, seed=2024, tune=True, form="fp32"):
+    """The batched ensemble: nscen seeded scenarios (problem.ensemble), all
+    advanced by the batched driver (device-resident step control, one graph
+    replay per attempted step of every scenario) in the FP32 perturbation
+    form or the FP64 form.  warmup accepted steps, the batch autotuner's
+    choice of tile shapes (graph capture included), then steps accepted
+    steps of every scenario timed on the host clock around magic_batch_run."""
     from magic_master_amd import problem as P
-    from magic_master_amd.hip import MagicBatch
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MagicBatch
 
     ens = P.ensemble(nscen, seed)
-    b = MagicBatch([e[2] for e in ens], [e[1] for e in ens], [e[3] for e in ens], None)
+    b = MagicBatch([e[2] for e in ens], [e[1] for e in ens], [e[3] for e in ens], None,
+                   form=FORM_FP32P if form == "fp32" else FORM_FP64)
     try:
         for s, (c, p, g, col) in enumerate(ens):
             b.start(s, 0.0, 1e300, c.dtv[0], c.dtv, c.cflv, warmup, c.mthbc)
@@ -355,6 +356,8 @@ def main():
                     help="skip the FP32 batched ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-fp64-batch", action="store_true",
+                    help="skip the FP64 form of the batched ensemble")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-window", type=float, default=20.0)
     ap.add_argument("--ensemble-only", action="store_true",
@@ -478,6 +481,14 @@ def main():
                               "separately (python -m magic_master_amd.report --ensemble 64 --t-end 420 "
                               "--check 64): at 420 s, w and rho' within 1.1e-5 of their peaks in all 64 "
                               "scenarios, T' within the 1e-4 bar in 62 of 64 (worst 1.25e-4)")}
+        if not a.no_fp64_batch:
+            e64 = run_ensemble(max(4, a.ensemble_steps // 2), 2, tune=not a.no_tune, form="fp64")
+            out["fp64_batched"] = {
+                "value": round(e64["value"], 3), "unit": "Mcell-updates/s", "dtype": "f64",
+                "vs_fp64_single_grid": round(e64["value"] / value, 3),
+                "workload": "the same 64 scenarios in the FP64 form, batched alike",
+                "steps_per_scenario": max(4, a.ensemble_steps // 2), "attempts": e64["attempts"],
+                "rejected": e64["rejected"], "tile_shapes": e64["shapes"], "wall_s": round(e64["wall"], 4)}
     if not a.no_serving:
         r = run_serving(a.serve_rate, a.serve_window, shapes=ens_shapes)
         out["serving"] = {

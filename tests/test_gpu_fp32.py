@@ -5,7 +5,9 @@ reported and held to the same bar), measured by magic_master_amd.report:
   * small grids, 10 claw3 steps, against the C oracle (FP64);
   * the full-domain workload (250x200x300) with the plume on from t = 0 and
     peak amplitudes 5, 20 and 60 mm/10 min, against the FP64 form on the GPU
-    (itself pinned to the oracle), at every 60 s of model time up to 120 s.
+    (itself pinned to the oracle), at every 60 s of model time up to 120 s;
+  * the full-domain workload as stated (plume peak at 2400 s) through its
+    500 steps, with the compensated global sums within 1e-9.
 The dt sequences agree to the FP32 rounding of the wave speeds (1e-5).
 """
 import copy
@@ -96,3 +98,28 @@ def test_fp32_form_full_domain_vs_fp64(full_domain, amplitude):
     for (t64, q64), (t32, q32) in zip(runs[FORM_FP64], runs[FORM_FP32P]):
         assert abs(t64 - t32) <= 1e-9 * t64
         _check(form_error(g, q32, q64, aux), f"amp {amplitude} t {t64:.0f}s")
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("amplitude", [5.0, 20.0, 60.0])
+def test_fp32_form_full_domain_500_steps(full_domain, amplitude):
+    """The full-domain workload as stated (plume peak at t = 2400 s) through
+    its 500 steps: both forms to t = 780 s (about 560 steps; the amplitude-5
+    plume's fields leave round-off level between 720 and 780 s), then w, T'
+    and rho' within 1e-4 of their FP64 peaks and the compensated global mass
+    and energy within 1e-9."""
+    from magic_master_amd.report import compare_forms
+    c, p0, g, aux, q = full_domain
+    p = copy.copy(p0)
+    p.amplitude = amplitude
+    print(f"[fp32 full 500 steps] amplitude {amplitude}", flush=True)
+    rows = compare_forms(c, p, g, aux, q, 780.0, 780.0)
+    r = rows[-1]
+    print(f"  t {r['t']:.0f} s steps {r['steps']}: " +
+          ", ".join(f"{k} {r[k]['max_rel']:.2e} of peak {r[k]['peak']:.2e}" for k in ("w", "T'", "rho'")) +
+          f", mass {r['mass']['rel']:.1e}, energy {r['energy']['rel']:.1e}", flush=True)
+    assert min(r["steps"]) >= 500 and r["steps"][0] == r["steps"][1]
+    for k in ("w", "T'", "rho'"):
+        assert r[k]["peak"] > 1e3 * {"w": 1e-6, "T'": 1e-6, "rho'": 1e-9}[k], (k, r[k])   # above round-off
+        assert r[k]["max_rel"] <= TOL, (k, r[k])
+    assert abs(r["mass"]["rel"]) <= 1e-9 and abs(r["energy"]["rel"]) <= 1e-9, r

@@ -11,7 +11,8 @@
 !   0  isothermal, rho = dens*exp(-z*gamma*grav/speed**2), R = 287
 !      (the formulas of setaux.f:55-87)
 !   1  the vertical profile of profile.data in the working directory, as
-!      setaux.f:38-212 builds it (no analytic winds: amp0 = amp1 = 0)
+!      setaux.f:38-212 builds it (with its analytic x wind when amp0 or
+!      amp1 is set)
 !   2  the standard temperature column integrated hydrostatically
 !      (same as magic_master_amd/problem.py:setaux_column)
 ! Frames are written by out3 as the reference's HDF5 files
@@ -229,7 +230,7 @@ contains
     end do
   end subroutine background
 
-  ! loadprof = 1 (setaux.f:38-212 without analytic winds): for each cell
+  ! loadprof = 1 (setaux.f:38-212): for each cell
   ! centre height the profile.data row with exactly that height in km (rows
   ! read in order; a missing height stops the run), columns height, O, N2,
   ! O2 number densities (CGS), density, temperature, H and electron number
@@ -238,13 +239,17 @@ contains
   ! composition ratios (slots 6-10), molecular and thermal diffusivities
   ! (slots 11, 12; layers above the 400th take the 400th's) and the virtual
   ! gravity from the pressure jumps (slot 17; 0 on the lowest ghost layer).
+  ! With amp0 or amp1 set, the analytic x wind replaces the profile's winds
+  ! (setaux.f:128-134): rho*wind_x + amp0*rho (amp0 enters twice, as in the
+  ! reference) and a y slot of 0.
   ! The literals keep the reference's kinds (3.43, 0.69, ... are default
   ! REAL), so the values are the reference's bit for bit.
   subroutine profile_background(a)
     double precision, intent(inout) :: a(maux, 1-mbc:mx+mbc, 1-mbc:my+mbc, 1-mbc:mz+mbc)
+    common /wind/ omega1, amp0, amp1, prop1z, z1width, z1pos, z1Gpos, phi, windscale1
     double precision :: col(19, 1-mbc:mz+mbc)
     double precision :: height, dox, dnit2, dox2, rhof, temp, dhyd, dne, windx, windy
-    double precision :: zc, rho, gm1, cnv, cp, dmolec, dtherm, pl, pr
+    double precision :: zc, rho, gm1, cnv, cp, dmolec, dtherm, pl, pr, scale
     integer :: k, ios
     col = 0.d0
     cnv = 1.d3                  ! profile.data in CGS (setaux.f:20-21, 111-116)
@@ -269,8 +274,17 @@ contains
        rho = (dox*16.d0 + dox2*32.d0 + dnit2*28.d0) * cnv
        rho = rho / (6.022d23)
        col(1, k) = rho
-       col(2, k) = rho * windx
-       col(3, k) = rho * windy
+       if (amp0 == 0.d0 .and. amp1 == 0.d0) then
+          col(2, k) = rho * windx
+          col(3, k) = rho * windy
+       else
+          ! col(1, 1) is still 0 in the ghost layers below k = 1, where the
+          ! reference reads aux(1,i,j,1) before setting it (a zeroed array)
+          scale = (col(1, 1) / rho)**0.5d0
+          col(2, k) = rho * wind_x(zc, scale)
+          col(2, k) = col(2, k) + amp0 * rho
+          col(3, k) = 0.d0
+       end if
        col(4, k) = 0.d0
        col(5, k) = rho * temp * col(18, k) / gm1 + .5*(col(2, k)**2) / rho + .5*(col(3, k)**2) / rho
        col(6, k) = dox*16.d0*cnv / (6.022d23*rho)
@@ -303,6 +317,18 @@ contains
        end do
     end do
   end subroutine profile_background
+
+  ! The analytic background wind in x (fcns.f:100-116) at t = 0, which is
+  ! what setaux asks for (it passes the integer constant 0 for t).
+  double precision function wind_x(z, scale)
+    double precision, intent(in) :: z, scale
+    common /wind/ omega1, amp0, amp1, prop1z, z1width, z1pos, z1Gpos, phi, windscale1
+    wind_x = amp1
+    wind_x = wind_x * cos(prop1z * (z - z1pos) - omega1 * 0.d0 - phi)
+    wind_x = wind_x * exp(-.5d0 * (((z - z1Gpos)**2) / z1width**2))
+    if (windscale1 == 1.d0) wind_x = wind_x * scale
+    wind_x = wind_x + amp0
+  end function wind_x
 
   subroutine write_frame(iframe, t)
     integer, intent(in) :: iframe

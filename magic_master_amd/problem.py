@@ -281,12 +281,43 @@ def _fill_aux_from_column(g: Grid, col) -> np.ndarray:
     return aux
 
 
+def windfnx(p: ProbData, z: float, scale: float) -> float:
+    """The analytic background wind in x (fcns.f:100-116) at t = 0, as setaux
+    calls it (setaux.f:59-60, 131-132 pass the integer constant 0 for the
+    double precision t; this restatement takes t = 0):
+    amp1 cos(prop1z (z - z1pos) - phi) exp(-(z - z1Gpos)**2 / (2 z1width**2)),
+    times scale when windscale1 = 1, plus amp0."""
+    f = np.float64                      # IEEE division: z1width = 0 gives exp(-inf) = 0
+    w = f(p.amp1)
+    w = w * math.cos(f(p.prop1z) * (f(z) - p.z1pos) - f(p.omega1) * 0.0 - p.phi)
+    dz = f(z) - p.z1Gpos
+    with np.errstate(divide="ignore", invalid="ignore"):
+        arg = f(-0.5) * ((dz * dz) / (f(p.z1width) * p.z1width))
+    w = w * math.exp(arg)
+    if p.windscale1 == 1.0:
+        w = w * scale
+    return float(w + p.amp0)
+
+
+def _wind_scale(rho1: float, rho: float) -> float:
+    """setaux.f:58, 131: (aux(1,i,j,1)/aux(1,i,j,k))**0.5 with the C library
+    pow, as the reference evaluates it.  In the ghost layers below k = 1 the
+    reference reads aux(1,i,j,1) before its loop has set it; rho1 = 0 there
+    (a zeroed aux array), so scale = 0."""
+    return math.pow(rho1 / rho, 0.5)
+
+
 def setaux_isothermal(g: Grid, p: ProbData) -> np.ndarray:
-    """setaux.f:46-87 + 176-187 for loadprof = 0 without winds: isothermal
-    background rho = dens*exp(-z*gamma*grav/speed**2) (fcns.f:22-45), R = 287,
-    nu = dmolec0*rho(z_1)/rho, kappa = gamma*dtherm0*rho(z_1)/rho."""
+    """The formulas of setaux.f:46-87 + 176-187 (its loadprof = 0 branch,
+    which the reference never reaches: setaux.f:38-43 sends loadprof <= 0 to
+    the aux(1) = 1 special case) without winds: isothermal background
+    rho = dens*exp(-z*gamma*grav/speed**2) (fcns.f:22-45), R = 287,
+    nu = dmolec0*rho(z_1)/rho, kappa = gamma*dtherm0*rho(z_1)/rho.  The
+    analytic winds are restated for the reachable profile branch
+    (setaux_profile), not for this one."""
     if p.amp0 != 0.0 or p.amp1 != 0.0:
-        raise NotImplementedError("background winds (amp0/amp1) are not supported")
+        raise NotImplementedError("analytic winds on the isothermal background (an unreachable branch of "
+                                  "setaux.f) are not restated; use loadprof = 1")
     nzp = g.mz + 2 * g.mbc
     s2 = p.speed ** 2
     dens = lambda z: p.dens * math.exp(-(z * p.gamma * p.grav / s2))
@@ -324,8 +355,7 @@ _F4 = lambda v: float(np.float32(v))
 
 def setaux_profile(g: Grid, p: ProbData, prof: np.ndarray, cgs: bool = True) -> np.ndarray:
     """setaux.f:38-212 for loadprof = 1 (the MSIS-style vertical profile from
-    profile.data) without analytic winds (amp0 = amp1 = 0, the winds come
-    from the profile): per cell centre height the row with exactly that
+    profile.data): per cell centre height the row with exactly that
     height (in km, rows read in order as the reference reads them; a missing
     height is an error, setaux.f:94-103), then gas constant and gamma from
     the O/N2/O2 composition, density from the number densities (cgs = True:
@@ -333,11 +363,15 @@ def setaux_profile(g: Grid, p: ProbData, prof: np.ndarray, cgs: bool = True) -> 
     composition ratios in slots 6-10, molecular and thermal diffusivities
     (slots 11, 12; held at the 400th layer above it) and the virtual gravity
     of slot 17 from the pressure jumps (setaux.f:176-187; the lowest ghost
-    layer has none, it is 0 here).  Same operations in the same order as the
-    reference, including its single-precision literals."""
-    if p.amp0 != 0.0 or p.amp1 != 0.0:
-        raise NotImplementedError("analytic background winds (amp0/amp1) are not supported")
+    layer has none, it is 0 here).  The x and y winds come from the profile,
+    or with amp0 or amp1 set from the analytic x wind: rho*windfnx + amp0*rho
+    (amp0 enters twice, as the reference adds it in windfnx and again,
+    setaux.f:128-133) and a y slot of 0 (setaux.f:134).  Same operations in
+    the same order as the reference, including its single-precision
+    literals."""
     prof = np.asarray(prof, dtype=np.float64)
+    winds = not (p.amp0 == 0.0 and p.amp1 == 0.0)
+    rho1 = 0.0   # aux(1,i,j,1), set when the loop reaches k = 1
     nzp = g.mz + 2 * g.mbc
     cnv = 1.0e3 if cgs else 1.0e-3
     col = np.zeros((nzp, g.maux))
@@ -360,8 +394,15 @@ def setaux_profile(g: Grid, p: ProbData, prof: np.ndarray, cgs: bool = True) -> 
         rho = (dox * 16.0 + dox2 * 32.0 + dnit2 * 28.0) * cnv
         rho = rho / 6.022e23
         a[0] = rho
-        a[1] = rho * windx
-        a[2] = rho * windy
+        if k == 1:
+            rho1 = rho
+        if not winds:
+            a[1] = rho * windx
+            a[2] = rho * windy
+        else:
+            a[1] = rho * windfnx(p, g.zcell(k), _wind_scale(rho1, rho))
+            a[1] = a[1] + p.amp0 * rho
+            a[2] = 0.0
         a[3] = 0.0
         a[4] = rho * temp * roverm / gamma1 + 0.5 * (a[1] ** 2) / rho + 0.5 * (a[2] ** 2) / rho
         a[5] = dox * 16.0 * cnv / (6.022e23 * rho)

@@ -135,3 +135,10 @@ def unshuffle_planes(b, shape):
     nz, ny, nx, meqn = shape
     a = np.frombuffer(np.ascontiguousarray(b.T).tobytes(), dtype="<f8").reshape(meqn, nz, ny, nx)
     return np.ascontiguousarray(np.moveaxis(a, 0, -1))
+
+
+# analytic x winds (setprob's /wind/): off, amp0 alone, amp1 alone, both
+# with windscale1 = 1
+WIND_CASES = [dict(), dict(amp0=12.0), dict(amp1=25.0, prop1z=2e-4, z1pos=3e3, z1Gpos=40e3, z1width=9e3, phi=0.3),
+              dict(amp0=-4.0, amp1=25.0, prop1z=2e-4, z1pos=3e3, z1Gpos=40e3, z1width=9e3, phi=0.3,
+                   windscale1=1.0)]

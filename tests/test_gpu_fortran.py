@@ -12,7 +12,7 @@ import pytest
 
 from magic_master_amd import problem as P
 from oracle.oracle import COracle
-from tests.helpers import interior, perturbed, rel_err
+from tests.helpers import WIND_CASES, interior, perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "magic_master_amd", "bin")
@@ -111,9 +111,11 @@ def test_claw3ez_driver_matches_oracle(tmp_path):
     assert rel_err(qg, qo[interior(g)]) <= 1e-10
 
 
-def test_claw3ez_driver_profile_background(tmp_path):
+@pytest.mark.parametrize("wind", [0, 3])
+def test_claw3ez_driver_profile_background(tmp_path, wind):
     """loadprof = 1: the driver builds the background from profile.data (the
-    reference's own file) as setaux.f does; 10 components as in the
+    reference's own file) as setaux.f does, without and with the analytic x
+    wind (amp0 and amp1 with windscale1 = 1); 10 components as in the
     reference's deck.  Against the oracle started from problem.setaux_profile
     (itself equal bit for bit to the compiled reference's setaux)."""
     import shutil
@@ -123,6 +125,9 @@ def test_claw3ez_driver_profile_background(tmp_path):
     c.zlower, c.zupper = 0.0, 20e3
     c.nout = c.nstop = c.nstepout = 3
     p.loadprof, p.nspecadv = 1, 5
+    for k, v in WIND_CASES[wind].items():
+        setattr(p, k, v)
+    print(f"[claw3ez profile] wind case {wind}: {WIND_CASES[wind]}", flush=True)
     g = P.Grid.from_claw(c)
     aux = P.setaux_profile(g, p, P.read_profile(src))
     q = P.qinit(g, aux)

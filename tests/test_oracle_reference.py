@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import COracle, Reference
-from tests.helpers import CASES, perturbed
+from tests.helpers import CASES, WIND_CASES, perturbed
 
 pytestmark = [pytest.mark.reference,
               pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built")]
@@ -107,6 +107,31 @@ def test_setaux_profile_bitwise(both, tmp_path, monkeypatch):
         slots = list(range(12)) + [17, 18]
         assert np.array_equal(mine[..., slots], ref[..., slots])
         assert np.array_equal(mine[1:, ..., 16], ref[1:, ..., 16])   # aux(17) above the lowest ghost layer
+
+
+@pytest.mark.parametrize("wind", [1, 2, 3])
+def test_setaux_profile_winds_bitwise(both, wind, tmp_path, monkeypatch):
+    """problem.setaux_profile (loadprof = 1) with the analytic x wind in place
+    of the profile's winds (setaux.f:121-136, fcns.f:100-116: amp0 added
+    twice, y slot 0), windscale1 = 1 included (its ghost layers below k = 1
+    read aux(1,i,j,1) before the reference sets it: 0 in both, from a zeroed
+    array)."""
+    import os
+    import shutil
+    from magic_master_amd import problem as P
+    O, R = both
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks", "profile.data")
+    shutil.copy(src, tmp_path / "profile.data")
+    monkeypatch.chdir(tmp_path)
+    p = P.ProbData(loadprof=1, nspecadv=5, **WIND_CASES[wind])
+    R.setup(p, [1, 1, 1, 1, 3, 1])
+    g = P.Grid(mx=3, my=2, mz=40, meqn=10, mbc=2, maux=32, xlower=0.0, ylower=0.0, zlower=0.0,
+               dx=2e3, dy=2e3, dz=1e3)
+    ref = R.setaux(g)
+    mine = P.setaux_profile(g, p, P.read_profile(src))
+    slots = list(range(12)) + [17, 18]
+    assert np.array_equal(mine[..., slots], ref[..., slots])
+    assert np.array_equal(mine[1:, ..., 16], ref[1:, ..., 16])
 
 
 def test_src3_without_viscosity_bitwise(both):

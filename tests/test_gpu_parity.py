@@ -158,6 +158,37 @@ def test_claw3_equals_its_operators_bitwise(form):
         b.close()
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_claw3_stopped_at_step3_leaves_q_as_step3(form):
+    """A claw3 loop that stops at step3 (a negative density in one cell:
+    MAGIC_ERHO, rpn3euX.f:140-153) still ends with the second sweep stream's
+    increments added to q, as magic_step3 called alone leaves it (bit for
+    bit; the failing cell's NaNs included)."""
+    from magic_master_amd.hip import FORM_FP32P, FORM_FP64, MAGIC_ERHO, MagicError, MagicHIP
+    (mx, my, mz), mthbc = CASES[1]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=9, vel=0.05)
+    m = g.mbc
+    q = q.copy()
+    q[m + mz // 2, m + my // 2, m + mx // 2, 0] = -1.0
+    fm = FORM_FP64 if form == "fp64" else FORM_FP32P
+    print(f"[claw3 error exit] {form} grid {mx}x{my}x{mz}", flush=True)
+    a, b = MagicHIP(g, p, aux, q, form=fm), MagicHIP(g, p, aux, q, form=fm)
+    try:
+        with pytest.raises(MagicError) as e:
+            a.claw3(0.0, -1.0, [2.0, 10.0, 0, 0, 0], c.cflv, [100, 0], c.method, mthbc)
+        assert e.value.code == MAGIC_ERHO
+        b.bc3(mthbc)
+        b.b4step3()
+        b.backup()
+        with pytest.raises(MagicError) as e:
+            b.step3(2.0)
+        assert e.value.code == MAGIC_ERHO
+        np.testing.assert_array_equal(a.get_q(), b.get_q())
+    finally:
+        a.close()
+        b.close()
+
+
 def test_src3_background_varying_within_layers(oracle):
     """The FP64 source pass reads the background from planes when it varies
     within a layer (here nu, kappa and the background x momentum vary from

@@ -573,8 +573,17 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                                 for (int m = 0; m < NC; m++) LT[tix(m, db, pa + da, ci)] += acc.D[db][da][m];
                         }
                     }
-                    __syncthreads();
-                    if (phase == 0 && da == 0 && t < TA) {
+                    // With tile-axis-major lanes a wave holds every pencil of
+                    // its chunk cells, so the entries one phase's sub-steps add
+                    // to are the wave's own (cells c-1 or c of its lanes): only
+                    // the wave's program order separates its sub-steps (the
+                    // fence keeps the compiler from moving one sub-step's tile
+                    // reads above the previous one's writes; it emits nothing),
+                    // and the workgroup waits once per phase.  Pencil-major
+                    // waves share entries: a barrier after every sub-step.
+                    if (PM || da == 2) __syncthreads();
+                    else __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    if (phase == 0 && da == (PM ? 0 : 2) && t < TA) {
                         // CFLup carried into the next chunk: the last assignment
                         // among this chunk's interfaces (every lane is past part
                         // 2, which reads s_cprev, once this barrier is passed)

@@ -16,9 +16,11 @@
 //     interfaces (overlap 2); TA, CH and the lane order form the tile shape;
 //   * the 3x3 increments of each interface (its cells i-1 and i into the
 //     pencil and its 8 transverse neighbours, step3.f:189-226) are summed in
-//     an LDS tile LT[b offset][a][cell] in barrier-separated sub-steps (one per
-//     phase and a offset, so no two lanes add to the same entry at once), and
-//     the tile is flushed to q with one coalesced read-modify-write per row;
+//     an LDS tile LT[b offset][a][cell] in sub-steps (one per phase and a
+//     offset, so no two lanes add to the same entry at once; separated by the
+//     wave's program order or by barriers, see the sub-step loop), and the
+//     tile is flushed to q once per chunk (FP64: one atomic add per entry;
+//     FP32: one coalesced read-modify-write per row);
 //   * tiles are coloured 2 ways along a and 3 ways along the other transverse
 //     axis b, so no two workgroups of a launch flush to the same rows.
 // The result is deterministic.

@@ -603,8 +603,8 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 const int ci = phase == 0 ? c - 1 : c;  // cell index within the chunk's tile
                 const T sgn = phase == 0 ? T(1.0) : -T(1.0);   // +3 cqxx mixing for amdq, -3 for apdq
                 const T cc = phase == 0 ? -d : d;        // sign of the correction-wave term
-                // components M0 .. M0+GW-1; `last`: a barrier after the third sub-step
-                auto group = [&](auto m0c, auto gwc, bool last) {
+                // components M0 .. M0+GW-1; `final`: the phase's last group
+                auto group = [&](auto m0c, auto gwc, bool final) {
                     constexpr int M0 = decltype(m0c)::value, GW = decltype(gwc)::value;
                     Acc<GW, T> acc;
                     acc.zero();
@@ -670,8 +670,12 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                                     for (int m = 0; m < GW; m++) LT[tix(M0 + m, db, pa + da, ci)] += acc.D[db][da][m];
                             }
                         }
-                        if (da < 2 || last) __syncthreads();
-                        if (M0 == 0 && phase == 0 && da == 0 && t < TA) {
+                        // as in the branch above: tile-axis-major waves wait
+                        // once per phase (after its final group), pencil-major
+                        // ones after every sub-step but a group's last
+                        if (PM ? (da < 2 || (final && phase == 1)) : (da == 2 && final)) __syncthreads();
+                        else if (!PM) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                        if (phase == 0 && (PM ? (M0 == 0 && da == 0) : (final && da == 2)) && t < TA) {
                             // CFLup carried into the next chunk: the last assignment
                             // among this chunk's interfaces (every lane is past part
                             // 2, which reads s_cprev, once this barrier is passed)
@@ -684,11 +688,11 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                 };
                 if constexpr (NTR <= 3) {
                     group(IConst<0>{}, IConst<5>{}, false);
-                    group(IConst<5>{}, IConst<NTR>{}, phase == 1);
+                    group(IConst<5>{}, IConst<NTR>{}, true);
                 } else {
                     group(IConst<0>{}, IConst<5>{}, false);
                     group(IConst<5>{}, IConst<3>{}, false);
-                    group(IConst<8>{}, IConst<NTR - 3>{}, phase == 1);
+                    group(IConst<8>{}, IConst<NTR - 3>{}, true);
                 }
             }
         }

@@ -549,9 +549,24 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
                         acc.h(2, 0, -k6h * ch, Bpq); acc.h(1, 0, -k6h * ch, Bmq);
                         acc.h(2, -1, k6h * ch, Bpq); acc.h(1, -1, k6h * ch, Bmq);
                     }
+                    if constexpr (sizeof(T) == 4) {
+                        // The FP32 form forms the cell's own normal update in
+                        // FP64 with the unrounded dt/dn and rounds it once: the
+                        // second-order correction nearly cancels the fluctuation
+                        // in smooth flow, and rounding the two products apart
+                        // (with dt/dn in FP32) grew, over some 300 steps, into
+                        // an error of T' of 1e-4 of its peak in the thin upper
+                        // layers of weak plumes.
+                        const double dD = A.dtdn;
 #pragma unroll
-                    for (int m = 0; m < NC; m++)
-                        acc.c11(m, phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
+                        for (int m = 0; m < NC; m++)
+                            acc.c11(m, (T)(phase == 0 ? -dD * (double)amdq[m] - dD * (double)cqxx[m]
+                                                      : -dD * (double)apdq[m] + dD * (double)cqxx[m]));
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < NC; m++)
+                            acc.c11(m, phase == 0 ? -d * amdq[m] - d * cqxx[m] : -d * apdq[m] + d * cqxx[m]);
+                    }
                 };
                 Acc<NC, T> acc;
                 acc.zero();

@@ -33,13 +33,17 @@ sys.path.insert(0, ROOT)
 METRIC = ("Mcell-updates/s FP64 vs FP32-batched on 1 MI355X at stated error; "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
-# The step is bound by vector arithmetic, not by HBM: about 4.1 k flop per
-# interface and sweep direction (rpn3's Roe solve, entropy fix and limiters,
-# about 0.6 k; flux3's transverse and double-transverse splits and updates,
-# about 3.5 k), three directions per cell-update.  MI355X vector peaks
-# (non-matrix): FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s.
-FLOP_PER_CELL_UPDATE = 3 * 4.1e3
-VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# The step is bound by vector arithmetic, not by HBM.  Floating-point
+# operations per cell-update of the 18 sweep launches of a step, counted on
+# the full-domain workload with rocprofv3 (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}
+# of each precision, times 64 lanes, an FMA counted twice; lanes masked off
+# at grid edges and the chunks' 2-interface overlap included): FP64 form
+# 12.68 k FP64; FP32 form 9.99 k FP32 + 1.73 k FP64.  MI355X vector peak
+# (non-matrix) is 78.6 TFLOP/s for FP64, and for FP32 too without packed
+# (two-wide) FP32 instructions, which this library does not use; the 157.3
+# TFLOP/s FP32 peak counts them.
+FLOP_PER_CELL_UPDATE = {"fp64": 12.68e3, "fp32": 9.99e3 + 1.73e3}
+VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 78.6}
 # Algorithmic bytes per cell-update at meqn = 6.  Hyperbolic pass: read q,
 # write q (2 x 48 B) + aux(17), aux(19) (16 B), shared evenly by the three
 # sweep directions; a direction's launches (tile colours) each cover an equal
@@ -448,10 +452,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "bytes_per_cell_update": hyp_b + src_b,
                      "step_hbm_frac": round(value * 1e6 * (hyp_b + src_b) / (HBM_PEAK_GBS * 1e9), 5),
-                     "step_vector": {"flop_per_cell_update": FLOP_PER_CELL_UPDATE,
-                                     "achieved_tflops": round(value * 1e6 * FLOP_PER_CELL_UPDATE / 1e12, 3),
+                     "step_vector": {"flop_per_cell_update": FLOP_PER_CELL_UPDATE[a.form],
+                                     "achieved_tflops": round(value * 1e6 * FLOP_PER_CELL_UPDATE[a.form] / 1e12, 3),
                                      "peak_tflops": VECTOR_PEAK_TFLOPS[a.form],
-                                     "frac": round(value * 1e6 * FLOP_PER_CELL_UPDATE / 1e12
+                                     "frac": round(value * 1e6 * FLOP_PER_CELL_UPDATE[a.form] / 1e12
                                                    / VECTOR_PEAK_TFLOPS[a.form], 4)}},
     }
     if a.workload == "full":

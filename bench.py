@@ -365,8 +365,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-window", type=float, default=20.0)
     ap.add_argument("--ensemble-only", action="store_true",
-                    help="only run the FP32 batched ensemble and print its figures (for profiling it "
+                    help="only run the batched ensemble and print its figures (for profiling it "
                          "in a process of its own)")
+    ap.add_argument("--ensemble-form", default="fp32", choices=["fp64", "fp32"],
+                    help="form of the --ensemble-only run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (read) + 1 GiB (write), 3 times: "
                          "the known-size kernel that HBM byte counters are scaled against")
@@ -375,9 +377,10 @@ def main():
         copy_calibration()
         return
     if a.ensemble_only:
-        e = run_ensemble(a.ensemble_steps, 2, tune=not a.no_tune)
-        print(json.dumps({"fp32_batched": {k: e[k] for k in ("value", "wall", "cells", "attempts", "shapes",
-                                                             "tuned", "nscen")}}), flush=True)
+        e = run_ensemble(a.ensemble_steps, 2, tune=not a.no_tune, form=a.ensemble_form)
+        print(json.dumps({f"{a.ensemble_form}_batched": {k: e[k] for k in ("value", "wall", "cells", "attempts",
+                                                                           "shapes", "tuned", "nscen")}}),
+              flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs the MAGIC step on one GPU; --gpus {a.gpus} is not "

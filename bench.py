@@ -487,7 +487,8 @@ def main():
             "accuracy_note": ("throughput only; the FP32 form's error against the FP64 form is measured "
                               "separately (python -m magic_master_amd.report --ensemble 64 --t-end 420 "
                               "--check 64): at 420 s, w and rho' within 1.2e-5 of their peaks in all 64 "
-                              "scenarios, T' within the 1e-4 bar in 63 of 64 (worst 1.08e-4)")}
+                              "scenarios, T' within the 1e-4 bar in 63 of 64 (worst 1.08e-4); on two more seeds "
+                              "(--seed 2025, 2026) 64 and 63 of 64 (worst 1.19e-4)")}
         if not a.no_fp64_batch:
             e64 = run_ensemble(max(4, a.ensemble_steps // 2), 2, tune=not a.no_tune, form="fp64")
             out["fp64_batched"] = {

@@ -103,6 +103,50 @@ def compare_runs(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, ev
     return rows
 
 
+def layer_error(scenario: int, nscen: int = 64, seed: int = 2024, t_end: float = 420.0,
+                nmin=(64, 64, 96), nmax=(200, 200, 300)) -> dict:
+    """Where in the column the FP32 form's error sits: scenario `scenario`
+    of problem.ensemble(nscen, seed) stepped alone from rest to t_end in
+    three configurations, the FP64 form with the sweep overlap (the
+    reference), the FP64 form with the sweeps in turn (another order of
+    additions) and the FP32 form with the sweeps in turn (the batch's
+    order).  Returns {"grid", "amplitude", "peak": {field: FP64 peak},
+    "worst": {field: (k, j, i) of the FP32 form's largest error},
+    "layers": one row per interior layer k: {"k", "rho_bar", field:
+    {"fp32", "order", "value"}}, each the layer's largest |difference| (or
+    |value| of the FP64 field) over the field's peak}."""
+    from . import problem as P
+    from .hip import FORM_FP32P, FORM_FP64, MagicHIP
+    c, p, g, col = P.ensemble(nscen, seed, nmin=nmin, nmax=nmax)[scenario]
+    aux = np.ascontiguousarray(np.broadcast_to(col[:, None, None, :], g.shape_aux))
+    q0 = P.qinit(g, aux)
+    out = []
+    for form, overlap in ((FORM_FP64, True), (FORM_FP64, False), (FORM_FP32P, False)):
+        d = MagicHIP(g, p, aux, q0, form=form)
+        try:
+            d.set_sweep_overlap(overlap)
+            t, dtv = 0.0, list(c.dtv)
+            while t < t_end * (1 - 1e-12):
+                t, dtv, _, _, _ = d.claw3(t, t_end, dtv, c.cflv, [1 << 30, 0], c.method, c.mthbc)
+                dtv[0] = dtv[4]
+            out.append(fields(g, d.get_q(), aux))
+        finally:
+            d.close()
+    ref, order, f32 = out
+    rho_bar = aux[g.mbc:g.mbc + g.mz, 0, 0, AUX_RHO - 1]
+    peak, worst, per = {}, {}, {}
+    for k in ref:
+        peak[k] = float(np.abs(ref[k]).max())
+        d32, dor = np.abs(f32[k] - ref[k]), np.abs(order[k] - ref[k])
+        worst[k] = tuple(int(v) for v in np.unravel_index(np.argmax(d32), d32.shape))
+        scale = peak[k] if peak[k] > 0 else 1.0
+        per[k] = [a.reshape(g.mz, -1).max(1) / scale for a in (d32, dor, np.abs(ref[k]))]
+    layers = [{"k": kk, "rho_bar": float(rho_bar[kk]),
+               **{f: {"fp32": float(per[f][0][kk]), "order": float(per[f][1][kk]), "value": float(per[f][2][kk])}
+                  for f in ref}} for kk in range(g.mz)]
+    return {"grid": [g.mx, g.my, g.mz], "amplitude": p.amplitude, "peak": peak, "worst": worst, "layers": layers}
+
+
 def compare_forms(c, p, g: Grid, aux: np.ndarray, q: np.ndarray, t_end: float, every: float,
                   on_row=None) -> list:
     """The FP32 perturbation form against the FP64 form (compare_runs; both
@@ -191,7 +235,28 @@ def main(argv=None):
                     help="instead: the batched FP32 ensemble of this many seeded scenarios to --t-end, the "
                          "first --check of them against the FP64 form stepped alone")
     ap.add_argument("--check", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=2024, help="seed of the ensemble's scenarios")
+    ap.add_argument("--layers", type=int, default=None, metavar="S",
+                    help="instead: layer by layer, the FP32 form's error in scenario S of the ensemble "
+                         "(--ensemble scenarios, default 64; --seed) stepped alone to --t-end, next to "
+                         "the FP64 form's with the other order of additions")
     a = ap.parse_args(argv)
+    if a.layers is not None:
+        r = layer_error(a.layers, a.ensemble or 64, a.seed, a.t_end)
+        mz = r["grid"][2]
+        print(f"scenario {a.layers} of seed {a.seed}: grid {r['grid']}, amplitude {r['amplitude']:.2f} mm/10 min, "
+              f"t = {a.t_end:g} s; peaks " + ", ".join(f"{k} {v:.4g}" for k, v in r["peak"].items())
+              + "; FP32 worst at (k, j, i) " + ", ".join(f"{k} {v}" for k, v in r["worst"].items()), flush=True)
+        print("    k   rho_bar   | per field: max |FP32 - FP64| / peak, max |other order - FP64| / peak, "
+              "max |FP64| / peak", flush=True)
+        for row in r["layers"][::max(1, mz // 24)]:
+            print(f"  {row['k']:3d} {row['rho_bar']:.3e} | " + " | ".join(
+                f"{k} {row[k]['fp32']:.2e} {row[k]['order']:.2e} {row[k]['value']:.2e}" for k in ("T'", "w", "rho'")),
+                flush=True)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(r, f, indent=1)
+        return
     from .hip import FORM_FP32P, FORM_FP64
     if a.ensemble > 0:
         def show_s(r):
@@ -199,7 +264,7 @@ def main(argv=None):
                 f"{k}: peak {r[k]['peak']:.3e} max/peak {r[k]['max_rel']:.2e} rms/peak {r[k]['rms_rel']:.2e}"
                 for k in ("w", "T'", "rho'")) + f" | mass {r['mass']['rel']:+.2e} energy {r['energy']['rel']:+.2e}",
                 flush=True)
-        summary, rows = compare_ensemble(a.ensemble, a.t_end, a.check, on_row=show_s)
+        summary, rows = compare_ensemble(a.ensemble, a.t_end, a.check, seed=a.seed, on_row=show_s)
         print("ensemble: " + json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in summary.items()}),
               flush=True)
         if a.json:

@@ -7,7 +7,9 @@ workloads:
 * compare_ensemble: a small seeded ensemble advanced by the batched driver
   in the FP32 form, each scenario against the FP64 form stepped alone to
   the same model time, within the FP32 bar (1e-4 of each field's peak, for
-  the fields whose peak is above round-off level).
+  the fields whose peak is above round-off level);
+* layer_error: the same comparison of one scenario stepped alone, layer by
+  layer.
 """
 import pytest
 
@@ -43,3 +45,23 @@ def test_compare_ensemble_small():
                   flush=True)
             if r[k]["peak"] >= floor:
                 assert r[k]["max_rel"] <= 1e-4, (r["scenario"], k, r[k])
+
+
+def test_layer_error_small():
+    """layer_error: one row per interior layer, the other FP64 order within
+    1e-9 of each field's peak in every layer, the FP32 form within 1e-4 of each live
+    field's peak, and each field's worst cell inside the grid."""
+    print("[report layers] scenario 1 of 2 (seed 7, 24-32 x 24-32 x 32-40 cells) to 150 s", flush=True)
+    r = report.layer_error(1, nscen=2, seed=7, t_end=150.0, nmin=(24, 24, 32), nmax=(32, 32, 40))
+    mx, my, mz = r["grid"]
+    assert len(r["layers"]) == mz and [row["k"] for row in r["layers"]] == list(range(mz))
+    rho = [row["rho_bar"] for row in r["layers"]]
+    assert all(a > b > 0 for a, b in zip(rho, rho[1:]))       # hydrostatic column: density falls with height
+    for k, floor in report.PEAK_FLOOR.items():
+        kk, jj, ii = r["worst"][k]
+        assert 0 <= kk < mz and 0 <= jj < my and 0 <= ii < mx
+        if r["peak"][k] > 0:
+            assert max(row[k]["value"] for row in r["layers"]) == pytest.approx(1.0)
+        assert max(row[k]["order"] for row in r["layers"]) <= 1e-9, k
+        if r["peak"][k] >= floor:
+            assert max(row[k]["fp32"] for row in r["layers"]) <= 1e-4, k

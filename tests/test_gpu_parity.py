@@ -119,6 +119,41 @@ def test_claw3_steps(case, oracle):
     assert rel_err(qg, qo, interior(g)) <= 1e-9
 
 
+THIN = [((1, 6, 5), [1, 1, 3, 3, 3, 1]), ((6, 1, 5), [3, 3, 1, 1, 3, 1]), ((2, 3, 2), [1, 1, 1, 1, 3, 1])]
+
+
+@pytest.mark.parametrize("case", range(len(THIN)))
+def test_claw3_thin_grids(case, oracle):
+    """Grids one or two cells thick along an axis (extrapolation there, since
+    a wall or periodic ghost of a one-cell axis reads the other side's
+    ghosts): 5 claw3 calls (the first with a rejected attempt) against the
+    oracle, the dt sequence within 1e-12 and q within 1e-9."""
+    (mx, my, mz), mthbc = THIN[case]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=3, vel=0.05)
+    print(f"[claw3 thin] grid {mx}x{my}x{mz}", flush=True)
+    d = _dev(g, p, aux, q)
+    try:
+        qo = q.copy()
+        dto = [50.0, 10.0, 0, 0, 0]
+        dtg = list(dto)
+        t_o = t_g = 0.0
+        for n in range(5):
+            st, t_o, dto, _, _, log_o = oracle.claw3(g, p, qo, aux, t_o, t_o - 1.0, dto, c.cflv,
+                                                     [100, 0], c.method, mthbc)
+            assert st == 0
+            t_g, dtg, _, _, log_g = d.claw3(t_g, t_g - 1.0, dtg, c.cflv, [100, 0], c.method, mthbc)
+            assert [acc for _, _, acc in log_o] == list(log_g.accepted), (n, log_o, log_g)
+            for (a, _, _), b in zip(log_o, log_g.dt):
+                assert abs(a - b) <= 1e-12 * abs(a)
+            dto[0] = dto[4]
+            dtg[0] = dtg[4]
+        qg = d.get_q()
+    finally:
+        d.close()
+    assert abs(t_g - t_o) <= 1e-12 * t_o
+    assert rel_err(qg, qo, interior(g)) <= 1e-9
+
+
 @pytest.mark.parametrize("form", ["fp64", "fp32"])
 def test_claw3_equals_its_operators_bitwise(form):
     """claw3 adds the second sweep stream's increments to q inside the source

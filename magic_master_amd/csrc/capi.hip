@@ -169,6 +169,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
              hipStreamSynchronize(c->stream) == hipSuccess;
         if (!ok) {
             magic_destroy(c);
+            (void)hipGetLastError();   // reported here; not left for the next launch check
             return MAGIC_ENOMEM;
         }
         *out = c;
@@ -194,6 +195,7 @@ extern "C" int magic_create_form(int mx, int my, int mz, int meqn, int mbc, int 
     }
     if (!ok) {
         magic_destroy(c);
+        (void)hipGetLastError();   // reported here; not left for the next launch check
         return MAGIC_ENOMEM;
     }
     *out = c;

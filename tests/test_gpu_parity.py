@@ -154,6 +154,40 @@ def test_claw3_thin_grids(case, oracle):
     assert rel_err(qg, qo, interior(g)) <= 1e-9
 
 
+@pytest.mark.parametrize("form", [0, 1])
+def test_create_beyond_device_memory_fails_cleanly(form, oracle):
+    """A grid of 4000^3 cells (terabytes) in either form: magic_create_form
+    returns MAGIC_ENOMEM with no context, and the library keeps working: a
+    small grid created next takes two claw3 calls that match the oracle."""
+    import ctypes as C
+    from magic_master_amd.hip import load_library
+    L = load_library()
+    print(f"[create too large] form {form}", flush=True)
+    h = C.c_void_p()
+    st = L.magic_create_form(4000, 4000, 4000, 6, 2, 19, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, form, C.byref(h))
+    assert st == 7 and not h.value, (st, h.value)   # MAGIC_ENOMEM
+    (mx, my, mz), mthbc = CASES[0]
+    c, p, g, aux, q = perturbed(mx, my, mz, mthbc, seed=5, vel=0.05)
+    d = _dev(g, p, aux, q)
+    try:
+        qo = q.copy()
+        dto = [5.0, 10.0, 0, 0, 0]
+        dtg = list(dto)
+        t_o = t_g = 0.0
+        for n in range(2):
+            st, t_o, dto, _, _, _ = oracle.claw3(g, p, qo, aux, t_o, t_o - 1.0, dto, c.cflv, [100, 0],
+                                                 c.method, mthbc)
+            assert st == 0
+            t_g, dtg, _, _, _ = d.claw3(t_g, t_g - 1.0, dtg, c.cflv, [100, 0], c.method, mthbc)
+            dto[0] = dto[4]
+            dtg[0] = dtg[4]
+        qg = d.get_q()
+    finally:
+        d.close()
+    assert abs(t_g - t_o) <= 1e-12 * t_o
+    assert rel_err(qg, qo, interior(g)) <= 1e-9
+
+
 @pytest.mark.parametrize("form", ["fp64", "fp32"])
 def test_claw3_equals_its_operators_bitwise(form):
     """claw3 adds the second sweep stream's increments to q inside the source

@@ -496,15 +496,17 @@ __device__ __forceinline__ void tile_sweep_body(const SweepArgsT<T> &A, int a_is
         // each increment to the tile with shared-memory atomics instead, with
         // or without a two-waves-per-SIMD register budget, measured 3-5 %
         // slower in FP64 and 7x slower in FP32 on the full-domain workload.)
-        // In FP64 with two or more tracers the components go in groups, the
-        // Euler components and then the tracers, at most three at a time,
-        // with one accumulator alive at a time (at five components it fits
-        // the registers; at ten it spilled a kilobyte per lane, and the
-        // full-domain step with five tracers ran 2.8x slower).  Groups add to
-        // disjoint tile entries, so a group's first sub-step needs no barrier
-        // after the previous group's last one.  FP32 accumulators fit
-        // without groups, and the extra barriers cost 3 % there.
-        if constexpr (NTR <= 1 || sizeof(T) == 4) {
+        // In FP64 with tracers the components go in groups, the Euler
+        // components and then the tracers, at most three at a time, with one
+        // accumulator alive at a time (at ten components it spilled a
+        // kilobyte per lane, and the full-domain step with five tracers ran
+        // 2.8x slower; with one tracer the groups hold 19 fewer AGPRs, and
+        // the full-domain step runs 4 % faster, bit for bit the same).
+        // Groups add to disjoint tile entries, so a group's first sub-step
+        // needs no barrier after the previous group's last one.  FP32
+        // accumulators fit without groups, and the extra barriers cost 3 %
+        // there.
+        if constexpr (NTR == 0 || sizeof(T) == 4) {
             for (int phase = 0; phase < 2; phase++) {
                 // phase 0 (L): interface I -> cell I-1; phase 1 (R): -> cell I
                 const bool on = act2 && (phase == 0 ? I >= 2 : I <= n);

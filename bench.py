@@ -38,11 +38,11 @@ HBM_PEAK_GBS = 8000.0
 # the full-domain workload with rocprofv3 (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}
 # of each precision, times 64 lanes, an FMA counted twice; lanes masked off
 # at grid edges and the chunks' 2-interface overlap included): FP64 form
-# 12.68 k FP64; FP32 form 9.99 k FP32 + 1.73 k FP64.  MI355X vector peak
+# 12.71 k FP64; FP32 form 9.99 k FP32 + 1.73 k FP64.  MI355X vector peak
 # (non-matrix) is 78.6 TFLOP/s for FP64, and for FP32 too without packed
 # (two-wide) FP32 instructions, which this library does not use; the 157.3
 # TFLOP/s FP32 peak counts them.
-FLOP_PER_CELL_UPDATE = {"fp64": 12.68e3, "fp32": 9.99e3 + 1.73e3}
+FLOP_PER_CELL_UPDATE = {"fp64": 12.71e3, "fp32": 9.99e3 + 1.73e3}
 VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 78.6}
 # Algorithmic bytes per cell-update at meqn = 6.  Hyperbolic pass: read q,
 # write q (2 x 48 B) + aux(17), aux(19) (16 B), shared evenly by the three
